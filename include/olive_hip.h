@@ -1,0 +1,271 @@
+/*
+ * olive_hip.h -- C ABI of libolive_hip.so, the MI355X (gfx950) frontier engine.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * integer status (OLV_OK == 0) and never throws.  On failure the message is
+ * available from olv_last_error() (thread-local).  Handles are not
+ * thread-safe: one olv_graph is driven from one host thread at a time.
+ *
+ * Device pointers are the default for algorithm inputs and outputs (the
+ * caller owns them, e.g. torch tensors or olv_malloc'd buffers).  Passing
+ * OLV_PTR_HOST in `flags` makes the library stage host buffers itself.
+ *
+ * The reference engine (Oliver<V,E,A>, a header-only CUDA template API) has
+ * no FFI; each declaration below names the reference code it replaces.
+ * The functor-generic part of that API (user gather/reduce/cond/update
+ * functors) lives in the C++ headers under olive_amd/include and calls the
+ * type-independent entry points declared here.
+ */
+#ifndef OLIVE_HIP_H
+#define OLIVE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+  OLV_OK = 0,
+  OLV_EINVAL = 1,      /* bad argument (null pointer, out-of-range id, size) */
+  OLV_ENOMEM = 2,      /* device or host allocation failed                   */
+  OLV_EHIP = 3,        /* a HIP runtime call failed                          */
+  OLV_ERANGE = 4,      /* size beyond what the ABI supports (e.g. E >= 2^32) */
+  OLV_EUNSUPPORTED = 5,/* feature not available for this graph / device      */
+  OLV_ENOCONV = 6,     /* iterative solve hit its iteration cap              */
+  OLV_EINTERNAL = 7    /* internal consistency check failed                  */
+};
+
+/* ---- flags ------------------------------------------------------------- */
+#define OLV_PTR_HOST        0x0001u  /* inputs/outputs are host pointers            */
+#define OLV_NO_DIRECTION_OPT 0x0002u /* BFS: top-down (push) only                    */
+#define OLV_FORCE_BOTTOM_UP  0x0004u /* BFS: bottom-up (pull) only, needs transpose  */
+#define OLV_HOST_LOOP        0x0008u /* drive the level loop from the host (one
+                                        launch group per level) instead of the
+                                        persistent cooperative kernel            */
+#define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
+                                        with hipEvents (olv_stats.kernel_ms)     */
+#define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
+
+/* weight dtypes for olv_graph_create */
+enum { OLV_W_NONE = 0, OLV_W_I32 = 1, OLV_W_F32 = 2 };
+
+typedef struct olv_graph olv_graph;
+
+/* Per-call statistics (all optional: pass NULL). */
+typedef struct olv_stats {
+  double   elapsed_ms;        /* device time of the algorithm, hipEvents      */
+  uint64_t edges_traversed;   /* E_r: sum of out-degrees of reached vertices  */
+  uint64_t vertices_reached;  /* V_r                                          */
+  uint64_t edges_inspected;   /* edges actually read (bottom-up may skip)     */
+  uint32_t iterations;        /* BFS/SSSP levels, PageRank iterations         */
+  uint32_t bottom_up_levels;  /* BFS: levels run bottom-up                    */
+  double   residual;          /* PageRank: last L1 delta                      */
+  double   kernel_ms;         /* OLV_PROFILE: summed hipEvent time of the
+                                 dominant kernel's launches                   */
+  uint32_t kernel_launches;   /* OLV_PROFILE: number of those launches        */
+  uint32_t reserved;
+} olv_stats;
+
+/* ---- runtime ----------------------------------------------------------- */
+const char *olv_last_error(void);
+int olv_version(void);                    /* 100*major + minor               */
+int olv_init(int device);                 /* select device, query CU count   */
+int olv_set_stream(void *hip_stream);     /* NULL = the null stream          */
+void *olv_get_stream(void);
+int olv_synchronize(void);                /* wait for the current stream     */
+int olv_device_info(int *cu_count, int *warp_size, size_t *total_mem);
+
+/* ---- device memory (replaces GRD<T>: grd.h:76-164) --------------------- */
+int olv_malloc(void **dptr, size_t bytes);                 /* GRD::reserve  */
+int olv_free(void *dptr);                                  /* GRD::del      */
+int olv_host_malloc(void **hptr, size_t bytes);            /* pinned staging */
+int olv_host_free(void *hptr);
+int olv_memcpy_h2d(void *dst, const void *src, size_t bytes); /* GRD::cache  */
+int olv_memcpy_d2h(void *dst, const void *src, size_t bytes); /* GRD::persist*/
+int olv_memcpy_d2d(void *dst, const void *src, size_t bytes);
+/* Fill `count` elements of 1/4/8 bytes with `value` on the device: replaces
+ * the host loop + full H2D copy of GRD::allTo (grd.h:90-97). */
+int olv_fill(void *dptr, uint64_t value, size_t count, int elem_bytes);
+
+/* ---- frontier (replaces VertexSubset: vertexSubset.h:38-156, bitmap.h) -- *
+ * A flag frontier is a bitmap of ceil(n/32) uint32 words plus an optional
+ * "touched" queue; a queue frontier is uint32 ids plus a device count.    */
+int olv_bitmap_words(uint32_t n, uint32_t *words);
+/* Compact a bitmap into a queue: wave64 ballot/popcount + LDS scan.
+ * count_dev (device, 1 uint32) receives the number of ids. */
+int olv_bitmap_to_queue(const uint32_t *bitmap, uint32_t n, uint32_t *queue,
+                        uint32_t *count_dev);
+/* Set bits of the ids in queue[0..*count_dev) (bitmap cleared first if
+ * clear != 0). */
+int olv_queue_to_bitmap(const uint32_t *queue, const uint32_t *count_dev,
+                        uint32_t *bitmap, uint32_t n, int clear);
+/* Split queue[0..*count_dev) by out-degree into the graph's degree buckets
+ * (thread / wave / workgroup per vertex).  bucket_queues holds 3 arrays of
+ * capacity n; bucket_counts_dev holds 3 uint32 (zeroed here).
+ * Replaces the fixed GroupSize of edgeFilterKernel (oliverKernel.h:63-66). */
+int olv_frontier_bin(const olv_graph *g, const uint32_t *queue,
+                     const uint32_t *count_dev, uint32_t *const bucket_queues[3],
+                     uint32_t *bucket_counts_dev);
+/* Degree-bucket thresholds used by olv_frontier_bin for this graph. */
+int olv_graph_buckets(const olv_graph *g, uint32_t *thread_max,
+                      uint32_t *wave_max);
+
+/* ---- graph (replaces Oliver::readGraph oliver.h:190-202 and the CSR of
+ *      csrGraph.h:54-84) ------------------------------------------------- */
+/* row_ptr[n+1] and col[m] (uint32, CSR, out-edges); weights[m] of
+ * weight_dtype or NULL.  Pointers are host pointers if OLV_PTR_HOST is set,
+ * else device pointers (copied).  Zero-edge graphs are accepted (the
+ * reference asserts on them, grd.h:77). */
+int olv_graph_create(olv_graph **out, uint32_t n, uint32_t m,
+                     const uint32_t *row_ptr, const uint32_t *col,
+                     const void *weights, int weight_dtype, unsigned flags);
+int olv_graph_destroy(olv_graph *g);
+/* Build the in-edge CSR (needed by bottom-up BFS, batched BFS, PageRank). */
+int olv_graph_build_transpose(olv_graph *g);
+int olv_graph_counts(const olv_graph *g, uint32_t *n, uint32_t *m);
+/* Device pointers of the resident arrays (NULL where absent).  in_eid maps an
+ * in-edge slot to its out-edge id. */
+int olv_graph_device_arrays(const olv_graph *g, const uint32_t **row_ptr,
+                            const uint32_t **col, const void **weights,
+                            const uint32_t **in_row_ptr,
+                            const uint32_t **in_col, const uint32_t **in_eid);
+/* Copy the in-edge CSR to the host (for tests). */
+int olv_graph_transpose_to_host(const olv_graph *g, uint32_t *in_row_ptr,
+                                uint32_t *in_col, uint32_t *in_eid);
+
+/* ---- algorithms -------------------------------------------------------- */
+/* BFS levels from src, INF = 0x7fffffff for unreached.  Replaces the BFS.cu
+ * level loop (BFS.cu:124-148) over edgeFilter/vertexFilter
+ * (oliver.h:56-113, oliverKernel.h:45-141) run to convergence. */
+int olv_bfs(const olv_graph *g, uint32_t src, int32_t *levels, unsigned flags,
+            olv_stats *st);
+
+/* SSSP distances over int32 weights (graph weights, or `weights` if not
+ * NULL), INF = 0x7fffffff.  Frontier relaxation as SSSP.cu:146-163 with
+ * cond(v, acc[v]) (SSSP.cu:72-75). */
+int olv_sssp(const olv_graph *g, uint32_t src, const int32_t *weights,
+             int32_t *dist, unsigned flags, olv_stats *st);
+
+/* Unweighted PageRank in fp32 with fp32 accumulation, testPageRank.cu:57-104
+ * semantics: r0 = 1/n, r <- d*sum_{u->v} r_u/outdeg(u) + (1-d)/n, no dangling
+ * redistribution, stop after max_iters iterations or when the L1 delta drops
+ * below epsilon.  Replaces PageRank.cu:122-133 (edgeMap/vertexMap/
+ * vertexReduce, oliver.h:121-183). */
+int olv_pagerank_f32(const olv_graph *g, float damping, uint32_t max_iters,
+                     double epsilon, float *ranks, unsigned flags,
+                     olv_stats *st);
+
+/* Weighted, teleported PageRank fixed point (no reference counterpart;
+ * reduces to olv_pagerank_f32 when w = 1/outdeg, t = 1/n):
+ *   r = d * P^T r + (1-d) t,  P[u][v] = w_uv (w indexed like col),
+ * iterated from r0 = t until ||r_k+1 - r_k||_1 < tol. */
+int olv_pagerank_weighted_f32(const olv_graph *g, const float *w,
+                              const float *t, float damping, float tol,
+                              uint32_t max_iters, float *ranks, unsigned flags,
+                              olv_stats *st);
+/* Vector-Jacobian product of the fixed point above for L = <gbar, r>:
+ *   lambda = gbar + d * P lambda          (adjoint solve on the out-CSR)
+ *   grad_t = (1-d) lambda,   grad_w[e=(u,v)] = d * r_u * lambda_v.
+ * Stops when ||lambda_k+1 - lambda_k||_1 < tol * ||gbar||_1.
+ * grad_w / grad_t / lambda_out may be NULL. */
+int olv_pagerank_vjp_f32(const olv_graph *g, const float *w,
+                         const float *ranks, const float *gbar, float damping,
+                         float tol, uint32_t max_iters, float *grad_w,
+                         float *grad_t, float *lambda_out, unsigned flags,
+                         olv_stats *st);
+
+/* Source lists (`srcs`) of the multi-source entry points below are always
+ * host arrays; OLV_PTR_HOST refers to the level / distance outputs. */
+
+/* BFS from a set of sources: levels[v] = hops to the nearest source.  The
+ * frontier of level 0 is the whole set (one persistent launch). */
+int olv_bfs_multisource(const olv_graph *g, const uint32_t *srcs, uint32_t k,
+                        int32_t *levels, unsigned flags, olv_stats *st);
+/* k-source BFS in one bit-parallel sweep per group of 256 sources (no
+ * reference counterpart; equal to k runs of olv_bfs).  levels is k x n, row s
+ * = levels from srcs[s]. */
+int olv_bfs_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
+                    int32_t *levels, unsigned flags, olv_stats *st);
+/* k-source SSSP as k concurrent instances in one persistent launch (equal to
+ * k runs of olv_sssp); dist is k x n. */
+int olv_sssp_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
+                     const int32_t *weights, int32_t *dist, unsigned flags,
+                     olv_stats *st);
+/* Block-diagonal union of k graphs as a new handle; vertex_offsets (host,
+ * k+1 entries, may be NULL) receives where each graph's ids start. */
+int olv_graph_union(olv_graph *const *gs, uint32_t k, olv_graph **out,
+                    uint32_t *vertex_offsets);
+/* BFS on k independent graphs in one launch: graph i from srcs[i] into
+ * levels[i] (n_i entries each; levels is a host array of k pointers). */
+int olv_bfs_multigraph(olv_graph *const *gs, uint32_t k, const uint32_t *srcs,
+                       int32_t *const *levels, unsigned flags, olv_stats *st);
+
+/* ---- hipGraph plans ---------------------------------------------------- */
+/* Capture a whole traversal (state reset, seeding, persistent kernel) once;
+ * olv_plan_launch replays it on the current stream.  `levels` (device) is
+ * bound at capture time.  A plan shares the graph's work buffers: do not
+ * run two plans of one graph concurrently. */
+typedef struct olv_plan olv_plan;
+int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k,
+                 int32_t *levels, unsigned flags, olv_plan **out);
+int olv_plan_bfs_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
+                         int32_t *levels, unsigned flags, olv_plan **out);
+/* st (optional) waits for the replay and reports levels and time. */
+int olv_plan_launch(olv_plan *p, olv_stats *st);
+int olv_plan_destroy(olv_plan *p);
+
+/* ---- autotuner --------------------------------------------------------- */
+/* Time the plain BFS over candidate degree-bucket thresholds (thread / wave /
+ * workgroup per vertex) and direction-switch parameters on this graph and
+ * keep the fastest in the handle.  Results are returned through the
+ * out-pointers (any may be NULL). */
+int olv_autotune(olv_graph *g, uint32_t *thread_max, uint32_t *wave_max,
+                 uint32_t *block_threads);
+int olv_set_buckets(olv_graph *g, uint32_t thread_max, uint32_t wave_max,
+                    uint32_t block_threads);
+/* Direction switch (Beamer): top-down -> bottom-up when the next frontier's
+ * out-edges exceed unexplored edges / alpha; back when the frontier has
+ * fewer than n / beta vertices. */
+int olv_set_direction(olv_graph *g, float alpha, float beta);
+int olv_get_tuning(const olv_graph *g, uint32_t *wave_thr, uint32_t *block_thr,
+                   float *alpha, float *beta);
+
+/* ---- seeded graph generators (host memory, deterministic for a seed) ---- */
+typedef struct olv_host_csr {
+  uint32_t n;
+  uint32_t m;
+  uint32_t *row_ptr;  /* n+1 */
+  uint32_t *col;      /* m, ascending within each row, duplicates kept */
+  void *weights;      /* m or NULL */
+  int weight_dtype;
+} olv_host_csr;
+
+/* R x C directed grid: id = C*row + col, edges to the 4 neighbours. */
+int olv_gen_grid(uint32_t rows, uint32_t cols, olv_host_csr *out);
+/* RMAT(scale, edge_factor, a, b, c) without noise or permutation; drops
+ * self-loops when drop_self_loops != 0; duplicates kept. */
+int olv_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b,
+                 double c, uint64_t seed, int drop_self_loops,
+                 olv_host_csr *out);
+/* n vertices, m edges, endpoints uniform (self-loops kept). */
+int olv_gen_uniform(uint32_t n, uint32_t m, uint64_t seed, olv_host_csr *out);
+/* Attach int32 weights uniform in [lo, hi] (by CSR position). */
+int olv_gen_weights_i32(olv_host_csr *g, int32_t lo, int32_t hi,
+                        uint64_t seed);
+/* Attach fp32 weights uniform in (0, 1], normalised to sum 1 per source. */
+int olv_gen_weights_f32_normalized(olv_host_csr *g, uint64_t seed);
+/* Flat-Dirichlet vector of length n (normalised Exp(1) draws). */
+int olv_gen_dirichlet(uint32_t n, uint64_t seed, float *out);
+/* Scale of graph i of a batch, uniform in [lo, hi]. */
+uint32_t olv_gen_batch_scale(uint64_t seed, uint32_t i, uint32_t lo,
+                             uint32_t hi);
+uint64_t olv_gen_batch_seed(uint64_t seed, uint32_t i);
+void olv_host_csr_free(olv_host_csr *g);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OLIVE_HIP_H */

@@ -1,0 +1,244 @@
+// Frontier steps shared by the BFS/SSSP drivers (host-driven, persistent and
+// multi-graph).  Each step is a __device__ function over a virtual block
+// range [vb, vb + nvb) so the same code runs as a standalone kernel per level
+// or inside one persistent cooperative kernel.
+//
+// Reference semantics (BFS.cu:38-79 over oliver.h:56-113): a vertex first
+// reached while expanding level L gets level L+1; INF = 0x7fffffff.  Levels
+// are unique, so push (top-down) and pull (bottom-up) steps may be mixed per
+// level without changing any output.
+#pragma once
+#include "device.hpp"
+
+namespace olv {
+
+struct BfsCtr {
+  unsigned long long mf;         // sum of out-degrees of vertices claimed
+  unsigned long long inspected;  // edges read
+  uint32_t nf;                   // vertices claimed
+  uint32_t qsize;                // length of the queue the TD step reads
+  uint32_t pad[2];
+};
+
+struct BfsArgs {
+  const uint32_t *row_ptr;
+  const uint32_t *col;
+  const uint32_t *in_row_ptr;
+  const uint32_t *in_col;
+  int32_t *levels;
+  uint32_t *visited;
+  uint32_t *queue;
+  uint32_t n;
+  uint32_t wave_thr;   // degree >= wave_thr: one wave per vertex
+  uint32_t block_thr;  // degree >= block_thr: whole workgroup per vertex
+};
+
+// Workload-balanced push expansion of queue[0, qsize).  Each out-list is
+// read by the whole workgroup (degree >= block_thr), by one wave
+// (>= wave_thr), or -- for the remaining short lists -- by a block-wide scan
+// that deals the edges of many vertices to consecutive lanes, so a wave
+// always reads 64 consecutive column entries.
+//
+// Visitor: `uint32_t row(uint32_t q)` maps a queue entry to its CSR row,
+// `P load(uint32_t q)` gives the per-source payload read once per frontier
+// entry, `void visit(uint32_t e, P payload)` handles edge e.
+template <int BLOCK, class Visitor>
+__device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint32_t *row_ptr, uint32_t wave_thr,
+                            uint32_t block_thr, uint32_t vb, uint32_t nvb, Visitor &vis,
+                            unsigned long long &insp) {
+  using P = decltype(vis.load(0u));
+  __shared__ uint32_t s_owner, s_beg, s_end;
+  __shared__ P s_own_payload;
+  __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+  __shared__ uint32_t s_edges[BLOCK];
+  __shared__ P s_payload[BLOCK];
+  const uint32_t lane = lane_id();
+  for (uint32_t base = vb * BLOCK; base < qsize; base += nvb * BLOCK) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t beg = 0, end = 0;
+    P pl{};
+    if (i < qsize) {
+      const uint32_t q = queue[i];
+      const uint32_t u = vis.row(q);
+      beg = row_ptr[u];
+      end = row_ptr[u + 1];
+      pl = vis.load(q);
+    }
+    insp += end - beg;
+    // workgroup per vertex
+    for (;;) {
+      if (threadIdx.x == 0) s_owner = 0xffffffffu;
+      __syncthreads();
+      if (end - beg >= block_thr) s_owner = threadIdx.x;
+      __syncthreads();
+      const uint32_t owner = s_owner;
+      if (owner == 0xffffffffu) break;
+      if (threadIdx.x == owner) {
+        s_beg = beg;
+        s_end = end;
+        s_own_payload = pl;
+        beg = end;
+      }
+      __syncthreads();
+      const uint32_t eb = s_beg, ee = s_end;
+      const P opl = s_own_payload;
+      for (uint32_t e = eb + threadIdx.x; e < ee; e += BLOCK) vis.visit(e, opl);
+      __syncthreads();
+    }
+    // wave per vertex
+    for (;;) {
+      const uint64_t mask = ballot(end - beg >= wave_thr);
+      if (!mask) break;
+      const uint32_t leader = __ffsll((unsigned long long)mask) - 1;
+      const uint32_t eb = __shfl(beg, leader, kWave), ee = __shfl(end, leader, kWave);
+      const P lpl = __shfl(pl, leader, kWave);
+      if (lane == leader) beg = end;
+      for (uint32_t e = eb + lane; e < ee; e += kWave) vis.visit(e, lpl);
+    }
+    // short lists: scan-based edge dealing
+    const uint32_t rem = end - beg;
+    uint32_t total;
+    const uint32_t off = block_exclusive_scan<BLOCK>(rem, s_scan, &total);
+    for (uint32_t done = 0; done < total; done += BLOCK) {
+      const uint32_t lo = off > done ? off : done;
+      const uint32_t hi = (off + rem) < (done + BLOCK) ? (off + rem) : (done + BLOCK);
+      for (uint32_t k = lo; k < hi; ++k) {
+        s_edges[k - done] = beg + (k - off);
+        s_payload[k - done] = pl;
+      }
+      __syncthreads();
+      if (done + threadIdx.x < total) vis.visit(s_edges[threadIdx.x], s_payload[threadIdx.x]);
+      __syncthreads();
+    }
+  }
+}
+
+// BFS claim: first visitor of an unvisited vertex sets its level and its bit
+// in the next-frontier bitmap.
+struct BfsVisitor {
+  const BfsArgs &a;
+  int32_t nlevel;
+  uint32_t *next_bm;
+  uint32_t nf = 0;
+  unsigned long long mf = 0;
+  __device__ BfsVisitor(const BfsArgs &aa, int32_t l, uint32_t *nb) : a(aa), nlevel(l), next_bm(nb) {}
+  __device__ __forceinline__ uint32_t row(uint32_t q) const { return q; }
+  __device__ __forceinline__ uint32_t load(uint32_t) const { return 0u; }
+  __device__ __forceinline__ void visit(uint32_t e, uint32_t) {
+    const uint32_t v = a.col[e];
+    const uint32_t w = v >> 5, bit = 1u << (v & 31);
+    if (a.visited[w] & bit) return;
+    const uint32_t old = atomicOr(&a.visited[w], bit);
+    if (old & bit) return;
+    a.levels[v] = nlevel;
+    atomicOr(&next_bm[w], bit);
+    nf += 1;
+    mf += a.row_ptr[v + 1] - a.row_ptr[v];
+  }
+};
+
+template <int BLOCK>
+__device__ void td_step(const BfsArgs &a, uint32_t qsize, int32_t nlevel, uint32_t *next_bm, uint32_t vb,
+                        uint32_t nvb, uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
+  BfsVisitor vis(a, nlevel, next_bm);
+  push_expand<BLOCK>(a.queue, qsize, a.row_ptr, a.wave_thr, a.block_thr, vb, nvb, vis, insp);
+  nf += vis.nf;
+  mf += vis.mf;
+}
+
+// Bottom-up step: every unvisited vertex scans its in-list for a parent in
+// the current frontier bitmap and stops at the first one.  A wave owns 64
+// consecutive vertices (two bitmap words), so visited/next words are written
+// without atomics.
+template <int BLOCK>
+__device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlevel, uint32_t *next_bm,
+                        uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf,
+                        unsigned long long &insp) {
+  const uint32_t lane = lane_id();
+  const uint32_t waves = nvb * (BLOCK / kWave);
+  const uint32_t gw = vb * (BLOCK / kWave) + threadIdx.x / kWave;
+  for (uint64_t base = (uint64_t)gw * kWave; base < a.n; base += (uint64_t)waves * kWave) {
+    const uint32_t v = (uint32_t)base + lane;
+    const bool inb = v < a.n;
+    const uint32_t w = v >> 5;
+    const uint32_t vis = inb ? a.visited[w] : 0xffffffffu;
+    const bool todo = inb && !((vis >> (v & 31)) & 1u);
+    if (!ballot(todo)) continue;
+    bool found = false;
+    if (todo) {
+      uint32_t e = a.in_row_ptr[v];
+      const uint32_t ee = a.in_row_ptr[v + 1];
+      uint32_t k = 0;
+      for (; e < ee; ++e) {
+        ++k;
+        const uint32_t u = a.in_col[e];
+        if (bm_test(front_bm, u)) {
+          found = true;
+          break;
+        }
+      }
+      insp += k;
+    }
+    const uint64_t fm = ballot(found);
+    if (found) {
+      a.levels[v] = nlevel;
+      nf += 1;
+      mf += a.row_ptr[v + 1] - a.row_ptr[v];
+    }
+    const uint32_t mine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
+    if ((lane == 0 || lane == 32) && mine) {
+      a.visited[w] = vis | mine;
+      next_bm[w] = mine;
+    }
+  }
+}
+
+// Bitmap -> queue.  One lane per word, block scan of popcounts, one global
+// atomic per block for the output base (ids ascend within a block's range).
+template <int BLOCK>
+__device__ void compact_step(const uint32_t *bm, uint32_t nwords, uint32_t *queue, uint32_t *count, uint32_t vb,
+                             uint32_t nvb) {
+  __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+  __shared__ uint32_t s_base;
+  for (uint32_t w0 = vb * BLOCK; w0 < nwords; w0 += nvb * BLOCK) {
+    const uint32_t w = w0 + threadIdx.x;
+    uint32_t word = w < nwords ? bm[w] : 0u;
+    uint32_t total;
+    const uint32_t off = block_exclusive_scan<BLOCK>((uint32_t)__popc(word), s_scan, &total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(count, total) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + off;
+    while (word) {
+      const uint32_t b = __ffs(word) - 1;
+      queue[pos++] = w * 32u + b;
+      word &= word - 1;
+    }
+    __syncthreads();
+  }
+}
+
+template <int BLOCK>
+__device__ __forceinline__ void flush_counters(BfsCtr *c, uint32_t nf, unsigned long long mf,
+                                               unsigned long long insp) {
+  __shared__ unsigned long long s_red[BLOCK / kWave];
+  const unsigned long long tnf = block_sum<BLOCK, unsigned long long>((unsigned long long)nf, s_red);
+  const unsigned long long tmf = block_sum<BLOCK, unsigned long long>(mf, s_red);
+  const unsigned long long tin = block_sum<BLOCK, unsigned long long>(insp, s_red);
+  if (threadIdx.x == 0) {
+    if (tnf) atomicAdd(&c->nf, (uint32_t)tnf);
+    if (tmf) atomicAdd(&c->mf, tmf);
+    if (tin) atomicAdd(&c->inspected, tin);
+  }
+}
+
+// Direction choice for the next level (Beamer et al., SC'12): go bottom-up
+// when the next frontier's out-edges exceed the unexplored edges / alpha;
+// return to top-down when the frontier holds fewer than n / beta vertices.
+__host__ __device__ __forceinline__ bool choose_bottom_up(bool cur_bu, bool can_bu, uint64_t nf, uint64_t mf,
+                                                          uint64_t mu, uint32_t n, float alpha, float beta) {
+  if (!can_bu) return false;
+  if (!cur_bu) return (double)mf * alpha > (double)mu;
+  return (double)nf * beta >= (double)n;
+}
+
+}  // namespace olv

@@ -1,0 +1,115 @@
+// Internal types shared by the translation units of libolive_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <vector>
+
+#include "error.hpp"
+#include "olive_hip.h"
+
+namespace olv {
+
+hipStream_t stream();
+int cu_count();
+int current_device();
+
+// Pull/push schedule of a CSR: vertices split into degree buckets, each run
+// with a different number of lanes per vertex; rows longer than kChunk are
+// split into chunks whose partial sums are combined in a fixed order.
+struct SegPlan {
+  static constexpr uint32_t kBuckets = 4;  // lanes per vertex: 1, 8, 64, block
+  uint32_t *list = nullptr;        // vertex ids grouped by bucket
+  uint32_t offsets[kBuckets + 1] = {0, 0, 0, 0, 0};
+  uint32_t bounds[kBuckets] = {0, 0, 0, 0};  // max degree of each bucket
+  // chunked rows (degree > chunk): (vertex, first chunk slot) and chunk table
+  uint32_t nhub = 0;
+  uint32_t nchunks = 0;
+  uint32_t chunk = 0;
+  uint32_t *hub_vertex = nullptr;   // nhub
+  uint32_t *hub_first = nullptr;    // nhub + 1 (chunk slot ranges)
+  uint32_t *chunk_hub = nullptr;    // nchunks: owning hub index
+  float *chunk_partial = nullptr;   // nchunks
+  uint32_t *hub_done = nullptr;     // nhub arrival counters
+  void release();
+};
+
+struct Tuning {
+  uint32_t td_block_thr = 256;  // degree >= this: whole workgroup per vertex
+  uint32_t td_wave_thr = 64;    // degree >= this: one wave per vertex
+  uint32_t block_threads = 256;
+  float alpha = 14.0f;          // top-down -> bottom-up when m_f > m_u / alpha
+  float beta = 24.0f;           // bottom-up -> top-down when n_f < n / beta
+  bool tuned = false;
+};
+
+struct BfsWork;   // bfs.hip
+struct PrWork;    // pagerank.hip
+
+}  // namespace olv
+
+struct olv_graph {
+  int device = 0;
+  uint32_t n = 0, m = 0;
+  uint32_t *row_ptr = nullptr, *col = nullptr;
+  void *weights = nullptr;
+  int wdtype = OLV_W_NONE;
+  uint32_t *in_row_ptr = nullptr, *in_col = nullptr, *in_eid = nullptr;
+  uint32_t max_outdeg = 0, max_indeg = 0;
+  float *inv_outdeg = nullptr;     // 1/outdeg (0 for sinks), fp32
+  olv::Tuning tune;
+  olv::SegPlan plan_in;   // buckets over the in-CSR (pull PageRank)
+  olv::SegPlan plan_out;  // buckets over the out-CSR (adjoint, gradients)
+  bool plan_in_ready = false, plan_out_ready = false;
+  olv::BfsWork *bfs = nullptr;
+  olv::PrWork *pr = nullptr;
+  uint32_t *union_map = nullptr;   // multigraph: unused here
+};
+
+namespace olv {
+
+// Device helpers implemented in runtime.hip
+int fill_u32(uint32_t *p, uint32_t v, size_t count);
+int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
+void free_bfs_work(olv_graph *g);
+void free_pr_work(olv_graph *g);
+
+// RAII device buffer for temporaries inside an entry point.
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes) {
+    if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) {
+      p = nullptr;
+      return fail(OLV_ENOMEM, "device allocation of %zu bytes failed", bytes);
+    }
+    return OLV_OK;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
+// Event-timed region on the current stream.
+struct Timer {
+  hipEvent_t a = nullptr, b = nullptr;
+  Timer() {
+    (void)hipEventCreate(&a);
+    (void)hipEventCreate(&b);
+  }
+  ~Timer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  void start() { (void)hipEventRecord(a, stream()); }
+  void stop() { (void)hipEventRecord(b, stream()); }
+  double ms() {
+    float t = 0.f;
+    (void)hipEventSynchronize(b);
+    (void)hipEventElapsedTime(&t, a, b);
+    return t;
+  }
+};
+
+}  // namespace olv

@@ -1,0 +1,448 @@
+// PageRank in fp32 (pull over the in-edge CSR), the weighted/teleported
+// fixed point, and its adjoint / vector-Jacobian product.
+//
+// The reference runs PageRank as edgeMap (thread per source vertex, fp64
+// atomicAdd per edge, oliverKernel.h:148-178 + PageRank.cu:51-61), vertexMap
+// (PageRank.cu:63-75) and a host-side vertexReduce (full D2H copy,
+// oliver.h:176-183) per iteration.  Here one fused launch per iteration
+// computes every vertex's in-sum with a fixed summation order, applies the
+// update, writes the next contribution vector and a per-block residual; a
+// one-workgroup finalize kernel totals the residual and raises a device
+// flag that turns the remaining pre-enqueued iterations into no-ops.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+#include "seg_core.hpp"
+
+namespace olv {
+
+struct PrWork {
+  uint32_t n = 0, m = 0;
+  float *buf[4] = {nullptr, nullptr, nullptr, nullptr};  // n floats each
+  float *w_perm = nullptr;                                // m floats
+  float *err_part = nullptr;
+  uint32_t err_cap = 0;
+  IterCtl *ctl = nullptr;
+  IterCtl *h_ctl = nullptr;
+};
+
+void free_pr_work(olv_graph *g) {
+  PrWork *w = g->pr;
+  if (!w) return;
+  for (float *p : w->buf)
+    if (p) (void)hipFree(p);
+  if (w->w_perm) (void)hipFree(w->w_perm);
+  if (w->err_part) (void)hipFree(w->err_part);
+  if (w->ctl) (void)hipFree(w->ctl);
+  if (w->h_ctl) (void)hipHostFree(w->h_ctl);
+  delete w;
+  g->pr = nullptr;
+}
+
+static int get_pr_work(olv_graph *g, bool need_wperm, uint32_t err_blocks, PrWork **out) {
+  if (!g->pr) {
+    PrWork *w = new PrWork();
+    g->pr = w;
+    w->n = g->n;
+    w->m = g->m;
+    for (auto &b : w->buf) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&b), (size_t)g->n * 4));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(IterCtl)));
+    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(IterCtl), hipHostMallocDefault));
+  }
+  PrWork *w = g->pr;
+  if (need_wperm && !w->w_perm) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->w_perm), std::max<size_t>(g->m, 1) * 4));
+  if (w->err_cap < err_blocks) {
+    if (w->err_part) OLV_HIP(hipFree(w->err_part));
+    w->err_part = nullptr;
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->err_part), (size_t)err_blocks * 4));
+    w->err_cap = err_blocks;
+  }
+  *out = w;
+  return OLV_OK;
+}
+
+static int ensure_plan(olv_graph *g, bool in_edges) {
+  if (in_edges) {
+    if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+    if (!g->plan_in_ready) {
+      OLV_TRY(build_seg_plan(g, true, g->plan_in));
+      g->plan_in_ready = true;
+    }
+  } else if (!g->plan_out_ready) {
+    OLV_TRY(build_seg_plan(g, false, g->plan_out));
+    g->plan_out_ready = true;
+  }
+  return OLV_OK;
+}
+
+// ----------------------------------------------------------------- ops
+// testPageRank.cu:79-91: sum_{u->v} r_u/outdeg(u), r' = d*sum + (1-d)/n,
+// err += |r' - r|.  contrib[u] = r_u * (1/outdeg(u)).
+struct PrOp {
+  const uint32_t *ci;
+  const float *contrib_cur;
+  float *contrib_next;
+  float *r;
+  const float *inv_outdeg;
+  float d, base;
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return contrib_cur[ci[k]]; }
+  __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
+    const float rn = d * s + base;
+    err += fabsf(rn - r[v]);
+    r[v] = rn;
+    contrib_next[v] = rn * inv_outdeg[v];
+  }
+};
+
+// r' = d * sum_{u->v} w_uv r_u + (1-d) t_v over the in-CSR with weights
+// permuted into in-slot order.
+struct WprOp {
+  const uint32_t *ci;
+  const float *w;
+  const float *r_cur;
+  float *r_next;
+  const float *t;
+  float d, omd;
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return w[k] * r_cur[ci[k]]; }
+  __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
+    const float rn = d * s + omd * t[v];
+    err += fabsf(rn - r_cur[v]);
+    r_next[v] = rn;
+  }
+};
+
+// lambda'_u = gbar_u + d * sum_{u->v} w_uv lambda_v over the out-CSR.
+struct AdjOp {
+  const uint32_t *ci;
+  const float *w;
+  const float *l_cur;
+  float *l_next;
+  const float *gbar;
+  float d;
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return w[k] * l_cur[ci[k]]; }
+  __device__ __forceinline__ void finish(uint32_t u, float s, float &err) const {
+    const float ln = gbar[u] + d * s;
+    err += fabsf(ln - l_cur[u]);
+    l_next[u] = ln;
+  }
+};
+
+// grad_w[e=(u,v)] = d * r_u * lambda_v, written while walking the out-CSR.
+struct GradOp {
+  const uint32_t *ci;
+  const float *r;
+  const float *lam;
+  float *grad_w;
+  float d;
+  __device__ __forceinline__ float term(uint32_t u, uint32_t k) const {
+    grad_w[k] = d * r[u] * lam[ci[k]];
+    return 0.f;
+  }
+  __device__ __forceinline__ void finish(uint32_t, float, float &) const {}
+};
+
+// --------------------------------------------------------------- kernels
+__global__ void pr_init_kernel(uint32_t n, float r0, const float *__restrict__ inv_outdeg, float *__restrict__ r,
+                               float *__restrict__ contrib, IterCtl *__restrict__ ctl, double eps) {
+  const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
+  for (uint32_t v = tid; v < n; v += gridDim.x * blockDim.x) {
+    r[v] = r0;
+    contrib[v] = r0 * inv_outdeg[v];
+  }
+  if (tid == 0) {
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->eps = eps;
+    ctl->residual = 0.0;
+  }
+}
+
+__global__ void ctl_init_kernel(IterCtl *ctl, double eps) {
+  ctl->done = 0;
+  ctl->iters = 0;
+  ctl->eps = eps;
+  ctl->residual = 0.0;
+}
+
+// eps = tol * sum |gbar| (single workgroup)
+__global__ void __launch_bounds__(1024) ctl_init_l1_kernel(IterCtl *ctl, const float *__restrict__ g, uint32_t n,
+                                                           double tol) {
+  __shared__ double s_red[1024 / kWave];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += fabs((double)g[i]);
+  s = block_sum<1024, double>(s, s_red);
+  if (threadIdx.x == 0) {
+    ctl->done = 0;
+    ctl->iters = 0;
+    ctl->eps = tol * s;
+    ctl->residual = 0.0;
+  }
+}
+
+__global__ void permute_kernel(const float *__restrict__ w, const uint32_t *__restrict__ eid, uint32_t m,
+                               float *__restrict__ out) {
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) out[k] = w[eid[k]];
+}
+
+__global__ void scale_kernel(const float *__restrict__ x, float a, uint32_t n, float *__restrict__ y) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[i];
+}
+
+static unsigned ew_grid(size_t n) {
+  size_t b = (n + 255) / 256;
+  return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * 8));
+}
+
+// hipEvent pairs bracketing each launch of the dominant kernel (OLV_PROFILE).
+struct KernelProfile {
+  std::vector<hipEvent_t> ev;
+  size_t used = 0;
+  bool on = false;
+  ~KernelProfile() {
+    for (hipEvent_t e : ev) (void)hipEventDestroy(e);
+  }
+  int reserve(size_t launches) {
+    on = true;
+    ev.resize(2 * launches, nullptr);
+    for (auto &e : ev) OLV_HIP(hipEventCreate(&e));
+    return OLV_OK;
+  }
+  hipEvent_t next() { return on && used < ev.size() ? ev[used++] : nullptr; }
+  // after the stream has drained
+  int total(double *ms, uint32_t *launches) {
+    double s = 0;
+    for (size_t i = 0; i + 1 < used; i += 2) {
+      float t = 0.f;
+      OLV_HIP(hipEventElapsedTime(&t, ev[i], ev[i + 1]));
+      s += t;
+    }
+    *ms = s;
+    *launches = (uint32_t)(used / 2);
+    return OLV_OK;
+  }
+};
+
+template <class Op>
+static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, PrWork *w,
+                             KernelProfile *prof = nullptr) {
+  const uint32_t blocks = d.role_start[5];
+  if (blocks) {
+    hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
+    if (a) OLV_HIP(hipEventRecord(a, stream()));
+    hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(blocks), dim3(kSegBlock), 0, stream(), d, rp, op,
+                       (const uint32_t *)&w->ctl->done, w->err_part);
+    if (b) OLV_HIP(hipEventRecord(b, stream()));
+  }
+  hipLaunchKernelGGL(seg_finalize_kernel<Op>, dim3(1), dim3(1024), 0, stream(), d, op, (const float *)w->err_part,
+                     blocks, w->ctl);
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
+static int read_ctl(PrWork *w) {
+  OLV_HIP(hipMemcpyAsync(w->h_ctl, w->ctl, sizeof(IterCtl), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  return OLV_OK;
+}
+
+// Host-pointer staging for OLV_PTR_HOST.
+struct Stage {
+  DevBuf buf;
+  const float *in(const float *p, size_t n, bool host, int *st) {
+    if (!host || !p) return p;
+    if ((*st = buf.alloc(n * 4)) != OLV_OK) return nullptr;
+    if (hipMemcpy(buf.p, p, n * 4, hipMemcpyHostToDevice) != hipSuccess) *st = fail(OLV_EHIP, "staging copy failed");
+    return buf.as<float>();
+  }
+  float *out(float *p, size_t n, bool host, int *st) {
+    if (!host || !p) return p;
+    if ((*st = buf.alloc(n * 4)) != OLV_OK) return nullptr;
+    return buf.as<float>();
+  }
+};
+
+}  // namespace olv
+
+using namespace olv;
+
+extern "C" {
+
+int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, double epsilon, float *ranks,
+                     unsigned flags, olv_stats *st) {
+  if (!cg || !ranks) return fail(OLV_EINVAL, "olv_pagerank_f32: NULL argument");
+  olv_graph *g = const_cast<olv_graph *>(cg);
+  OLV_TRY(ensure_plan(g, true));
+  const SegDev d = make_seg_dev(g->plan_in);
+  PrWork *w = nullptr;
+  OLV_TRY(get_pr_work(g, false, std::max(1u, d.role_start[5]), &w));
+  const bool host = (flags & OLV_PTR_HOST) != 0;
+  int s = OLV_OK;
+  Stage so;
+  float *r = so.out(ranks, g->n, host, &s);
+  OLV_TRY(s);
+  KernelProfile prof;
+  if (flags & OLV_PROFILE) OLV_TRY(prof.reserve(max_iters));
+  Timer t;
+  t.start();
+  hipLaunchKernelGGL(pr_init_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), g->n, 1.0f / (float)g->n,
+                     (const float *)g->inv_outdeg, r, w->buf[0], w->ctl, epsilon);
+  const float base = (float)((1.0 - (double)damping) / (double)g->n);
+  for (uint32_t it = 0; it < max_iters; ++it) {
+    PrOp op{g->in_col, w->buf[it & 1], w->buf[(it + 1) & 1], r, g->inv_outdeg, damping, base};
+    OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, &prof));
+  }
+  t.stop();
+  const double ms = t.ms();
+  OLV_TRY(read_ctl(w));
+  double kms = 0;
+  uint32_t kl = 0;
+  if (prof.on) OLV_TRY(prof.total(&kms, &kl));
+  if (host) OLV_HIP(hipMemcpy(ranks, r, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = ms;
+    st->iterations = w->h_ctl->iters;
+    st->residual = w->h_ctl->residual;
+    st->edges_traversed = (uint64_t)g->m * w->h_ctl->iters;
+    st->edges_inspected = st->edges_traversed;
+    st->vertices_reached = g->n;
+    st->kernel_ms = kms;
+    st->kernel_launches = kl;
+  }
+  return OLV_OK;
+}
+
+int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float *tele, float damping, float tol,
+                              uint32_t max_iters, float *ranks, unsigned flags, olv_stats *st) {
+  if (!cg || !wts || !tele || !ranks) return fail(OLV_EINVAL, "olv_pagerank_weighted_f32: NULL argument");
+  olv_graph *g = const_cast<olv_graph *>(cg);
+  OLV_TRY(ensure_plan(g, true));
+  const SegDev d = make_seg_dev(g->plan_in);
+  PrWork *w = nullptr;
+  OLV_TRY(get_pr_work(g, true, std::max(1u, d.role_start[5]), &w));
+  const bool host = (flags & OLV_PTR_HOST) != 0;
+  int s = OLV_OK;
+  Stage sw, st_t, so;
+  const float *dw = sw.in(wts, g->m, host, &s);
+  OLV_TRY(s);
+  const float *dt = st_t.in(tele, g->n, host, &s);
+  OLV_TRY(s);
+  float *r = so.out(ranks, g->n, host, &s);
+  OLV_TRY(s);
+  Timer t;
+  t.start();
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
+  OLV_HIP(hipMemcpyAsync(w->buf[0], dt, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, stream(), w->ctl, (double)tol);
+  const uint32_t group = 8;
+  uint32_t it = 0;
+  for (; it < max_iters;) {
+    const uint32_t stop = std::min(max_iters, it + group);
+    for (; it < stop; ++it) {
+      WprOp op{g->in_col, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], dt, damping, 1.0f - damping};
+      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w));
+    }
+    OLV_TRY(read_ctl(w));
+    if (w->h_ctl->done) break;
+  }
+  t.stop();
+  const double ms = t.ms();
+  OLV_TRY(read_ctl(w));
+  const uint32_t iters = w->h_ctl->iters;
+  OLV_HIP(hipMemcpyAsync(r, w->buf[iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  if (host) OLV_HIP(hipMemcpy(ranks, r, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+  else OLV_HIP(hipStreamSynchronize(stream()));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = ms;
+    st->iterations = iters;
+    st->residual = w->h_ctl->residual;
+    st->edges_traversed = (uint64_t)g->m * iters;
+    st->edges_inspected = st->edges_traversed;
+    st->vertices_reached = g->n;
+  }
+  if (!w->h_ctl->done) return fail(OLV_ENOCONV, "olv_pagerank_weighted_f32: residual %.3e after %u iterations (tol %.3e)",
+                                   w->h_ctl->residual, iters, (double)tol);
+  return OLV_OK;
+}
+
+int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ranks, const float *gbar, float damping,
+                         float tol, uint32_t max_iters, float *grad_w, float *grad_t, float *lambda_out,
+                         unsigned flags, olv_stats *st) {
+  if (!cg || !wts || !ranks || !gbar) return fail(OLV_EINVAL, "olv_pagerank_vjp_f32: NULL argument");
+  olv_graph *g = const_cast<olv_graph *>(cg);
+  OLV_TRY(ensure_plan(g, false));
+  const SegDev d = make_seg_dev(g->plan_out);
+  PrWork *w = nullptr;
+  OLV_TRY(get_pr_work(g, false, std::max(1u, d.role_start[5]), &w));
+  const bool host = (flags & OLV_PTR_HOST) != 0;
+  int s = OLV_OK;
+  Stage sw, sr, sg, sgw, sgt, sl;
+  const float *dw = sw.in(wts, g->m, host, &s);
+  OLV_TRY(s);
+  const float *dr = sr.in(ranks, g->n, host, &s);
+  OLV_TRY(s);
+  const float *dg = sg.in(gbar, g->n, host, &s);
+  OLV_TRY(s);
+  float *dgw = sgw.out(grad_w, g->m, host, &s);
+  OLV_TRY(s);
+  float *dgt = sgt.out(grad_t, g->n, host, &s);
+  OLV_TRY(s);
+  float *dl = sl.out(lambda_out, g->n, host, &s);
+  OLV_TRY(s);
+  Timer t;
+  t.start();
+  // buffers 2/3 hold lambda (0/1 left free for a concurrent forward's state)
+  float *lam[2] = {w->buf[2], w->buf[3]};
+  OLV_HIP(hipMemcpyAsync(lam[0], dg, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  hipLaunchKernelGGL(ctl_init_l1_kernel, dim3(1), dim3(1024), 0, stream(), w->ctl, dg, g->n, (double)tol);
+  const uint32_t group = 8;
+  uint32_t it = 0;
+  for (; it < max_iters;) {
+    const uint32_t stop = std::min(max_iters, it + group);
+    for (; it < stop; ++it) {
+      AdjOp op{g->col, dw, lam[it & 1], lam[(it + 1) & 1], dg, damping};
+      OLV_TRY(enqueue_iteration(d, g->row_ptr, op, w));
+    }
+    OLV_TRY(read_ctl(w));
+    if (w->h_ctl->done) break;
+  }
+  const uint32_t iters = w->h_ctl->iters;
+  const float *lf = lam[iters & 1];
+  if (dgw && g->m) {
+    GradOp gop{g->col, dr, lf, dgw, damping};
+    if (d.role_start[5])
+      hipLaunchKernelGGL(seg_iter_kernel<GradOp>, dim3(d.role_start[5]), dim3(kSegBlock), 0, stream(), d, g->row_ptr,
+                         gop, (const uint32_t *)nullptr, w->err_part);
+    // chunked hub rows wrote their edges inside the chunk blocks already
+  }
+  if (dgt) hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf, 1.0f - damping, g->n, dgt);
+  if (dl) OLV_HIP(hipMemcpyAsync(dl, lf, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  OLV_HIP(hipGetLastError());
+  t.stop();
+  const double ms = t.ms();
+  if (host) {
+    if (grad_w && g->m) OLV_HIP(hipMemcpy(grad_w, dgw, (size_t)g->m * 4, hipMemcpyDeviceToHost));
+    if (grad_t) OLV_HIP(hipMemcpy(grad_t, dgt, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+    if (lambda_out) OLV_HIP(hipMemcpy(lambda_out, dl, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+  }
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = ms;
+    st->iterations = iters;
+    st->residual = w->h_ctl->residual;
+    st->edges_traversed = (uint64_t)g->m * (iters + 1);
+    st->edges_inspected = st->edges_traversed;
+    st->vertices_reached = g->n;
+  }
+  if (!w->h_ctl->done) return fail(OLV_ENOCONV, "olv_pagerank_vjp_f32: residual %.3e after %u iterations",
+                                   w->h_ctl->residual, iters);
+  return OLV_OK;
+}
+
+}  // extern "C"

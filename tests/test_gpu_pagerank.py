@@ -1,0 +1,163 @@
+"""GPU parity for PageRank (fp32 with fp32 accumulation) against the fp64
+oracle, and for the weighted fixed point and its VJP against fp64 dense
+solves and central finite differences.
+
+Tolerances (stated, not fitted):
+  * unweighted PageRank, fixed iterations: max relative error <= 1e-5
+  * weighted fixed point at tol 1e-7: L1 error <= 2e-6, max rel err <= 1e-4
+  * VJP: relative L2 error of grad_w and grad_t <= 1e-4; directional
+    derivatives within 1e-4 (relative) of fp64 central differences
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import DATA_FILES, GOLDEN, data_file
+
+pytestmark = pytest.mark.gpu
+G = np.load(os.path.join(GOLDEN, "golden.npz"))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def maxrel(a, b):
+    return float(np.max(np.abs(a.astype(np.float64) - b) / np.abs(b)))
+
+
+@pytest.mark.parametrize("name", DATA_FILES)
+def test_pagerank_data_files(olv, name):
+    say(f"pr {name}")
+    csr = olv.load_edge_list(data_file(name))
+    g = olv.Graph(csr)
+    out, st = g.pagerank(20)
+    ref, updates, err = oracle.pagerank(csr.row_ptr, csr.col, 20)
+    assert maxrel(out.numpy(), ref) <= 1e-5
+    assert st.iterations == updates
+
+
+@pytest.mark.parametrize("scale", [10, 14, 20])
+def test_pagerank_rmat(olv, scale):
+    say(f"pr rmat{scale}")
+    csr = olv.gen_rmat(scale, 16, seed=1)
+    g = olv.Graph(csr)
+    out, st = g.pagerank(20)
+    ref, updates, err = oracle.pagerank(csr.row_ptr, csr.col, 20)
+    assert st.iterations == updates == 20
+    e = maxrel(out.numpy(), ref)
+    say(f"  max rel err {e:.3e}")
+    assert e <= 1e-5
+    assert abs(st.residual - err) <= 1e-4 * err
+
+
+def test_pagerank_converges_like_reference(olv):
+    # with a large iteration cap the epsilon rule decides, as in testPageRank
+    csr = olv.load_edge_list(data_file("starGraph_1K"))
+    g = olv.Graph(csr)
+    out, st = g.pagerank(1000, epsilon=1e-5)
+    ref, updates, _ = oracle.pagerank(csr.row_ptr, csr.col, 1000, epsilon=1e-5)
+    assert st.iterations == updates
+    assert maxrel(out.numpy(), ref) <= 1e-5
+
+
+def test_pagerank_golden_rmat10(olv):
+    g = olv.Graph(olv.CsrGraph(G["rmat10/row_ptr"], G["rmat10/col"]))
+    out, _ = g.pagerank(20)
+    assert maxrel(out.numpy(), G["rmat10/pr20"]) <= 1e-5
+
+
+def test_pagerank_bit_reproducible(olv):
+    g = olv.Graph(olv.gen_rmat(16, 16, seed=3))
+    a = g.pagerank(20)[0].numpy()
+    b = g.pagerank(20)[0].numpy()
+    assert np.array_equal(a, b)
+
+
+# ------------------------------------------------ weighted + adjoint
+def weighted_case(olv, scale, seed):
+    csr = olv.with_weights_f32_normalized(olv.gen_rmat(scale, 16, seed=seed), seed=seed)
+    t = olv.gen_dirichlet(csr.n, seed=seed)
+    return csr, t
+
+
+def test_weighted_pagerank_forward(olv):
+    csr, t = weighted_case(olv, 12, 5)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    T = olv.DeviceBuffer.from_numpy(t)
+    r, st = g.pagerank_weighted(W, T, tol=1e-7)
+    ref, _ = oracle.pagerank_weighted(csr.row_ptr, csr.col, csr.weights.astype(np.float64),
+                                      t.astype(np.float64), tol=1e-15)
+    got = r.numpy().astype(np.float64)
+    assert np.abs(got - ref).sum() <= 2e-6
+    assert maxrel(got, ref) <= 1e-4
+    # dense fp64 solve agrees with the oracle it was checked against
+    say(f"  weighted iters {st.iterations} L1 {np.abs(got - ref).sum():.2e}")
+
+
+def test_weighted_reduces_to_plain(olv):
+    csr = olv.gen_rmat(12, 16, seed=8)
+    deg = csr.outdeg()
+    src = np.repeat(np.arange(csr.n), deg)
+    w = (1.0 / deg[src]).astype(np.float32)
+    t = np.full(csr.n, 1.0 / csr.n, np.float32)
+    g = olv.Graph(csr)
+    r, st = g.pagerank_weighted(olv.DeviceBuffer.from_numpy(w), olv.DeviceBuffer.from_numpy(t), tol=1e-9,
+                                max_iters=500)
+    ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, 1000, epsilon=1e-13)
+    assert maxrel(r.numpy(), ref) <= 1e-4
+
+
+def test_vjp_against_fp64(olv):
+    csr, t = weighted_case(olv, 12, 6)
+    rng = np.random.default_rng(0)
+    gbar = rng.standard_normal(csr.n).astype(np.float32)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    T = olv.DeviceBuffer.from_numpy(t)
+    r, _ = g.pagerank_weighted(W, T, tol=1e-8)
+    gw, gt, lam, st = g.pagerank_vjp(W, r, olv.DeviceBuffer.from_numpy(gbar), tol=1e-7)
+    w64, t64, g64 = csr.weights.astype(np.float64), t.astype(np.float64), gbar.astype(np.float64)
+    r64, _ = oracle.pagerank_weighted(csr.row_ptr, csr.col, w64, t64, tol=1e-15)
+    gw_ref, gt_ref, lam_ref = oracle.pagerank_vjp(csr.row_ptr, csr.col, w64, r64, g64)
+    rel = lambda a, b: np.linalg.norm(a.astype(np.float64) - b) / np.linalg.norm(b)
+    say(f"  vjp iters {st.iterations} rel gw {rel(gw.numpy(), gw_ref):.2e} gt {rel(gt.numpy(), gt_ref):.2e}")
+    assert rel(gt.numpy(), gt_ref) <= 1e-4
+    assert rel(gw.numpy(), gw_ref) <= 1e-4
+    # fp64 central differences of L = <gbar, r(w, t)> along random directions
+    L = lambda ww, tt: float(g64 @ oracle.pagerank_weighted(csr.row_ptr, csr.col, ww, tt, tol=1e-15)[0])
+    gwn, gtn = gw.numpy().astype(np.float64), gt.numpy().astype(np.float64)
+    for _ in range(3):
+        dw = rng.standard_normal(csr.m)
+        dt = rng.standard_normal(csr.n)
+        h = 1e-5
+        fd = (L(w64 + h * dw, t64 + h * dt) - L(w64 - h * dw, t64 - h * dt)) / (2 * h)
+        an = gwn @ dw + gtn @ dt
+        assert abs(fd - an) <= 1e-4 * max(abs(fd), 1e-3), (fd, an)
+
+
+def test_vjp_dense_scale12(olv):
+    """Scale-12 instance of the RMAT-24 generator against a dense fp64 solve."""
+    csr, t = weighted_case(olv, 12, 24)
+    n = csr.n
+    rng = np.random.default_rng(2)
+    gbar = rng.standard_normal(n).astype(np.float32)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    r, _ = g.pagerank_weighted(W, olv.DeviceBuffer.from_numpy(t), tol=1e-8)
+    gw, gt, lam, _ = g.pagerank_vjp(W, r, olv.DeviceBuffer.from_numpy(gbar), tol=1e-7)
+    P = np.zeros((n, n))
+    src = np.repeat(np.arange(n), csr.outdeg())
+    np.add.at(P, (src, csr.col.astype(np.int64)), csr.weights.astype(np.float64))
+    r_ref = np.linalg.solve(np.eye(n) - 0.85 * P.T, 0.15 * t.astype(np.float64))
+    lam_ref = np.linalg.solve(np.eye(n) - 0.85 * P, gbar.astype(np.float64))
+    assert np.abs(r.numpy() - r_ref).sum() <= 2e-6
+    assert np.linalg.norm(lam.numpy() - lam_ref) / np.linalg.norm(lam_ref) <= 1e-4
+    gw_ref = 0.85 * r_ref[src] * lam_ref[csr.col.astype(np.int64)]
+    assert np.linalg.norm(gw.numpy() - gw_ref) / np.linalg.norm(gw_ref) <= 1e-4
+    assert np.linalg.norm(gt.numpy() - 0.15 * lam_ref) / np.linalg.norm(0.15 * lam_ref) <= 1e-4

@@ -1,0 +1,174 @@
+"""GPU parity for BFS and SSSP through the C ABI: bit-exact against the
+oracle (and the networkx golden vectors) on the reference's data files,
+the grid, RMAT and uniform graphs, in every driver mode."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from conftest import DATA_FILES, GOLDEN, data_file
+
+pytestmark = pytest.mark.gpu
+INF = 0x7FFFFFFF
+G = np.load(os.path.join(GOLDEN, "golden.npz"))
+
+MODES = {
+    "persistent": 0,
+    "host_loop": 0x8,
+    "push_only": 0x2,
+    "push_only_host_loop": 0x2 | 0x8,
+}
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+@pytest.mark.parametrize("name", DATA_FILES)
+def test_bfs_data_files(olv, name, mode):
+    say(f"bfs {name} {mode}")
+    csr = olv.load_edge_list(data_file(name))
+    g = olv.Graph(csr)
+    srcs = [0] if csr.n < 3 else [0, csr.n // 2, csr.n - 1]
+    for s in srcs:
+        out, st = g.bfs(s, flags=MODES[mode])
+        got = out.numpy()
+        assert np.array_equal(got, G[f"{name}/bfs/{s}"]), (name, s)
+        assert np.array_equal(got, oracle.bfs(csr.row_ptr, csr.col, s))
+        # one expansion per level plus the final empty one
+        assert st.iterations == got[got != INF].max() + 1
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_bfs_grid128(olv, mode):
+    say(f"bfs grid128 {mode}")
+    csr = olv.gen_grid(128, 128)
+    g = olv.Graph(csr)
+    out, st = g.bfs(0, flags=MODES[mode])
+    i = np.arange(csr.n)
+    assert np.array_equal(out.numpy(), (i // 128 + i % 128).astype(np.int32))
+    assert st.iterations == 255  # 254 levels + the empty one
+
+
+@pytest.mark.parametrize("scale,seed", [(12, 1), (16, 2), (18, 3)])
+def test_bfs_rmat(olv, scale, seed):
+    say(f"bfs rmat{scale}")
+    csr = olv.gen_rmat(scale, 16, seed=seed)
+    g = olv.Graph(csr)
+    deg = csr.outdeg()
+    srcs = [int(np.argmax(deg)), 0, int(np.flatnonzero(deg)[len(np.flatnonzero(deg)) // 2])]
+    for s in srcs:
+        ref = oracle.bfs(csr.row_ptr, csr.col, s)
+        for mode, fl in MODES.items():
+            out, st = g.bfs(s, flags=fl)
+            assert np.array_equal(out.numpy(), ref), (scale, s, mode)
+        out, st = g.bfs(s)
+        reached = ref != INF
+        assert st.vertices_reached == reached.sum()
+        assert st.edges_traversed == deg[reached].sum()
+
+
+def test_bfs_rmat22_full_size(olv):
+    say("bfs rmat22")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    g = olv.Graph(csr)
+    s = int(np.argmax(csr.outdeg()))
+    ref = oracle.bfs(csr.row_ptr, csr.col, s)
+    for fl in (0, 0x8):
+        out, st = g.bfs(s, flags=fl)
+        assert np.array_equal(out.numpy(), ref)
+    assert st.bottom_up_levels > 0
+
+
+def test_bfs_deterministic_and_reusable(olv):
+    csr = olv.gen_rmat(14, 16, seed=9)
+    g = olv.Graph(csr)
+    a = g.bfs(5)[0].numpy()
+    for _ in range(5):
+        assert np.array_equal(g.bfs(5)[0].numpy(), a)
+
+
+def test_bfs_multisource(olv):
+    csr = olv.gen_rmat(13, 8, seed=4)
+    g = olv.Graph(csr)
+    srcs = [3, 100, 5000, 3]
+    out, st = g.bfs_multisource(srcs)
+    ref = np.min(np.stack([oracle.bfs(csr.row_ptr, csr.col, s).astype(np.int64) for s in srcs]), axis=0)
+    assert np.array_equal(out.numpy(), ref.astype(np.int32))
+
+
+def test_bfs_errors(olv):
+    g = olv.Graph(olv.gen_grid(4, 4))
+    with pytest.raises(olv.OliveError) as e:
+        g.bfs(16)
+    assert e.value.code == 1
+
+
+# -------------------------------------------------------------------- SSSP
+@pytest.mark.parametrize("mode", ["persistent", "host_loop"])
+def test_sssp_weighted_data_files(olv, mode):
+    for name in DATA_FILES:
+        csr = olv.load_edge_list(data_file(name))
+        if csr.weights is None:
+            continue
+        say(f"sssp {name} {mode}")
+        g = olv.Graph(csr)
+        for s in [0, csr.n // 2, csr.n - 1]:
+            out, st = g.sssp(s, flags=MODES[mode])
+            assert np.array_equal(out.numpy(), G[f"{name}/sssp/{s}"]), (name, s)
+
+
+@pytest.mark.parametrize("mode", ["persistent", "host_loop"])
+def test_sssp_uniform(olv, mode):
+    say(f"sssp uniform {mode}")
+    csr = olv.with_weights_i32(olv.gen_uniform(1 << 16, 1 << 19, seed=7), 1, 64, seed=7)
+    g = olv.Graph(csr)
+    for s in (0, 777):
+        ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, s)
+        out, st = g.sssp(s, flags=MODES[mode])
+        assert np.array_equal(out.numpy(), ref)
+        assert np.array_equal(oracle.sssp(csr.row_ptr, csr.col, csr.weights, s)[0], ref)
+
+
+def test_sssp_golden_unif2k(olv):
+    rp, col, w = G["unif2k/row_ptr"], G["unif2k/col"], G["unif2k/w"]
+    g = olv.Graph(olv.CsrGraph(rp, col, w))
+    for s in (0, 999):
+        assert np.array_equal(g.sssp(s)[0].numpy(), G[f"unif2k/sssp/{s}"])
+
+
+def test_sssp_unit_weights_equal_bfs(olv):
+    csr = olv.gen_rmat(14, 16, seed=2)
+    csr1 = csr.with_weights(np.ones(csr.m, np.int32))
+    g = olv.Graph(csr1)
+    for s in (0, int(np.argmax(csr.outdeg()))):
+        assert np.array_equal(g.sssp(s)[0].numpy(), g.bfs(s)[0].numpy())
+
+
+def test_sssp_rmat_heavy_weights(olv):
+    csr = olv.with_weights_i32(olv.gen_rmat(15, 16, seed=5), 1, 1000, seed=5)
+    g = olv.Graph(csr)
+    s = int(np.argmax(csr.outdeg()))
+    assert np.array_equal(g.sssp(s)[0].numpy(), oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, s))
+
+
+def test_sssp_negative_cycle_reports_enoconv(olv):
+    csr = olv.CsrGraph(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.array([-1, -1], np.int32))
+    g = olv.Graph(csr)
+    for fl in (0, 0x8):
+        with pytest.raises(olv.OliveError) as e:
+            g.sssp(0, flags=fl)
+        assert e.value.code == 6
+
+
+def test_sssp_full_size_uniform(olv):
+    say("sssp uniform 2^22 x 2^25")
+    csr = olv.with_weights_i32(olv.gen_uniform(1 << 22, 1 << 25, seed=1), 1, 64, seed=1)
+    g = olv.Graph(csr, transpose=False)
+    out, st = g.sssp(0)
+    ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 0)
+    assert np.array_equal(out.numpy(), ref)
