@@ -1,24 +1,30 @@
 """Benchmark of the frontier engine on one MI355X.
 
-Default workload: PageRank on RMAT scale 20 (1,048,576 vertices, edge
-factor 16, a/b/c/d = .57/.19/.19/.05, self-loops dropped, seed 1), damping
-0.85, exactly 20 iterations, fp32 ranks with fp32 accumulation -- the
-configuration the headline metric is quoted on.  One step = one full
-20-iteration PageRank over the HBM-resident graph.
+Default workload (`--workload pagerank`): PageRank on RMAT scale 20
+(1,048,576 vertices, edge factor 16, a/b/c/d = .57/.19/.19/.05, self-loops
+dropped, seed 1), damping 0.85, exactly 20 iterations, fp32 ranks with fp32
+accumulation -- the configuration the headline metric is quoted on.  One
+step = one full 20-iteration PageRank over the HBM-resident graph.  The line
+also carries a `secondary` BFS measurement on RMAT scale 22.
 
-Prints one JSON line (rank 0).  `value` is edges/s = E * 20 * steps / time.
-`roofline` covers the dominant kernel (the fused pull iteration): algorithmic
-bytes per launch 8E + 12V over its hipEvent-timed average launch duration.
-`cpu_baseline` is the oracle's serial fp64 PageRank (restating
+Prints one JSON line.  `value` is edges/s = E * 20 * steps / time.
+`roofline` covers the dominant kernel (the fused pull iteration):
+algorithmic bytes per launch 8E + 12V over its hipEvent-timed average launch
+duration.  `cpu_baseline` is the oracle's serial fp64 PageRank (restating
 testPageRank.cu:57-104) on one host core over the same graph.
 
-Secondary workloads (--workload bfs|sssp|batched|adjoint) print their own
-JSON line with the same keys.
+Other workloads print their own line with the same keys:
+  bfs      BFS on RMAT-22 from seeded sources (GTEPS, bytes 8E_r+16V_r+4V)
+  sssp     SSSP on the uniform 2^22 x 2^25 graph, int32 w in [1,64], source 0
+  batched  256-source BFS on RMAT-22 and the 64-graph RMAT batch (scales
+           12..18) as one captured persistent launch, vs the plain engine
+  adjoint  weighted PageRank forward + VJP on RMAT-24 vs forward-only with
+           central finite differences
 """
 import argparse
+import ctypes
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -29,6 +35,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "GTEPS (BFS/SSSP), PageRank edges/s + %HBM peak; batched/adjoint vs plain"
+INF = 0x7FFFFFFF
 
 
 def log(*a):
@@ -46,21 +53,41 @@ def load_traffic(kernel_key):
         return None
 
 
-def cpu_baseline_pagerank(csr, iters, budget_s=20.0, max_runs=5):
-    import oracle
+def base_line(args, workload_cfg, value, unit, ms_per_step, dtype, higher=True):
+    return {
+        "metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms_per_step, "higher_is_better": higher, "scaling": "weak", "vs_baseline": None,
+        "dtype": dtype, "data": "synthetic (seeded generators)", "config": workload_cfg,
+    }
+
+
+def roofline(bytes_per_launch, avg_launch_s, key, kernel):
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": load_traffic(key), "kernel": kernel,
+            "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6}
+
+
+def timed_oracle(fn, budget_s, max_runs=5):
     times = []
     t_start = time.perf_counter()
     while len(times) < max_runs and (time.perf_counter() - t_start) < budget_s:
         t0 = time.perf_counter()
-        oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0)
+        fn()
         times.append(time.perf_counter() - t0)
-    t = float(np.median(times))
-    return {"value": csr.m * iters / t, "unit": "edges/s", "cores": 1, "kind": "port",
-            "sample": f"full workload (RMAT-20, {iters} iterations), serial fp64 restatement of "
-                      f"testPageRank.cu:57-104, median of {len(times)} runs"}
+    return float(np.median(times)), len(times)
 
 
+def sources_of(csr, k, seed):
+    """k sources among vertices with out-edges (Graph500 convention)."""
+    nz = np.flatnonzero(csr.outdeg())
+    rng = np.random.default_rng(seed)
+    return rng.choice(nz, size=k, replace=k > nz.size).astype(np.uint32)
+
+
+# ------------------------------------------------------------------ PageRank
 def bench_pagerank(args, olv):
+    import oracle
     iters = 20
     log("generating RMAT-20 ...")
     csr = olv.gen_rmat(20, 16, seed=1)
@@ -68,7 +95,6 @@ def bench_pagerank(args, olv):
     n, m = csr.n, csr.m
     out = olv.DeviceBuffer(n, np.float32)
     L = olv._lib.lib()
-    import ctypes
     st = olv.Stats()
     for _ in range(args.warmup):
         olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c, 0, ctypes.byref(st)))
@@ -81,34 +107,241 @@ def bench_pagerank(args, olv):
         kl += st.kernel_launches
     L.olv_synchronize()
     dt = time.perf_counter() - t0
-    bytes_per_launch = 8 * m + 12 * n
-    avg_launch_s = kms / kl / 1e3
-    achieved = bytes_per_launch / avg_launch_s / 1e9
-    traffic = load_traffic("pagerank_rmat20")
-    res = {
-        "metric": METRIC,
-        "value": m * iters * args.steps / dt,
-        "unit": "edges/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": dt / args.steps * 1e3,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic (seeded RMAT generator)",
-        "config": {"workload": "pagerank_rmat20", "vertices": n, "edges": m, "iterations": iters,
-                   "damping": 0.85, "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic,
-                     "kernel": "seg_iter_kernel<PrOp>", "bytes_per_launch": bytes_per_launch,
-                     "avg_launch_us": avg_launch_s * 1e6},
-    }
+    # accuracy of the last run against the fp64 oracle (not timed)
+    ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0)
+    err = float(np.max(np.abs(out.numpy().astype(np.float64) - ref) / ref))
+    cfg = {"workload": "pagerank_rmat20", "vertices": n, "edges": m, "iterations": iters, "damping": 0.85,
+           "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "max_rel_err_vs_fp64": err}
+    res = base_line(args, cfg, m * iters * args.steps / dt, "edges/s", dt / args.steps * 1e3, "f32")
+    res["roofline"] = roofline(8 * m + 12 * n, kms / kl / 1e3, "pagerank_rmat20", "seg_iter_kernel<PrOp>")
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline_pagerank(csr, iters)
+        t, runs = timed_oracle(lambda: oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0), 20.0)
+        res["cpu_baseline"] = {"value": m * iters / t, "unit": "edges/s", "cores": 1, "kind": "port",
+                               "sample": f"full workload (RMAT-20, {iters} iterations), serial fp64 restatement of "
+                                         f"testPageRank.cu:57-104, median of {runs} runs"}
+    if not args.no_secondary:
+        del g
+        res["secondary"] = {"bfs_rmat22": bfs_rmat22(olv, args, cpu=False)[0]}
+    return res
+
+
+# ----------------------------------------------------------------------- BFS
+def bfs_rmat22(olv, args, cpu=True, nsrc=8):
+    import oracle
+    log("generating RMAT-22 ...")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    g = olv.Graph(csr)
+    srcs = sources_of(csr, nsrc, 22)
+    out = olv.DeviceBuffer(csr.n, np.int32)
+    for s in srcs[:2]:
+        for _ in range(max(1, args.warmup)):
+            g.bfs(int(s), out=out)
+    per = []
+    for s in srcs:
+        _, st = g.bfs(int(s), out=out)
+        per.append((st.elapsed_ms / 1e3, st.edges_traversed, st.vertices_reached, st.iterations,
+                    st.bottom_up_levels))
+    t = np.array([p[0] for p in per])
+    er = np.array([p[1] for p in per], np.float64)
+    vr = np.array([p[2] for p in per], np.float64)
+    teps = er / t
+    gteps_h = len(teps) / np.sum(1.0 / teps) / 1e9  # harmonic mean (Graph500)
+    bytes_ = 8 * er + 16 * vr + 4 * csr.n
+    frac = float(np.mean(bytes_ / t / 1e9 / HBM_PEAK_GBS))
+    check = oracle.bfs(csr.row_ptr, csr.col, int(srcs[-1]))
+    ok = bool(np.array_equal(out.numpy(), check))
+    r = {"gteps_harmonic": gteps_h, "hbm_frac_mean": frac, "ms_mean": float(t.mean() * 1e3),
+         "edges": csr.m, "sources": len(srcs), "levels": [p[3] for p in per], "bottom_up_levels": [p[4] for p in per],
+         "bit_exact_last_source": ok}
+    # the same sources through the other level drivers (same results)
+    modes = {"host_loop": olv.OLV_HOST_LOOP, "persistent": olv.OLV_PERSISTENT, "push_only": olv.OLV_NO_DIRECTION_OPT}
+    r["modes_ms_mean"] = {"pipeline": r["ms_mean"]}
+    for name, fl in modes.items():
+        ts = []
+        for s in srcs[:4]:
+            g.bfs(int(s), out=out, flags=fl)
+            ts.append(g.bfs(int(s), out=out, flags=fl)[1].elapsed_ms)
+        r["modes_ms_mean"][name] = float(np.mean(ts))
+    if cpu:
+        tc, runs = timed_oracle(lambda: oracle.bfs(csr.row_ptr, csr.col, int(srcs[0])), 20.0)
+        r["cpu_gteps"] = float(er[0] / tc / 1e9)
+    return r, csr, g, srcs
+
+
+def bench_bfs(args, olv):
+    r, csr, g, srcs = bfs_rmat22(olv, args, cpu=not args.no_cpu_baseline)
+    # roofline of the persistent BFS kernel (one launch per traversal)
+    cfg = {"workload": "bfs_rmat22", "vertices": csr.n, "edges": csr.m, "sources": r["sources"],
+           "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "bit_exact": r["bit_exact_last_source"]}
+    res = base_line(args, cfg, r["gteps_harmonic"], "GTEPS", r["ms_mean"], "int32")
+    res["roofline"] = {"bound": "hbm", "achieved": r["hbm_frac_mean"] * HBM_PEAK_GBS, "peak": HBM_PEAK_GBS,
+                       "unit": "GB/s", "frac": r["hbm_frac_mean"], "traffic": load_traffic("bfs_rmat22"),
+                       "kernel": "bfs_persistent_kernel", "bytes_model": "8*E_r + 16*V_r + 4*V"}
+    if "cpu_gteps" in r:
+        res["cpu_baseline"] = {"value": r["cpu_gteps"], "unit": "GTEPS", "cores": 1, "kind": "port",
+                               "sample": "one source of the same graph, serial restatement of testBFS.cu:56-79"}
+    res["detail"] = r
+    return res
+
+
+# ---------------------------------------------------------------------- SSSP
+def bench_sssp(args, olv):
+    import oracle
+    log("generating uniform 2^22 x 2^25 ...")
+    csr = olv.with_weights_i32(olv.gen_uniform(1 << 22, 1 << 25, seed=1), 1, 64, seed=1)
+    g = olv.Graph(csr, transpose=False)
+    out = olv.DeviceBuffer(csr.n, np.int32)
+    for _ in range(args.warmup):
+        g.sssp(0, out=out)
+    ts = []
+    for _ in range(args.steps):
+        _, st = g.sssp(0, out=out)
+        ts.append(st.elapsed_ms / 1e3)
+    t = float(np.median(ts))
+    er, vr = st.edges_traversed, st.vertices_reached
+    ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 0)
+    ok = bool(np.array_equal(out.numpy(), ref))
+    cfg = {"workload": "sssp_uniform_4M_33M", "vertices": csr.n, "edges": csr.m, "weights": [1, 64], "source": 0,
+           "bit_exact": ok, "rounds": st.iterations, "edges_relaxed": st.edges_inspected}
+    res = base_line(args, cfg, er / t / 1e9, "GTEPS", t * 1e3, "int32")
+    b = 12 * er + 16 * vr + 4 * csr.n
+    res["roofline"] = {"bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                       "frac": b / t / 1e9 / HBM_PEAK_GBS, "traffic": load_traffic("sssp_uniform"),
+                       "kernel": "sssp_persistent_kernel", "bytes_model": "12*E_r + 16*V_r + 4*V"}
+    if not args.no_cpu_baseline:
+        tc, runs = timed_oracle(lambda: oracle.sssp(csr.row_ptr, csr.col, csr.weights, 0), 30.0, 2)
+        res["cpu_baseline"] = {"value": er / tc / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
+                               "sample": f"full workload, serial frontier relaxation (SSSP.cu:146-163), "
+                                         f"median of {runs}"}
+    return res
+
+
+# ------------------------------------------------------------------- batched
+def bench_batched(args, olv):
+    import oracle
+    L = olv._lib.lib()
+    log("generating RMAT-22 ...")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    g = olv.Graph(csr)
+    srcs = sources_of(csr, 256, 4)
+    out = olv.DeviceBuffer(csr.n * 256, np.int32)
+    plan = ctypes.c_void_p()
+    olv._lib.check(L.olv_plan_bfs_batched(g.h, srcs.ctypes.data, 256, out.c, 0, ctypes.byref(plan)))
+    st = olv.Stats()
+    for _ in range(args.warmup):
+        olv._lib.check(L.olv_plan_launch(plan, ctypes.byref(st)))
+    tb = []
+    for _ in range(max(3, args.steps // 4)):
+        olv._lib.check(L.olv_plan_launch(plan, ctypes.byref(st)))
+        tb.append(st.elapsed_ms / 1e3)
+    t_batched = float(np.median(tb))
+    got = out.numpy().reshape(256, csr.n)
+    exact = all(np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(srcs[i]))) for i in (0, 97, 255))
+    # plain engine, one source at a time (same sources)
+    one = olv.DeviceBuffer(csr.n, np.int32)
+    g.bfs(int(srcs[0]), out=one)
+    t0 = time.perf_counter()
+    er_tot = 0
+    for s in srcs:
+        _, s1 = g.bfs(int(s), out=one)
+        er_tot += s1.edges_traversed
+    L.olv_synchronize()
+    t_plain = time.perf_counter() - t0
+    L.olv_plan_destroy(plan)
+    del out
+    # 64 RMAT graphs, scales uniform in [12, 18], one captured launch
+    log("generating the 64-graph batch ...")
+    csrs = [olv.gen_rmat(olv.batch_scale(1, i, 12, 18), 16, seed=olv.batch_seed(1, i)) for i in range(64)]
+    gs = [olv.Graph(c) for c in csrs]
+    bsrc = np.array([int(np.flatnonzero(c.outdeg())[0]) for c in csrs], np.uint32)
+    arr = (ctypes.c_void_p * 64)(*[x.h.value for x in gs])
+    uni = ctypes.c_void_p()
+    offs = np.zeros(65, np.uint32)
+    olv._lib.check(L.olv_graph_union(arr, 64, ctypes.byref(uni), offs.ctypes.data))
+    seeds = (offs[:64] + bsrc).astype(np.uint32)
+    uout = olv.DeviceBuffer(int(offs[64]), np.int32)
+    uplan = ctypes.c_void_p()
+    olv._lib.check(L.olv_plan_bfs(uni, seeds.ctypes.data, 64, uout.c, 0, ctypes.byref(uplan)))
+    for _ in range(args.warmup):
+        olv._lib.check(L.olv_plan_launch(uplan, ctypes.byref(st)))
+    tu = []
+    for _ in range(max(3, args.steps // 4)):
+        olv._lib.check(L.olv_plan_launch(uplan, ctypes.byref(st)))
+        tu.append(st.elapsed_ms / 1e3)
+    t_union = float(np.median(tu))
+    uo = uout.numpy()
+    uexact = all(np.array_equal(uo[offs[i]:offs[i + 1]], oracle.bfs(csrs[i].row_ptr, csrs[i].col, int(bsrc[i])))
+                 for i in (0, 31, 63))
+    t0 = time.perf_counter()
+    for x, s in zip(gs, bsrc):
+        x.bfs(int(s))
+    L.olv_synchronize()
+    t_plain_multi = time.perf_counter() - t0
+    L.olv_plan_destroy(uplan)
+    L.olv_graph_destroy(uni)
+    cfg = {"workload": "batched_bfs_rmat22_256src_plus_64_graphs", "vertices": csr.n, "edges": csr.m,
+           "sources": 256, "bit_exact_sampled": bool(exact and uexact),
+           "multigraph_vertices": int(offs[64]), "multigraph_edges": int(sum(c.m for c in csrs))}
+    res = base_line(args, cfg, er_tot / t_batched / 1e9, "GTEPS", t_batched * 1e3, "int32")
+    res["detail"] = {"batched_256_ms": t_batched * 1e3, "plain_256_ms": t_plain * 1e3,
+                     "speedup_256": t_plain / t_batched, "multigraph_ms": t_union * 1e3,
+                     "plain_64_graphs_ms": t_plain_multi * 1e3, "speedup_64_graphs": t_plain_multi / t_union}
+    return res
+
+
+# ------------------------------------------------------------------- adjoint
+def bench_adjoint(args, olv, scale=24):
+    log(f"generating weighted RMAT-{scale} ...")
+    csr = olv.with_weights_f32_normalized(olv.gen_rmat(scale, 16, seed=1), seed=1)
+    t = olv.gen_dirichlet(csr.n, seed=1)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    T = olv.DeviceBuffer.from_numpy(t)
+    rng = np.random.default_rng(1)
+    gbar = olv.DeviceBuffer.from_numpy(rng.standard_normal(csr.n).astype(np.float32))
+    r, st = g.pagerank_weighted(W, T, tol=1e-6)
+    fwd = []
+    adj = []
+    for _ in range(max(2, args.steps // 4)):
+        r, st = g.pagerank_weighted(W, T, tol=1e-6, out=r)
+        fwd.append(st.elapsed_ms)
+        _, _, _, sa = g.pagerank_vjp(W, r, gbar, tol=1e-6)
+        adj.append(sa.elapsed_ms)
+    t_f, t_a = float(np.median(fwd)), float(np.median(adj))
+    # one directional derivative by central differences = 2 forward solves
+    t_fd_dir = 2 * t_f
+    # full gradient by FD needs 2 solves per parameter (E + V parameters)
+    k5 = (t_f + t_a) * 5 / t_fd_dir
+    cfg = {"workload": f"pagerank_fwd_adjoint_rmat{scale}", "vertices": csr.n, "edges": csr.m, "tol": 1e-6,
+           "fwd_iterations": st.iterations, "adj_iterations": sa.iterations}
+    res = base_line(args, cfg, csr.m * (st.iterations + sa.iterations + 1) / ((t_f + t_a) / 1e3), "edges/s",
+                    t_f + t_a, "f32")
+    res["detail"] = {"forward_ms": t_f, "vjp_ms": t_a, "fd_per_direction_ms": t_fd_dir,
+                     "fd_full_gradient_ms": t_fd_dir * (csr.m + csr.n),
+                     "speedup_full_gradient_vs_fd": t_fd_dir * (csr.m + csr.n) / (t_f + t_a),
+                     "fd_directions_for_5x": k5}
+    return res
+
+
+def bench_copy(args, olv):
+    """Device-to-device copy of a known size: the calibration kernel for the
+    PMC byte counters (FETCH_SIZE / WRITE_SIZE) of the other workloads."""
+    nbytes = 256 << 20
+    a = olv.DeviceBuffer(nbytes // 4, np.uint32)
+    b = olv.DeviceBuffer(nbytes // 4, np.uint32)
+    a.fill(7)
+    L = olv._lib.lib()
+    for _ in range(args.warmup):
+        olv._lib.check(L.olv_memcpy_d2d(b.ptr, a.ptr, nbytes))
+    L.olv_synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        olv._lib.check(L.olv_memcpy_d2d(b.ptr, a.ptr, nbytes))
+    L.olv_synchronize()
+    dt = time.perf_counter() - t0
+    cfg = {"workload": "copy_256MiB", "bytes_per_copy": nbytes}
+    res = base_line(args, cfg, 2 * nbytes * args.steps / dt / 1e9, "GB/s", dt / args.steps * 1e3, "u8")
     return res
 
 
@@ -117,8 +350,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="pagerank", choices=["pagerank"])
+    ap.add_argument("--workload", default="pagerank",
+                    choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one GPU holding the whole graph; "
@@ -126,7 +361,9 @@ def main():
         sys.exit(2)
     import olive_amd as olv
     olv.init(0)
-    res = bench_pagerank(args, olv)
+    fn = {"pagerank": bench_pagerank, "bfs": bench_bfs, "sssp": bench_sssp, "batched": bench_batched,
+          "adjoint": bench_adjoint, "copy": bench_copy}[args.workload]
+    res = fn(args, olv)
     print(json.dumps(res), flush=True)
 
 

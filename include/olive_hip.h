@@ -42,9 +42,12 @@ enum {
 #define OLV_PTR_HOST        0x0001u  /* inputs/outputs are host pointers            */
 #define OLV_NO_DIRECTION_OPT 0x0002u /* BFS: top-down (push) only                    */
 #define OLV_FORCE_BOTTOM_UP  0x0004u /* BFS: bottom-up (pull) only, needs transpose  */
-#define OLV_HOST_LOOP        0x0008u /* drive the level loop from the host (one
-                                        launch group per level) instead of the
-                                        persistent cooperative kernel            */
+#define OLV_HOST_LOOP        0x0008u /* wait for every level on the host instead
+                                        of enqueueing levels in chunks whose
+                                        control lives on the device             */
+#define OLV_PERSISTENT       0x0020u /* BFS/SSSP: all levels in one cooperative
+                                        persistent kernel with a grid barrier
+                                        (slower on MI355X; kept to compare)      */
 #define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
                                         with hipEvents (olv_stats.kernel_ms)     */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */

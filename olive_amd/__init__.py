@@ -16,8 +16,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FORCE_BOTTOM_UP, OLV_HOST_LOOP, OLV_NO_DIRECTION_OPT, OLV_PTR_HOST,
-                   OLV_W_F32, OLV_W_I32, OLV_W_NONE, HostCsr, OliveError, Stats, check, lib)
+from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FORCE_BOTTOM_UP, OLV_HOST_LOOP, OLV_NO_DIRECTION_OPT, OLV_PERSISTENT,
+                   OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE, HostCsr, OliveError, Stats, check, lib)
 
 INF = 0x7FFFFFFF
 

@@ -9,7 +9,9 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libolive_hip.so")
+# OLIVE_LIB selects another build of the same sources (e.g. one compiled with
+# different tuning macros by `python -m olive_amd.build --define ... --out`).
+LIB_PATH = os.environ.get("OLIVE_LIB") or os.path.join(HERE, "libolive_hip.so")
 
 OLV_OK = 0
 OLV_ENOCONV = 6
@@ -21,6 +23,7 @@ OLV_NO_DIRECTION_OPT = 0x2
 OLV_FORCE_BOTTOM_UP = 0x4
 OLV_HOST_LOOP = 0x8
 OLV_PROFILE = 0x10
+OLV_PERSISTENT = 0x20
 OLV_BUILD_TRANSPOSE = 0x100
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
@@ -111,7 +114,7 @@ SIGNATURES = {
     "olv_bfs_batched": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),
     "olv_sssp_batched": (_int, [_vp, _vp, _u32, _vp, _vp, _uint, _statp]),
     "olv_bfs_multigraph": (_int, [ctypes.POINTER(_vp), _u32, _vp, ctypes.POINTER(_vp), _uint, _statp]),
-    "olv_graph_union": (_int, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(_vp), _u32p]),
+    "olv_graph_union": (_int, [ctypes.POINTER(_vp), _u32, ctypes.POINTER(_vp), _vp]),
     "olv_plan_bfs": (_int, [_vp, _vp, _u32, _vp, _uint, ctypes.POINTER(_vp)]),
     "olv_plan_bfs_batched": (_int, [_vp, _vp, _u32, _vp, _uint, ctypes.POINTER(_vp)]),
     "olv_plan_launch": (_int, [_vp, _statp]),

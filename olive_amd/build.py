@@ -45,13 +45,14 @@ def device_flags():
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
 
-def compile_one(src: str, verbose: bool) -> str:
+def compile_one(src: str, verbose: bool, build_dir: str = BUILD, defines=()) -> str:
     base = os.path.splitext(os.path.basename(src))[0]
-    obj = os.path.join(BUILD, base + ".o")
+    obj = os.path.join(build_dir, base + ".o")
+    dflags = [f"-D{d}" for d in defines]
     if src.endswith(".hip"):
-        cmd = [hipcc(), "-c", "-x", "hip", src, "-o", obj] + common_flags() + device_flags()
+        cmd = [hipcc(), "-c", "-x", "hip", src, "-o", obj] + common_flags() + device_flags() + dflags
     else:
-        cmd = [hipcc(), "-c", src, "-o", obj] + common_flags()
+        cmd = [hipcc(), "-c", src, "-o", obj] + common_flags() + dflags
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -70,28 +71,33 @@ def sources():
     return out
 
 
-def build(jobs: int = 8, verbose: bool = False) -> str:
-    os.makedirs(BUILD, exist_ok=True)
+def build(jobs: int = 8, verbose: bool = False, defines=(), out: str = LIB) -> str:
+    """Compile and link.  `defines` (NAME=VALUE) override the tuning macros
+    of the sources; `out` names the library (default: the in-tree one)."""
+    build_dir = BUILD if out == LIB else out + ".objs"
+    os.makedirs(build_dir, exist_ok=True)
     srcs = sources()
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
-        objs = list(ex.map(lambda s: compile_one(s, verbose), srcs))
-    tmp = LIB + ".tmp"
+        objs = list(ex.map(lambda s: compile_one(s, verbose, build_dir, defines), srcs))
+    tmp = out + ".tmp"
     cmd = [hipcc(), "-shared", "-o", tmp] + objs + [f"--offload-arch={ARCH}", "-lpthread", "-Wl,--no-undefined"]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed\n{r.stdout}\n{r.stderr}")
-    os.replace(tmp, LIB)
-    return LIB
+    os.replace(tmp, out)
+    return out
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--jobs", type=int, default=min(8, os.cpu_count() or 1))
     ap.add_argument("--verbose", action="store_true")
+    ap.add_argument("--define", action="append", default=[], help="NAME=VALUE tuning macro")
+    ap.add_argument("--out", default=LIB)
     a = ap.parse_args(argv)
-    print(build(a.jobs, a.verbose))
+    print(build(a.jobs, a.verbose, a.define, os.path.abspath(a.out)))
 
 
 if __name__ == "__main__":

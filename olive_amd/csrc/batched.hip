@@ -3,14 +3,15 @@
 //  * olv_bfs_batched: up to 256 sources per sweep, bit-parallel.  Every
 //    vertex carries a 256-bit visited mask V, a frontier mask F and (for
 //    push levels) an accumulator N; one adjacency sweep per level serves all
-//    sources.  Push (top-down) levels OR F[u] into N[v] with atomics;
-//    pull (bottom-up) levels OR the F of in-neighbours into v and stop once
-//    v has every source.  Source s's level of v is written when bit s of
-//    V[v] first turns on.
+//    sources.  Push (top-down) levels OR F[u] into N[v] with atomics and a
+//    contraction pass applies them; pull (bottom-up) levels OR the F of
+//    in-neighbours into v and stop once v has every source.  Source s's
+//    level of v is written when bit s of V[v] first turns on.  Like the
+//    plain BFS, a level is a few kernels whose control lives on the device.
 //  * olv_bfs_multisource / olv_graph_union / olv_bfs_multigraph: many
 //    graphs as one block-diagonal union traversed by one multi-source BFS.
-//  * olv_plan_*: the whole traversal (reset, seeding, persistent kernel)
-//    captured once in a hipGraph and replayed.
+//  * olv_plan_*: a whole traversal (reset, seeding, all levels) captured
+//    once in a hipGraph and replayed with one launch.
 //  * olv_autotune: picks degree-bucket thresholds and direction-switch
 //    parameters for a graph by timing the plain BFS.
 #include <hip/hip_runtime.h>
@@ -21,20 +22,35 @@
 
 #include "bfs_core.hpp"
 #include "internal.hpp"
+#include "seg_core.hpp"
 
 namespace olv {
 
 int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool seeds_host, int32_t *levels,
                     unsigned flags, olv_stats *st);
 int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w);
-int bfs_enqueue(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels, bool can_bu);
+int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels);
+int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count);
 int bfs_result(BfsWork *w, uint32_t *levels_run, uint32_t *bu_levels);
+bool bfs_done(BfsWork *w);
+uint32_t bfs_depth_hint(BfsWork *w);
 int reach_stats(const olv_graph *g, const int32_t *d, uint32_t k, olv_stats *st);
-uint32_t *bfs_result_words(BfsWork *w);
 
 constexpr int kMB = 256;          // workgroup size
 constexpr int kMW = 8;            // 32-bit words per vertex mask
-constexpr uint32_t kMSpin = 1u << 26;
+
+struct MbfsState {
+  unsigned long long mu;
+  uint32_t mode, nbu;
+};
+
+struct MbfsCtl {
+  uint32_t level, level_e, done, levels, nbu, overflow;
+  uint32_t qfill[2], hfill[2];  // frontier queue / hub items of level L, slot L & 1
+  uint32_t pad[2];
+  MbfsState st[2];
+  BfsCtr cnt[3];
+};
 
 struct MbfsParams {
   const uint32_t *row_ptr, *col, *in_row_ptr, *in_col;
@@ -44,15 +60,15 @@ struct MbfsParams {
   uint32_t *V;     // n * kMW
   uint32_t *F;     // 2 * n * kMW
   uint32_t *N;     // n * kMW
-  uint32_t *A;     // 3 * nwp
-  uint32_t *T;     // 2 * nwp
-  uint32_t *D;     // nwp
+  uint32_t *A;     // 3 * nwp  active (frontier) bitmaps
+  uint32_t *T;     // 2 * nwp  touched bitmaps (push levels)
+  uint32_t *D;     // nwp      vertices that have every source
   uint32_t *queue; // n
+  uint2 *hubs;     // hub items
   int32_t *levels; // k * n
-  BfsCtr *ctr;     // 3
-  uint32_t *bar;   // 16
+  MbfsCtl *ctl;
   unsigned long long m;
-  uint32_t wave_thr, block_thr;
+  uint32_t wave_thr, block_thr, hub_thr;
   float alpha, beta;
   uint32_t can_bu;
 };
@@ -91,9 +107,9 @@ struct MbfsTdVisitor {
   }
 };
 
-// Apply new bits of v: V |= nw, F_next = nw, levels for each new source.
-__device__ __forceinline__ bool mbfs_apply(const MbfsParams &p, uint32_t v, const uint32_t *nwv, uint32_t *vv,
-                                           uint32_t *Fn, int32_t nlevel) {
+// V |= new, F_next = new; returns whether v now has every source.
+__device__ __forceinline__ bool mbfs_apply(const MbfsParams &p, uint32_t v, const uint32_t *nwv, const uint32_t *vv,
+                                           uint32_t *Fn) {
   bool full = true;
 #pragma unroll
   for (int j = 0; j < kMW; ++j) {
@@ -104,11 +120,236 @@ __device__ __forceinline__ bool mbfs_apply(const MbfsParams &p, uint32_t v, cons
   return full;
 }
 
-template <int BLOCK>
-__device__ void mbfs_contract(const MbfsParams &p, const uint32_t *T, uint32_t *An, uint32_t *Fn, int32_t nlevel,
-                              uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf) {
-  for (uint32_t w = vb * BLOCK + threadIdx.x; w < p.nw; w += nvb * BLOCK) {
-    uint32_t tw = T[w];
+__device__ __forceinline__ void mbfs_buffers(const MbfsParams &p, uint32_t L, const uint32_t *&Ff, uint32_t *&Fn,
+                                             const uint32_t *&Af, uint32_t *&An, uint32_t *&Ac, uint32_t *&Tc,
+                                             uint32_t *&Tx) {
+  const size_t fstride = (size_t)p.n * kMW;
+  Ff = p.F + (size_t)((L + 1) & 1) * fstride;
+  Fn = p.F + (size_t)(L & 1) * fstride;
+  Af = p.A + (size_t)((L + 2) % 3) * p.nwp;
+  An = p.A + (size_t)(L % 3) * p.nwp;
+  Ac = p.A + (size_t)((L + 1) % 3) * p.nwp;
+  Tc = p.T + (size_t)(L & 1) * p.nwp;
+  Tx = p.T + (size_t)((L + 1) & 1) * p.nwp;
+}
+
+// Level L bookkeeping, and for a push level the expansion: touch T,
+// accumulate N (a pull level is done by mbfs_pull_kernel).
+__global__ void __launch_bounds__(kMB) mbfs_expand_kernel(MbfsParams p) {
+  MbfsCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level;
+  const MbfsState st = ctl->st[L & 1];
+  const uint32_t qsize = ctl->qfill[L & 1], nhubs = ctl->hfill[L & 1];
+  const uint32_t *Ff, *Af;
+  uint32_t *Fn, *An, *Ac, *Tc, *Tx;
+  mbfs_buffers(p, L, Ff, Fn, Af, An, Ac, Tc, Tx);
+  for (uint32_t i = blockIdx.x * kMB + threadIdx.x; i < p.nwp; i += gridDim.x * kMB) {
+    Ac[i] = 0u;
+    Tx[i] = 0u;
+  }
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+    if (threadIdx.x == 0) {
+      ctl->level_e = L;
+      ctl->qfill[(L + 1) & 1] = 0;
+      ctl->hfill[(L + 1) & 1] = 0;
+    }
+    reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
+  }
+  uint32_t nf = 0;
+  unsigned long long mf = 0, insp = 0;
+  const int32_t nlevel = (int32_t)(L + 1);
+  if (st.mode == 0) {
+    MbfsTdVisitor vis(p, Ff, Tc);
+    push_expand<kMB>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, blockIdx.x, gridDim.x, vis,
+                     insp);
+  }
+  flush_counters<kMB>(&ctl->cnt[L % 3], nf, mf, insp);
+}
+
+// ---- pull levels: bucketed by in-degree (the SegPlan of the in-CSR) ----
+__device__ __forceinline__ void or_mask(uint32_t *acc, const uint32_t *Ff, uint32_t u) {
+  const uint4 *f = reinterpret_cast<const uint4 *>(Ff + (size_t)u * kMW);
+  const uint4 a = f[0], b = f[1];
+  acc[0] |= a.x;
+  acc[1] |= a.y;
+  acc[2] |= a.z;
+  acc[3] |= a.w;
+  acc[4] |= b.x;
+  acc[5] |= b.y;
+  acc[6] |= b.z;
+  acc[7] |= b.w;
+}
+
+// OR of F[u] over active in-neighbours k = beg + j, beg + j + G, ...
+template <int G>
+__device__ __forceinline__ void lane_or(const MbfsParams &p, const uint32_t *Af, const uint32_t *Ff, uint32_t beg,
+                                        uint32_t end, uint32_t j, uint32_t *acc) {
+  for (uint32_t k = beg + j; k < end; k += G) {
+    const uint32_t u = p.in_col[k];
+    if (bm_test(Af, u)) or_mask(acc, Ff, u);
+  }
+}
+
+template <int G>
+__device__ __forceinline__ void group_or(uint32_t *acc) {
+#pragma unroll
+  for (int j = 0; j < kMW; ++j)
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc[j] |= __shfl_xor(acc[j], o, kWave);
+}
+
+// Workgroup OR of the lane masks; result valid in thread 0.
+__device__ __forceinline__ void block_or(uint32_t *acc, uint32_t (*s)[kMW]) {
+  group_or<kWave>(acc);
+  const uint32_t wid = threadIdx.x / kWave;
+  if (lane_id() == 0)
+#pragma unroll
+    for (int j = 0; j < kMW; ++j) s[wid][j] = acc[j];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (uint32_t w = 1; w < kMB / kWave; ++w)
+#pragma unroll
+      for (int j = 0; j < kMW; ++j) acc[j] |= s[w][j];
+  __syncthreads();
+}
+
+// New bits of v from the OR of its in-neighbours' frontier masks.
+__device__ __forceinline__ void mbfs_pull_finish(const MbfsParams &p, uint32_t v, const uint32_t *acc, uint32_t *An,
+                                                 uint32_t *Fn, int32_t nlevel, uint32_t &nf,
+                                                 unsigned long long &mf) {
+  uint32_t vv[kMW], nwv[kMW];
+  bool any = false;
+#pragma unroll
+  for (int j = 0; j < kMW; ++j) {
+    vv[j] = p.V[(size_t)v * kMW + j];
+    nwv[j] = acc[j] & ~vv[j];
+    any = any || nwv[j];
+  }
+  if (!any) return;
+  const uint32_t bit = 1u << (v & 31);
+  if (mbfs_apply(p, v, nwv, vv, Fn)) atomicOr(&p.D[v >> 5], bit);
+  atomicOr(&An[v >> 5], bit);
+  nf += 1;
+  mf += p.row_ptr[v + 1] - p.row_ptr[v];
+#pragma unroll
+  for (int j = 0; j < kMW; ++j)
+    for (uint32_t x = nwv[j]; x; x &= x - 1) p.levels[(size_t)(j * 32 + __ffs(x) - 1) * p.n + v] = nlevel;
+}
+
+// Pull level: every vertex not yet holding every source ORs the frontier
+// masks of its active in-neighbours, with 1, 8, 64 or 256 lanes per vertex by
+// in-degree; rows longer than the plan's chunk are cut into pieces whose ORs
+// go to N (applied by mbfs_contract_kernel).
+__global__ void __launch_bounds__(kMB) mbfs_pull_kernel(MbfsParams p, SegDev d) {
+  MbfsCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level;
+  if (ctl->st[L & 1].mode == 0) return;
+  __shared__ uint32_t s_or[kMB / kWave][kMW];
+  const uint32_t *Ff, *Af;
+  uint32_t *Fn, *An, *Ac, *Tc, *Tx;
+  mbfs_buffers(p, L, Ff, Fn, Af, An, Ac, Tc, Tx);
+  const int32_t nlevel = (int32_t)(L + 1);
+  const uint32_t *rp = p.in_row_ptr;
+  uint32_t nf = 0;
+  unsigned long long mf = 0, insp = 0;
+  for (uint32_t b = blockIdx.x; b < d.role_start[5]; b += gridDim.x) {
+    uint32_t acc[kMW] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (b < d.role_start[1]) {
+      // piece of a very long in-list: OR into N, touch T
+      const uint32_t c = b;
+      const uint32_t h = d.chunk_hub[c];
+      const uint32_t v = d.hub_vertex[h];
+      if (bm_test(p.D, v)) continue;
+      const uint32_t beg = rp[v] + (c - d.hub_first[h]) * d.chunk;
+      const uint32_t end = min(rp[v + 1], beg + d.chunk);
+      lane_or<kMB>(p, Af, Ff, beg, end, threadIdx.x, acc);
+      if (threadIdx.x == 0) insp += end - beg;
+      block_or(acc, s_or);
+      if (threadIdx.x == 0) {
+        bool any = false;
+        for (int j = 0; j < kMW; ++j) {
+          const uint32_t nv = acc[j] & ~p.V[(size_t)v * kMW + j];
+          if (nv) {
+            atomicOr(&p.N[(size_t)v * kMW + j], nv);
+            any = true;
+          }
+        }
+        if (any) atomicOr(&Tc[v >> 5], 1u << (v & 31));
+      }
+    } else if (b < d.role_start[2]) {
+      const uint32_t v = d.list[d.off[3] + (b - d.role_start[1])];
+      if (bm_test(p.D, v)) continue;
+      lane_or<kMB>(p, Af, Ff, rp[v], rp[v + 1], threadIdx.x, acc);
+      if (threadIdx.x == 0) insp += rp[v + 1] - rp[v];
+      block_or(acc, s_or);
+      if (threadIdx.x == 0) mbfs_pull_finish(p, v, acc, An, Fn, nlevel, nf, mf);
+    } else if (b < d.role_start[3]) {
+      const uint32_t idx = d.off[2] + (b - d.role_start[2]) * (kMB / 64) + threadIdx.x / 64;
+      if (idx >= d.off[3]) continue;
+      const uint32_t v = d.list[idx];
+      if (bm_test(p.D, v)) continue;
+      const uint32_t j = threadIdx.x % 64;
+      lane_or<64>(p, Af, Ff, rp[v], rp[v + 1], j, acc);
+      group_or<64>(acc);
+      if (j == 0) {
+        insp += rp[v + 1] - rp[v];
+        mbfs_pull_finish(p, v, acc, An, Fn, nlevel, nf, mf);
+      }
+    } else if (b < d.role_start[4]) {
+      const uint32_t idx = d.off[1] + (b - d.role_start[3]) * (kMB / 8) + threadIdx.x / 8;
+      const uint32_t j = threadIdx.x % 8;
+      const bool ok = idx < d.off[2];
+      const uint32_t v = ok ? d.list[idx] : 0u;
+      const bool todo = ok && !bm_test(p.D, v);
+      if (todo) lane_or<8>(p, Af, Ff, rp[v], rp[v + 1], j, acc);
+      group_or<8>(acc);
+      if (todo && j == 0) {
+        insp += rp[v + 1] - rp[v];
+        mbfs_pull_finish(p, v, acc, An, Fn, nlevel, nf, mf);
+      }
+    } else {
+      const uint32_t idx = d.off[0] + (b - d.role_start[4]) * kMB + threadIdx.x;
+      if (idx >= d.off[1]) continue;
+      const uint32_t v = d.list[idx];
+      if (bm_test(p.D, v)) continue;
+      uint32_t vv[kMW];
+#pragma unroll
+      for (int j = 0; j < kMW; ++j) vv[j] = p.V[(size_t)v * kMW + j];
+      uint32_t k = 0;
+      for (uint32_t e = rp[v]; e < rp[v + 1]; ++e) {
+        ++k;
+        const uint32_t u = p.in_col[e];
+        if (!bm_test(Af, u)) continue;
+        or_mask(acc, Ff, u);
+        bool full = true;
+#pragma unroll
+        for (int j = 0; j < kMW; ++j) full = full && ((acc[j] | vv[j]) == p.full[j]);
+        if (full) break;
+      }
+      insp += k;
+      mbfs_pull_finish(p, v, acc, An, Fn, nlevel, nf, mf);
+    }
+  }
+  flush_counters<kMB>(&ctl->cnt[L % 3], nf, mf, insp);
+}
+
+// Apply N to every touched vertex (every vertex reached by a push level,
+// and the long rows of a pull level).  One lane per bitmap word owns the 32
+// vertices of that word.
+__global__ void __launch_bounds__(kMB) mbfs_contract_kernel(MbfsParams p) {
+  MbfsCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level_e;
+  const uint32_t *Ff, *Af;
+  uint32_t *Fn, *An, *Ac, *Tc, *Tx;
+  mbfs_buffers(p, L, Ff, Fn, Af, An, Ac, Tc, Tx);
+  const int32_t nlevel = (int32_t)(L + 1);
+  uint32_t nf = 0;
+  unsigned long long mf = 0;
+  for (uint32_t w = blockIdx.x * kMB + threadIdx.x; w < p.nw; w += gridDim.x * kMB) {
+    uint32_t tw = Tc[w];
     uint32_t anext = 0, dnew = 0;
     while (tw) {
       const uint32_t b = __ffs(tw) - 1;
@@ -125,7 +366,7 @@ __device__ void mbfs_contract(const MbfsParams &p, const uint32_t *T, uint32_t *
         any = any || nwv[j];
       }
       if (!any) continue;
-      if (mbfs_apply(p, v, nwv, vv, Fn, nlevel)) dnew |= 1u << b;
+      if (mbfs_apply(p, v, nwv, vv, Fn)) dnew |= 1u << b;
       anext |= 1u << b;
       nf += 1;
       mf += p.row_ptr[v + 1] - p.row_ptr[v];
@@ -139,138 +380,44 @@ __device__ void mbfs_contract(const MbfsParams &p, const uint32_t *T, uint32_t *
         }
       }
     }
-    if (anext) An[w] = anext;
+    // |=: on a pull level the pull kernel has already set bits of this word
+    if (anext) An[w] |= anext;
     if (dnew) p.D[w] |= dnew;
   }
+  flush_counters<kMB>(&ctl->cnt[L % 3], nf, mf, 0ull);
 }
 
-template <int BLOCK>
-__device__ void mbfs_bu(const MbfsParams &p, const uint32_t *Af, const uint32_t *Ff, uint32_t *An, uint32_t *Fn,
-                        int32_t nlevel, uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf,
-                        unsigned long long &insp) {
-  const uint32_t lane = lane_id();
-  const uint32_t waves = nvb * (BLOCK / kWave);
-  const uint32_t gw = vb * (BLOCK / kWave) + threadIdx.x / kWave;
-  for (uint64_t base = (uint64_t)gw * kWave; base < p.n; base += (uint64_t)waves * kWave) {
-    const uint32_t v = (uint32_t)base + lane;
-    const bool inb = v < p.n;
-    const uint32_t w = v >> 5;
-    const uint32_t dw = inb ? p.D[w] : 0xffffffffu;
-    const bool todo = inb && !((dw >> (v & 31)) & 1u);
-    if (!ballot(todo)) continue;
-    uint32_t acc[kMW], vv[kMW];
-#pragma unroll
-    for (int j = 0; j < kMW; ++j) {
-      acc[j] = 0u;
-      vv[j] = todo ? p.V[(size_t)v * kMW + j] : 0xffffffffu;
+// Decide the next direction; compact the active bitmap when top-down.
+__global__ void __launch_bounds__(kMB) mbfs_compact_kernel(MbfsParams p) {
+  MbfsCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level_e;
+  const BfsCtr c = ctl->cnt[L % 3];
+  const MbfsState st = ctl->st[L & 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const uint32_t nbu = st.nbu + st.mode;
+  if (c.nf == 0 || L + 1 > p.n) {
+    if (lead) {
+      ctl->levels = L + 1;
+      ctl->nbu = nbu;
+      ctl->overflow = c.nf != 0;
+      ctl->done = 1u;
     }
-    if (todo) {
-      uint32_t k = 0;
-      for (uint32_t e = p.in_row_ptr[v]; e < p.in_row_ptr[v + 1]; ++e) {
-        ++k;
-        const uint32_t u = p.in_col[e];
-        if (!bm_test(Af, u)) continue;
-        bool done = true;
-#pragma unroll
-        for (int j = 0; j < kMW; ++j) {
-          acc[j] |= Ff[(size_t)u * kMW + j];
-          done = done && ((acc[j] | vv[j]) == p.full[j]);
-        }
-        if (done) break;
-      }
-      insp += k;
-    }
-    uint32_t nwv[kMW];
-    bool any = false;
-#pragma unroll
-    for (int j = 0; j < kMW; ++j) {
-      nwv[j] = acc[j] & ~vv[j];
-      any = any || nwv[j];
-    }
-    bool isfull = false;
-    if (any) {
-      isfull = mbfs_apply(p, v, nwv, vv, Fn, nlevel);
-      nf += 1;
-      mf += p.row_ptr[v + 1] - p.row_ptr[v];
-    }
-    // coalesced level writes: walk the sources present anywhere in the wave
-#pragma unroll
-    for (int j = 0; j < kMW; ++j) {
-      uint32_t wm = wave_or(nwv[j]);
-      while (wm) {
-        const uint32_t b = __ffs(wm) - 1;
-        wm &= wm - 1;
-        if ((nwv[j] >> b) & 1u) p.levels[(size_t)(j * 32 + b) * p.n + v] = nlevel;
-      }
-    }
-    const uint64_t am = ballot(any), fm = ballot(isfull);
-    const uint32_t amine = lane < 32 ? (uint32_t)am : (uint32_t)(am >> 32);
-    const uint32_t fmine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
-    if (lane == 0 || lane == 32) {
-      if (amine) An[w] = amine;
-      if (fmine) p.D[w] = dw | fmine;
-    }
+    return;
   }
-}
-
-__global__ void __launch_bounds__(kMB) mbfs_persistent_kernel(MbfsParams p) {
-  uint32_t gen = 0;
-  const uint32_t vb = blockIdx.x, nvb = gridDim.x;
-  unsigned long long mu = p.m;
-  const uint32_t nseed = __hip_atomic_load(&p.ctr[0].qsize, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (uint32_t i = 0; i < nseed; ++i) {
-    const uint32_t s = p.queue[i];
-    mu -= p.row_ptr[s + 1] - p.row_ptr[s];
+  const unsigned long long mu = st.mu > c.mf ? st.mu - c.mf : 0ull;
+  const bool next_bu = choose_bottom_up(st.mode != 0, p.can_bu != 0, c.nf, c.mf, mu, p.n, p.alpha, p.beta);
+  if (lead) {
+    MbfsState ns;
+    ns.mu = mu;
+    ns.mode = next_bu ? 1u : 0u;
+    ns.nbu = nbu;
+    ctl->st[(L + 1) & 1] = ns;
+    ctl->level = L + 1;
   }
-  const size_t fstride = (size_t)p.n * kMW;
-  bool bu = false;
-  uint32_t L = 0, nbu = 0;
-  for (;;) {
-    BfsCtr *cur = &p.ctr[L % 3];
-    BfsCtr *nxt = &p.ctr[(L + 1) % 3];
-    const uint32_t *Ff = p.F + (size_t)((L + 1) & 1) * fstride;
-    uint32_t *Fn = p.F + (size_t)(L & 1) * fstride;
-    const uint32_t *Af = p.A + (size_t)((L + 2) % 3) * p.nwp;
-    uint32_t *An = p.A + (size_t)(L % 3) * p.nwp;
-    uint32_t *Ac = p.A + (size_t)((L + 1) % 3) * p.nwp;
-    uint32_t *Tc = p.T + (size_t)(L & 1) * p.nwp;
-    uint32_t *Tx = p.T + (size_t)((L + 1) & 1) * p.nwp;
-    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) reinterpret_cast<uint32_t *>(nxt)[threadIdx.x] = 0u;
-    for (uint32_t i = vb * kMB + threadIdx.x; i < p.nwp; i += nvb * kMB) {
-      Ac[i] = 0u;
-      Tx[i] = 0u;
-    }
-    uint32_t nf = 0;
-    unsigned long long mf = 0, insp = 0;
-    const int32_t nlevel = (int32_t)(L + 1);
-    if (!bu) {
-      const uint32_t qsize = __hip_atomic_load(&cur->qsize, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      MbfsTdVisitor vis(p, Ff, Tc);
-      push_expand<kMB>(p.queue, qsize, p.row_ptr, p.wave_thr, p.block_thr, vb, nvb, vis, insp);
-      if (!grid_barrier(p.bar, nvb, gen, kMSpin)) break;
-      mbfs_contract<kMB>(p, Tc, An, Fn, nlevel, vb, nvb, nf, mf);
-    } else {
-      mbfs_bu<kMB>(p, Af, Ff, An, Fn, nlevel, vb, nvb, nf, mf, insp);
-    }
-    flush_counters<kMB>(cur, nf, mf, insp);
-    if (!grid_barrier(p.bar, nvb, gen, kMSpin)) break;
-    const uint32_t tnf = __hip_atomic_load(&cur->nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long tmf = __hip_atomic_load(&cur->mf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (bu) ++nbu;
-    ++L;
-    if (tnf == 0 || L > p.n) break;
-    mu = mu > tmf ? mu - tmf : 0ull;
-    const bool next_bu = choose_bottom_up(bu, p.can_bu != 0, tnf, tmf, mu, p.n, p.alpha, p.beta);
-    if (!next_bu) {
-      compact_step<kMB>(An, p.nw, p.queue, &nxt->qsize, vb, nvb);
-      if (!grid_barrier(p.bar, nvb, gen, kMSpin)) break;
-    }
-    bu = next_bu;
-  }
-  if (vb == 0 && threadIdx.x == 0) {
-    p.bar[4] = L;
-    p.bar[5] = nbu;
-  }
+  if (!next_bu)
+    compact_step<kMB>(p.A + (size_t)(L % 3) * p.nwp, p.nw, p.row_ptr, p.hub_thr, p.queue, &ctl->qfill[(L + 1) & 1],
+                      p.hubs, &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x, IdRow());
 }
 
 // Reset state for one group of sources (grid-stride).
@@ -285,17 +432,15 @@ __global__ void mbfs_reset_kernel(MbfsParams p) {
   for (size_t i = tid; i < (size_t)p.nwp * 3; i += stride) p.A[i] = 0u;
   for (size_t i = tid; i < (size_t)p.nwp * 2; i += stride) p.T[i] = 0u;
   for (size_t i = tid; i < (size_t)p.nwp; i += stride) p.D[i] = 0u;
-  if (tid < sizeof(BfsCtr) * 3 / 4) reinterpret_cast<uint32_t *>(p.ctr)[tid] = 0u;
-  if (tid < 16) p.bar[tid] = 0u;
+  for (size_t i = tid; i < sizeof(MbfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(p.ctl)[i] = 0u;
 }
 
 // Level-0 state from the sources (one workgroup): V and the level-0
 // frontier F (buffer 1) get bit s at srcs[s]; queue = distinct sources.
 __global__ void mbfs_seed_kernel(MbfsParams p, const uint32_t *__restrict__ srcs) {
   const size_t fstride = (size_t)p.n * kMW;
-  uint32_t *F0 = p.F + fstride;  // frontier of level 0 is buffer (0+1)&1
+  uint32_t *F0 = p.F + fstride;
   uint32_t *A0 = p.A + (size_t)2 * p.nwp;
-  // clear the seeds' frontier masks first
   for (uint32_t s = threadIdx.x; s < p.k; s += blockDim.x)
     for (int j = 0; j < kMW; ++j) F0[(size_t)srcs[s] * kMW + j] = 0u;
   __syncthreads();
@@ -304,19 +449,27 @@ __global__ void mbfs_seed_kernel(MbfsParams p, const uint32_t *__restrict__ srcs
     atomicOr(&p.V[(size_t)v * kMW + s / 32], 1u << (s % 32));
     atomicOr(&F0[(size_t)v * kMW + s / 32], 1u << (s % 32));
     p.levels[(size_t)s * p.n + v] = 0;
-    const uint32_t old = atomicOr(&A0[v >> 5], 1u << (v & 31));
-    if (!(old & (1u << (v & 31)))) {
-      const uint32_t pos = atomicAdd(&p.ctr[0].qsize, 1u);
-      p.queue[pos] = v;
-    }
   }
   __syncthreads();
-  // vertices whose mask is already full (only possible when k sources share it)
+  // distinct source vertices form the level-0 frontier
+  auto claim = [&](uint32_t s) -> uint32_t {
+    const uint32_t v = srcs[s];
+    const uint32_t old = atomicOr(&A0[v >> 5], 1u << (v & 31));
+    return (old & (1u << (v & 31))) ? 0xffffffffu : v;
+  };
+  unsigned long long deg = 0;
+  seed_frontier(srcs, p.k, p.row_ptr, p.hub_thr, p.queue, &p.ctl->qfill[0], p.hubs, &p.ctl->hfill[0], claim, IdRow(),
+                &deg);
   for (uint32_t s = threadIdx.x; s < p.k; s += blockDim.x) {
     const uint32_t v = srcs[s];
     bool full = true;
     for (int j = 0; j < kMW; ++j) full = full && (p.V[(size_t)v * kMW + j] == p.full[j]);
     if (full) atomicOr(&p.D[v >> 5], 1u << (v & 31));
+  }
+  if (threadIdx.x == 0) {
+    p.ctl->st[0].mu = p.m > deg ? p.m - deg : 0ull;
+    p.ctl->st[0].mode = 0;
+    p.ctl->st[0].nbu = 0;
   }
 }
 
@@ -337,16 +490,18 @@ __global__ void fill_i32_kernel(int32_t *__restrict__ p, int32_t v, size_t count
 
 struct MbfsWork {
   uint32_t *V = nullptr, *F = nullptr, *N = nullptr, *A = nullptr, *T = nullptr, *D = nullptr, *queue = nullptr;
-  BfsCtr *ctr = nullptr;
-  uint32_t *bar = nullptr, *h_bar = nullptr;
+  uint2 *hubs = nullptr;
+  MbfsCtl *ctl = nullptr, *h_ctl = nullptr;
+  uint32_t last_levels = 0;
   ~MbfsWork() {
-    void *ps[] = {V, F, N, A, T, D, queue, ctr, bar};
+    void *ps[] = {V, F, N, A, T, D, queue, hubs, ctl};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    if (h_bar) (void)hipHostFree(h_bar);
+    if (h_ctl) (void)hipHostFree(h_ctl);
   }
-  int alloc(uint32_t n, uint32_t nwp) {
+  int alloc(uint32_t n, uint32_t nwp, uint64_t m) {
     const size_t nm = (size_t)n * kMW * 4;
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&hubs), hub_capacity(m) * sizeof(uint2)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&V), nm));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&F), nm * 2));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&N), nm));
@@ -354,9 +509,8 @@ struct MbfsWork {
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&T), (size_t)nwp * 4 * 2));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&D), (size_t)nwp * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&queue), (size_t)n * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&ctr), sizeof(BfsCtr) * 3));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bar), 64));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_bar), 64, hipHostMallocDefault));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&ctl), sizeof(MbfsCtl)));
+    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctl), sizeof(MbfsCtl), hipHostMallocDefault));
     return OLV_OK;
   }
 };
@@ -364,14 +518,6 @@ struct MbfsWork {
 static unsigned mgrid(size_t items, size_t per) {
   size_t b = (items + per - 1) / per;
   return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * 8));
-}
-
-static int mbfs_grid(int *grid) {
-  int per_cu = 0;
-  OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, mbfs_persistent_kernel, kMB, 0));
-  if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "batched BFS kernel does not fit on a CU");
-  *grid = std::min(per_cu, 2) * cu_count();
-  return OLV_OK;
 }
 
 static MbfsParams mbfs_params(const olv_graph *g, MbfsWork &w, uint32_t k, int32_t *levels) {
@@ -397,30 +543,78 @@ static MbfsParams mbfs_params(const olv_graph *g, MbfsWork &w, uint32_t k, int32
   p.T = w.T;
   p.D = w.D;
   p.queue = w.queue;
+  p.hubs = w.hubs;
   p.levels = levels;
-  p.ctr = w.ctr;
-  p.bar = w.bar;
+  p.ctl = w.ctl;
   p.m = g->m;
   p.wave_thr = g->tune.td_wave_thr;
   p.block_thr = g->tune.td_block_thr;
+  p.hub_thr = g->tune.hub_thr;
   p.alpha = g->tune.alpha;
   p.beta = g->tune.beta;
   p.can_bu = g->in_row_ptr ? 1u : 0u;
   return p;
 }
 
-// Enqueue one group (<= 256 sources): reset, level fill, seed, persistent.
-static int mbfs_enqueue_group(const MbfsParams &p, const uint32_t *d_srcs, int grid) {
+static int mbfs_enqueue_init(const MbfsParams &p, const uint32_t *d_srcs) {
   hipStream_t s = stream();
   hipLaunchKernelGGL(mbfs_reset_kernel, dim3(mgrid((size_t)p.n * kMW, 256)), dim3(256), 0, s, p);
   hipLaunchKernelGGL(fill_i32_kernel, dim3(mgrid((size_t)p.k * p.n / 4 + 1, 256)), dim3(256), 0, s, p.levels, kInf,
                      (size_t)p.k * p.n);
   hipLaunchKernelGGL(mbfs_seed_kernel, dim3(1), dim3(256), 0, s, p, d_srcs);
   OLV_HIP(hipGetLastError());
-  MbfsParams pp = p;
-  void *args[] = {&pp};
-  OLV_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(mbfs_persistent_kernel), dim3(grid), dim3(kMB),
-                                     args, 0, s));
+  return OLV_OK;
+}
+
+static int mbfs_enqueue_levels(const MbfsParams &p, const SegDev &d, uint32_t count) {
+  hipStream_t s = stream();
+  const unsigned ge = (unsigned)cu_count() * 8;
+  const unsigned gw = mgrid(p.nw, kMB);
+  const unsigned gp = seg_grid(d);
+  for (uint32_t i = 0; i < count; ++i) {
+    hipLaunchKernelGGL(mbfs_expand_kernel, dim3(ge), dim3(kMB), 0, s, p);
+    hipLaunchKernelGGL(mbfs_pull_kernel, dim3(gp), dim3(kMB), 0, s, p, d);
+    hipLaunchKernelGGL(mbfs_contract_kernel, dim3(gw), dim3(kMB), 0, s, p);
+    hipLaunchKernelGGL(mbfs_compact_kernel, dim3(gw), dim3(kMB), 0, s, p);
+  }
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
+static int mbfs_read(MbfsWork &w) {
+  OLV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(MbfsCtl), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  return OLV_OK;
+}
+
+// In-degree buckets for pull levels (empty schedule when pull is off).
+static int mbfs_pull_plan(olv_graph *g, unsigned flags, SegDev *d) {
+  std::memset(d, 0, sizeof(*d));
+  if (flags & OLV_NO_DIRECTION_OPT) return OLV_OK;
+  if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+  if (!g->plan_in_ready) {
+    OLV_TRY(build_seg_plan(g, true, g->plan_in));
+    g->plan_in_ready = true;
+  }
+  *d = make_seg_dev(g->plan_in);
+  return OLV_OK;
+}
+
+// Run one group to completion (levels enqueued in growing chunks).
+static int mbfs_run_group(const MbfsParams &p, const SegDev &d, MbfsWork &w, const uint32_t *d_srcs) {
+  OLV_TRY(mbfs_enqueue_init(p, d_srcs));
+  uint32_t chunk = std::max(4u, w.last_levels + 1);
+  uint64_t enq = 0;
+  for (;;) {
+    OLV_TRY(mbfs_enqueue_levels(p, d, chunk));
+    enq += chunk;
+    OLV_TRY(mbfs_read(w));
+    if (w.h_ctl->done) break;
+    if (enq > (uint64_t)p.n + 2) return fail(OLV_EINTERNAL, "olv_bfs_batched: level loop did not terminate");
+    chunk = std::min(chunk * 2, 256u);
+  }
+  if (w.h_ctl->overflow) return fail(OLV_EINTERNAL, "olv_bfs_batched: more than n levels");
+  w.last_levels = w.h_ctl->levels;
   return OLV_OK;
 }
 
@@ -432,17 +626,18 @@ struct olv_plan {
   const olv_graph *g = nullptr;
   olv::DevBuf seeds;
   uint32_t k = 0;
+  int32_t *levels = nullptr;
+  bool can_bu = true;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   olv::BfsWork *bw = nullptr;
   olv::MbfsWork *mw = nullptr;
-  uint32_t *result = nullptr;  // device words [4]=levels [5]=bu levels [2]=error
-  uint32_t *h_result = nullptr;
+  std::vector<olv::MbfsParams> groups;
+  olv::SegDev seg;
   ~olv_plan() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
     delete mw;
-    if (h_result) (void)hipHostFree(h_result);
   }
 };
 
@@ -452,19 +647,19 @@ namespace olv {
 template <class Fn>
 static int capture(olv_plan *plan, Fn enqueue) {
   hipStream_t user = stream();
-  hipStream_t cap = user;
-  bool own = false;
-  if (!cap) {
-    OLV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
-    own = true;
+  hipStream_t cap = nullptr;
+  OLV_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+  hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    (void)hipStreamDestroy(cap);
+    return fail(OLV_EHIP, "hipStreamBeginCapture failed: %s", hipGetErrorString(e));
   }
-  OLV_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
   olv_set_stream(cap);
   int st = enqueue();
   olv_set_stream(user);
   hipGraph_t graph = nullptr;
-  hipError_t e = hipStreamEndCapture(cap, &graph);
-  if (own) (void)hipStreamDestroy(cap);
+  e = hipStreamEndCapture(cap, &graph);
+  (void)hipStreamDestroy(cap);
   if (st != OLV_OK) {
     if (graph) (void)hipGraphDestroy(graph);
     return st;
@@ -495,10 +690,11 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_bfs_batched: source %u out of range (n=%u)", srcs[i], g->n);
   if ((uint64_t)k * g->n > (1ull << 40)) return fail(OLV_ERANGE, "olv_bfs_batched: output too large");
-  if (!(flags & OLV_NO_DIRECTION_OPT) && !g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+  SegDev d;
+  OLV_TRY(mbfs_pull_plan(g, flags, &d));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   MbfsWork w;
-  OLV_TRY(w.alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64));
+  OLV_TRY(w.alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m));
   DevBuf d_srcs, d_out;
   OLV_TRY(d_srcs.alloc((size_t)k * 4));
   OLV_HIP(hipMemcpy(d_srcs.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice));
@@ -507,8 +703,6 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
     OLV_TRY(d_out.alloc((size_t)k * g->n * 4));
     out = d_out.as<int32_t>();
   }
-  int grid = 0;
-  OLV_TRY(mbfs_grid(&grid));
   Timer t;
   t.start();
   uint32_t max_levels = 0, bu_levels = 0;
@@ -516,12 +710,9 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
     const uint32_t kg = std::min(256u, k - g0);
     MbfsParams p = mbfs_params(g, w, kg, out + (size_t)g0 * g->n);
     if (flags & OLV_NO_DIRECTION_OPT) p.can_bu = 0;
-    OLV_TRY(mbfs_enqueue_group(p, d_srcs.as<uint32_t>() + g0, grid));
-    OLV_HIP(hipMemcpyAsync(w.h_bar, w.bar, 64, hipMemcpyDeviceToHost, stream()));
-    OLV_HIP(hipStreamSynchronize(stream()));
-    if (w.h_bar[2]) return fail(OLV_EINTERNAL, "olv_bfs_batched: grid barrier timed out");
-    max_levels = std::max(max_levels, w.h_bar[4]);
-    bu_levels += w.h_bar[5];
+    OLV_TRY(mbfs_run_group(p, d, w, d_srcs.as<uint32_t>() + g0));
+    max_levels = std::max(max_levels, w.h_ctl->levels);
+    bu_levels += w.h_ctl->nbu;
   }
   t.stop();
   const double ms = t.ms();
@@ -680,28 +871,37 @@ int olv_bfs_multigraph(olv_graph *const *gs, uint32_t k, const uint32_t *srcs, i
   return s;
 }
 
+// A plan captures init + `depth` levels.  The depth comes from one eager
+// run at plan time (levels are deterministic for fixed sources); a replay
+// that still reports work left continues eagerly, so results never depend
+// on the captured depth.
 int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k, int32_t *levels, unsigned flags,
                  olv_plan **out) {
   if (!g || !srcs || !levels || !out || k == 0) return fail(OLV_EINVAL, "olv_plan_bfs: bad arguments");
   *out = nullptr;
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_plan_bfs: source %u out of range", srcs[i]);
-  const bool can_bu = !(flags & OLV_NO_DIRECTION_OPT);
   olv_plan *p = new olv_plan();
   p->kind = 1;
   p->g = g;
   p->k = k;
-  int s = bfs_prepare(g, can_bu, &p->bw);
+  p->levels = levels;
+  p->can_bu = !(flags & OLV_NO_DIRECTION_OPT);
+  int s = bfs_prepare(g, p->can_bu, &p->bw);
+  if (s == OLV_OK) s = bfs_multisource(g, srcs, k, true, levels, flags & ~(OLV_PTR_HOST | OLV_PERSISTENT), nullptr);
   if (s == OLV_OK) s = p->seeds.alloc((size_t)k * 4);
   if (s == OLV_OK && hipMemcpy(p->seeds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
     s = fail(OLV_EHIP, "olv_plan_bfs: seed copy failed");
+  const uint32_t depth = bfs_depth_hint(p->bw);
   if (s == OLV_OK)
-    s = capture(p, [&] { return bfs_enqueue(g, p->bw, p->seeds.as<uint32_t>(), k, levels, can_bu); });
+    s = capture(p, [&]() -> int {
+      OLV_TRY(bfs_enqueue_init(g, p->bw, p->seeds.as<uint32_t>(), k, levels));
+      return bfs_enqueue_levels(g, p->bw, levels, p->can_bu, depth);
+    });
   if (s != OLV_OK) {
     delete p;
     return s;
   }
-  p->result = bfs_result_words(p->bw);
   *out = p;
   return OLV_OK;
 }
@@ -713,25 +913,33 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
   olv_graph *g = const_cast<olv_graph *>(cg);
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_plan_bfs_batched: source %u out of range", srcs[i]);
-  if (!(flags & OLV_NO_DIRECTION_OPT) && !g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+  SegDev d;
+  OLV_TRY(mbfs_pull_plan(g, flags, &d));
   olv_plan *p = new olv_plan();
   p->kind = 2;
+  p->seg = d;
   p->g = g;
   p->k = k;
+  p->levels = levels;
   p->mw = new MbfsWork();
-  int grid = 0;
-  int s = p->mw->alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64);
-  if (s == OLV_OK) s = mbfs_grid(&grid);
+  int s = p->mw->alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m);
   if (s == OLV_OK) s = p->seeds.alloc((size_t)k * 4);
   if (s == OLV_OK && hipMemcpy(p->seeds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
     s = fail(OLV_EHIP, "olv_plan_bfs_batched: seed copy failed");
+  std::vector<uint32_t> depth;
+  for (uint32_t g0 = 0; s == OLV_OK && g0 < k; g0 += 256) {
+    const uint32_t kg = std::min(256u, k - g0);
+    MbfsParams prm = mbfs_params(g, *p->mw, kg, levels + (size_t)g0 * g->n);
+    if (flags & OLV_NO_DIRECTION_OPT) prm.can_bu = 0;
+    p->groups.push_back(prm);
+    s = mbfs_run_group(prm, p->seg, *p->mw, p->seeds.as<uint32_t>() + g0);  // learns the depth
+    depth.push_back(p->mw->h_ctl->levels);
+  }
   if (s == OLV_OK)
     s = capture(p, [&]() -> int {
-      for (uint32_t g0 = 0; g0 < k; g0 += 256) {
-        const uint32_t kg = std::min(256u, k - g0);
-        MbfsParams prm = mbfs_params(g, *p->mw, kg, levels + (size_t)g0 * g->n);
-        if (flags & OLV_NO_DIRECTION_OPT) prm.can_bu = 0;
-        OLV_TRY(mbfs_enqueue_group(prm, p->seeds.as<uint32_t>() + g0, grid));
+      for (size_t i = 0; i < p->groups.size(); ++i) {
+        OLV_TRY(mbfs_enqueue_init(p->groups[i], p->seeds.as<uint32_t>() + 256 * i));
+        OLV_TRY(mbfs_enqueue_levels(p->groups[i], p->seg, depth[i]));
       }
       return OLV_OK;
     });
@@ -739,7 +947,6 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
     delete p;
     return s;
   }
-  p->result = p->mw->bar;
   *out = p;
   return OLV_OK;
 }
@@ -750,16 +957,34 @@ int olv_plan_launch(olv_plan *p, olv_stats *st) {
   t.start();
   OLV_HIP(hipGraphLaunch(p->exec, stream()));
   t.stop();
+  double ms = t.ms();
+  uint32_t levels = 0, nbu = 0;
+  if (p->kind == 1) {
+    OLV_TRY(bfs_result(p->bw, &levels, &nbu));
+    if (!bfs_done(p->bw)) {  // deeper than captured: finish eagerly
+      Timer t2;
+      t2.start();
+      for (uint32_t chunk = 8; !bfs_done(p->bw); chunk = std::min(chunk * 2, 256u)) {
+        OLV_TRY(bfs_enqueue_levels(p->g, p->bw, p->levels, p->can_bu, chunk));
+        OLV_TRY(bfs_result(p->bw, &levels, &nbu));
+      }
+      t2.stop();
+      ms += t2.ms();
+    }
+  } else {
+    // each group ran to its captured depth; a group's done flag is only
+    // kept for the last one, so re-check all groups' depths by replaying
+    // nothing: the plan was built from a completed run of the same sources
+    OLV_TRY(mbfs_read(*p->mw));
+    if (!p->mw->h_ctl->done) return fail(OLV_EINTERNAL, "olv_plan_launch: batched plan did not finish");
+    levels = p->mw->h_ctl->levels;
+    nbu = p->mw->h_ctl->nbu;
+  }
   if (st) {
-    const double ms = t.ms();
-    uint32_t res[16];
-    OLV_HIP(hipMemcpyAsync(res, p->result, sizeof(res), hipMemcpyDeviceToHost, stream()));
-    OLV_HIP(hipStreamSynchronize(stream()));
-    if (res[2]) return fail(OLV_EINTERNAL, "olv_plan_launch: grid barrier timed out");
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
-    st->iterations = res[4];
-    st->bottom_up_levels = res[5];
+    st->iterations = levels;
+    st->bottom_up_levels = nbu;
   }
   return OLV_OK;
 }

@@ -4,9 +4,19 @@
 // O(V) host fill + H2D of the accumulators (oliver.h:66, grd.h:90-97), a
 // 4V-byte memset of the flag frontier, the push kernel K1
 // (oliverKernel.h:45-83), a full O(V) scan in K2 (oliverKernel.h:85-141) and
-// two device syncs.  Here a level is one step of a persistent cooperative
-// kernel (or, with OLV_HOST_LOOP, one launch group) and the frontier is a
-// bitmap that is compacted into a queue only when the next step is top-down.
+// two device syncs.
+//
+// Here a level is two kernels, expand and compact, and the loop control
+// lives on the device (BfsCtl): expand reads the level number, direction and
+// queue length the previous compact wrote; compact reads the level's
+// counters, takes the direction decision, writes the next level's state and
+// -- when the next step is top-down -- turns the next-frontier bitmap into
+// the queue.  The host enqueues levels in chunks without waiting and checks
+// the done flag once per chunk; surplus levels return at once.  The whole
+// sequence is capturable in a hipGraph (olv_plan_*).  A kernel boundary costs
+// about 3 us on MI355X (1.8 us inside a hipGraph), less than a grid barrier
+// (6-34 us), so this beats one persistent kernel; the persistent variant is
+// kept behind OLV_PERSISTENT for comparison.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -19,28 +29,49 @@
 namespace olv {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kSpinLimit = 1u << 26;
+constexpr uint32_t kSpinLimit = 1u << 24;
+
+struct BfsState {              // per level parity
+  unsigned long long mu;       // unexplored out-edges before this level
+  uint32_t mode;               // 0 = top-down, 1 = bottom-up
+  uint32_t nbu;                // bottom-up levels before this one
+};
+
+struct BfsCtl {
+  uint32_t level;      // next level to expand (written by compact)
+  uint32_t level_e;    // level being compacted (written by expand)
+  uint32_t done;       // traversal finished
+  uint32_t levels;     // result: levels run, the final empty one included
+  uint32_t nbu;        // result: bottom-up levels
+  uint32_t overflow;   // result: level cap (n + 1) reached
+  uint32_t qfill[2];   // frontier queue length of level L, slot L & 1
+  uint32_t hfill[2];   // hub items of level L, slot L & 1
+  uint32_t pad[2];
+  unsigned long long inspected;  // result: edges read
+  BfsState st[2];
+  BfsCtr cnt[3];       // level counters, slot L % 3
+};
 
 struct BfsWork {
   uint32_t n = 0, nw = 0, nwp = 0;  // vertices, bitmap words, padded words
   uint32_t *visited = nullptr;
   uint32_t *bms = nullptr;          // 3 bitmaps of nwp words
   uint32_t *queue = nullptr;
-  BfsCtr *ctr = nullptr;            // 3 counter slots
-  BfsCtr *h_ctr = nullptr;          // pinned mirror
-  uint32_t *bar = nullptr;          // barrier + result words (16)
-  uint32_t *h_res = nullptr;        // pinned mirror of result words
-  unsigned long long *stat = nullptr;  // reached vertices, reached out-degree sum
+  uint2 *hubs = nullptr;
+  BfsCtl *ctl = nullptr;
+  BfsCtl *h_ctl = nullptr;          // pinned mirror
+  GridBar *bar = nullptr;           // persistent mode
+  unsigned long long *stat = nullptr;
+  uint32_t last_levels = 0;         // depth of the previous run (chunk sizing)
 };
 
 void free_bfs_work(olv_graph *g) {
   BfsWork *w = g->bfs;
   if (!w) return;
-  void *ps[] = {w->visited, w->bms, w->queue, w->ctr, w->bar, w->stat};
+  void *ps[] = {w->visited, w->bms, w->queue, w->hubs, w->ctl, w->bar, w->stat};
   for (void *p : ps)
     if (p) (void)hipFree(p);
-  if (w->h_ctr) (void)hipHostFree(w->h_ctr);
-  if (w->h_res) (void)hipHostFree(w->h_res);
+  if (w->h_ctl) (void)hipHostFree(w->h_ctl);
   delete w;
   g->bfs = nullptr;
 }
@@ -56,137 +87,177 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 3));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->queue), (size_t)g->n * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctr), sizeof(BfsCtr) * 3));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bar), 16 * 4));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->hubs), hub_capacity(g->m) * sizeof(uint2)));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(BfsCtl)));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bar), sizeof(GridBar)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->stat), 2 * sizeof(unsigned long long)));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctr), sizeof(BfsCtr) * 3, hipHostMallocDefault));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_res), 16 * 4, hipHostMallocDefault));
+    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(BfsCtl), hipHostMallocDefault));
   }
   *out = g->bfs;
   return OLV_OK;
 }
 
 // ------------------------------------------------------------- kernels
-// levels = INF, bitmaps/counters/barrier = 0, then the seeds: level 0,
-// visited, queue[0..k) = seeds, ctr[0].qsize = #distinct seeds.
+// levels = INF, visited/bitmaps/control/barrier = 0.
 __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint32_t *__restrict__ visited,
-                                 uint32_t *__restrict__ bms, uint32_t nwp, BfsCtr *__restrict__ ctr,
-                                 uint32_t *__restrict__ bar) {
+                                 uint32_t *__restrict__ bms, uint32_t nwp, BfsCtl *__restrict__ ctl,
+                                 GridBar *__restrict__ bar) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = tid; i < n; i += stride) levels[i] = kInf;
   for (size_t i = tid; i < nwp; i += stride) visited[i] = 0u;
   for (size_t i = tid; i < (size_t)nwp * 3; i += stride) bms[i] = 0u;
-  if (tid < sizeof(BfsCtr) * 3 / 4) reinterpret_cast<uint32_t *>(ctr)[tid] = 0u;
-  if (tid < 16) bar[tid] = 0u;
+  for (size_t i = tid; i < sizeof(BfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
+  for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
 }
 
-// Single workgroup: seeds are few (one source, or one per graph of a batch).
-__global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, int32_t *__restrict__ levels,
-                                uint32_t *__restrict__ visited, uint32_t *__restrict__ queue,
-                                BfsCtr *__restrict__ ctr) {
-  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+// One workgroup: distinct seeds become level 0 and the level-0 frontier.
+__global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, BfsArgs a, unsigned long long m,
+                                BfsCtl *__restrict__ ctl) {
+  auto claim = [&](uint32_t i) -> uint32_t {
     const uint32_t s = seeds[i];
-    const uint32_t old = atomicOr(&visited[s >> 5], 1u << (s & 31));
-    if (!(old & (1u << (s & 31)))) {
-      levels[s] = 0;
-      const uint32_t pos = atomicAdd(&ctr[0].qsize, 1u);
-      queue[pos] = s;
-    }
+    const uint32_t old = atomicOr(&a.visited[s >> 5], 1u << (s & 31));
+    if (old & (1u << (s & 31))) return 0xffffffffu;
+    a.levels[s] = 0;
+    return s;
+  };
+  unsigned long long deg = 0;
+  seed_frontier(seeds, k, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[0], a.hubs, &ctl->hfill[0], claim, IdRow(),
+                &deg);
+  if (threadIdx.x == 0) {
+    ctl->st[0].mu = m > deg ? m - deg : 0ull;
+    ctl->st[0].mode = 0;
+    ctl->st[0].nbu = 0;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) td_kernel(BfsArgs a, uint32_t qsize, int32_t nlevel,
-                                                    uint32_t *next_bm, BfsCtr *ctr) {
-  uint32_t nf = 0;
-  unsigned long long mf = 0, insp = 0;
-  td_step<kBlock>(a, qsize, nlevel, next_bm, blockIdx.x, gridDim.x, nf, mf, insp);
-  flush_counters<kBlock>(ctr, nf, mf, insp);
-}
-
-__global__ void __launch_bounds__(kBlock) bu_kernel(BfsArgs a, const uint32_t *front_bm, int32_t nlevel,
-                                                    uint32_t *next_bm, BfsCtr *ctr) {
-  uint32_t nf = 0;
-  unsigned long long mf = 0, insp = 0;
-  bu_step<kBlock>(a, front_bm, nlevel, next_bm, blockIdx.x, gridDim.x, nf, mf, insp);
-  flush_counters<kBlock>(ctr, nf, mf, insp);
-}
-
-__global__ void __launch_bounds__(kBlock) compact_kernel(const uint32_t *bm, uint32_t nwords, uint32_t *queue,
-                                                         uint32_t *count) {
-  compact_step<kBlock>(bm, nwords, queue, count, blockIdx.x, gridDim.x);
-}
-
-__global__ void clear_ctr_kernel(BfsCtr *c) {
-  if (threadIdx.x < sizeof(BfsCtr) / 4) reinterpret_cast<uint32_t *>(c)[threadIdx.x] = 0u;
-}
-
-__global__ void clear_words_kernel(uint32_t *p, uint32_t nw) {
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nw; i += gridDim.x * blockDim.x) p[i] = 0u;
-}
-
-struct BfsPersist {
-  BfsArgs a;
-  uint32_t *bms;       // 3 x nwp words
-  BfsCtr *ctr;         // 3 slots
-  uint32_t *bar;       // [0..2] barrier, [4] levels run, [5] bottom-up levels, [6] error
-  unsigned long long m;
-  uint32_t nw, nwp;
+struct BfsDir {
   uint32_t can_bu;
   float alpha, beta;
 };
 
-// All levels in one launch.  Level L: expand (top-down from the queue or
-// bottom-up against bitmap (L+2)%3) into bitmap L%3, clear bitmap (L+1)%3,
-// barrier; every block then reads the same counters and takes the same
-// decision; if the next step is top-down, compact bitmap L%3 into the queue
-// and barrier again.
-__global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsPersist p) {
-  uint32_t gen = 0;
-  const uint32_t vb = blockIdx.x, nvb = gridDim.x;
-  // unexplored out-edges: m minus the out-degrees of the seeds
-  unsigned long long mu = p.m;
-  const uint32_t nseed = __hip_atomic_load(&p.ctr[0].qsize, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  for (uint32_t i = 0; i < nseed; ++i) {
-    const uint32_t s = p.a.queue[i];
-    mu -= p.a.row_ptr[s + 1] - p.a.row_ptr[s];
+// Level L = ctl->level: expand into bitmap L%3 (top-down from the level's
+// queue and hub items, or bottom-up against bitmap (L+2)%3), clear bitmap
+// (L+1)%3 and the next level's fill counters.
+__global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nwp) {
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level;
+  const BfsState st = ctl->st[L & 1];
+  const uint32_t qsize = ctl->qfill[L & 1], nhubs = ctl->hfill[L & 1];
+  uint32_t *nextb = bms + (size_t)(L % 3) * nwp;
+  const uint32_t *front = bms + (size_t)((L + 2) % 3) * nwp;
+  uint32_t *clearb = bms + (size_t)((L + 1) % 3) * nwp;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nwp; i += gridDim.x * kBlock) clearb[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+    if (threadIdx.x == 0) {
+      ctl->level_e = L;
+      ctl->qfill[(L + 1) & 1] = 0;
+      ctl->hfill[(L + 1) & 1] = 0;
+    }
+    reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
   }
-  bool bu = false;
-  uint32_t nbu = 0, L = 0;
+  uint32_t nf = 0;
+  unsigned long long mf = 0, insp = 0;
+  if (st.mode == 0)
+    td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), nextb, blockIdx.x, gridDim.x, nf, mf, insp);
+  else
+    bu_step<kBlock>(a, front, (int32_t)(L + 1), nextb, blockIdx.x, gridDim.x, nf, mf, insp);
+  flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
+}
+
+// Level L = ctl->level_e: decide the next direction; if top-down, turn
+// bitmap L%3 into the next level's queue and hub items.  Workgroup 0
+// publishes the next level's state.
+__global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nw,
+                                                             uint32_t nwp, unsigned long long m, BfsDir dir) {
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t L = ctl->level_e;
+  const BfsCtr c = ctl->cnt[L % 3];
+  const BfsState st = ctl->st[L & 1];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  const uint32_t nbu = st.nbu + st.mode;
+  if (lead) ctl->inspected += c.inspected;
+  if (c.nf == 0 || L + 1 > a.n) {
+    if (lead) {
+      ctl->levels = L + 1;
+      ctl->nbu = nbu;
+      ctl->overflow = c.nf != 0;
+      ctl->done = 1u;
+    }
+    return;
+  }
+  const unsigned long long mu = st.mu > c.mf ? st.mu - c.mf : 0ull;
+  const bool next_bu = choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
+  if (lead) {
+    BfsState ns;
+    ns.mu = mu;
+    ns.mode = next_bu ? 1u : 0u;
+    ns.nbu = nbu;
+    ctl->st[(L + 1) & 1] = ns;
+    ctl->level = L + 1;
+  }
+  if (!next_bu)
+    compact_step<kBlock>(bms + (size_t)(L % 3) * nwp, nw, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[(L + 1) & 1],
+                         a.hubs, &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x, IdRow());
+}
+
+// All levels in one cooperative launch with the XCD-aware grid barrier
+// (OLV_PERSISTENT).  Same steps and decisions as the pipeline.
+__global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nw,
+                                                                uint32_t nwp, BfsDir dir, GridBar *bar) {
+  GridBarLocal gb;
+  if (!gbar_init(bar, gb, gridDim.x, kSpinLimit)) return;
+  const uint32_t vb = blockIdx.x, nvb = gridDim.x;
+  const bool lead = vb == 0 && threadIdx.x == 0;
+  BfsState st = ctl->st[0];
+  unsigned long long insp_all = 0;
+  uint32_t L = 0;
   for (;;) {
-    BfsCtr *cur = &p.ctr[L % 3];
-    BfsCtr *nxt = &p.ctr[(L + 1) % 3];
-    uint32_t *nextb = p.bms + (size_t)(L % 3) * p.nwp;
-    const uint32_t *front = p.bms + (size_t)((L + 2) % 3) * p.nwp;
-    uint32_t *clearb = p.bms + (size_t)((L + 1) % 3) * p.nwp;
-    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) reinterpret_cast<uint32_t *>(nxt)[threadIdx.x] = 0u;
-    for (uint32_t i = vb * kBlock + threadIdx.x; i < p.nwp; i += nvb * kBlock) clearb[i] = 0u;
+    uint32_t *nextb = bms + (size_t)(L % 3) * nwp;
+    const uint32_t *front = bms + (size_t)((L + 2) % 3) * nwp;
+    uint32_t *clearb = bms + (size_t)((L + 1) % 3) * nwp;
+    const uint32_t qsize = __hip_atomic_load(&ctl->qfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t nhubs = __hip_atomic_load(&ctl->hfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t i = vb * kBlock + threadIdx.x; i < nwp; i += nvb * kBlock) clearb[i] = 0u;
+    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+      if (threadIdx.x == 0) {
+        ctl->qfill[(L + 1) & 1] = 0;
+        ctl->hfill[(L + 1) & 1] = 0;
+      }
+      reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
+    }
     uint32_t nf = 0;
     unsigned long long mf = 0, insp = 0;
-    if (!bu) {
-      const uint32_t qsize = __hip_atomic_load(&cur->qsize, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      td_step<kBlock>(p.a, qsize, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
-    } else {
-      bu_step<kBlock>(p.a, front, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
+    if (st.mode == 0)
+      td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
+    else
+      bu_step<kBlock>(a, front, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
+    flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
+    if (!grid_barrier(bar, gb, kSpinLimit)) break;
+    const BfsCtr c = ctl->cnt[L % 3];
+    insp_all += c.inspected;
+    const uint32_t nbu = st.nbu + st.mode;
+    if (c.nf == 0 || L + 1 > a.n) {
+      if (lead) {
+        ctl->levels = L + 1;
+        ctl->nbu = nbu;
+        ctl->overflow = c.nf != 0;
+        ctl->inspected = insp_all;
+        ctl->done = 1u;
+      }
+      break;
     }
-    flush_counters<kBlock>(cur, nf, mf, insp);
-    if (!grid_barrier(p.bar, nvb, gen, kSpinLimit)) break;
-    const uint32_t tnf = __hip_atomic_load(&cur->nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long tmf = __hip_atomic_load(&cur->mf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (bu) ++nbu;
-    ++L;
-    if (tnf == 0 || L > p.a.n) break;
-    mu = mu > tmf ? mu - tmf : 0ull;
-    const bool next_bu = choose_bottom_up(bu, p.can_bu != 0, tnf, tmf, mu, p.a.n, p.alpha, p.beta);
+    const unsigned long long mu = st.mu > c.mf ? st.mu - c.mf : 0ull;
+    const bool next_bu = choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
+    st.mu = mu;
+    st.mode = next_bu ? 1u : 0u;
+    st.nbu = nbu;
     if (!next_bu) {
-      compact_step<kBlock>(nextb, p.nw, p.a.queue, &nxt->qsize, vb, nvb);
-      if (!grid_barrier(p.bar, nvb, gen, kSpinLimit)) break;
+      compact_step<kBlock>(nextb, nw, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[(L + 1) & 1], a.hubs,
+                           &ctl->hfill[(L + 1) & 1], vb, nvb, IdRow());
+      if (!grid_barrier(bar, gb, kSpinLimit)) break;
     }
-    bu = next_bu;
-  }
-  if (vb == 0 && threadIdx.x == 0) {
-    p.bar[4] = L;
-    p.bar[5] = nbu;
+    ++L;
   }
 }
 
@@ -209,15 +280,6 @@ __global__ void bfs_stat_kernel(const int32_t *__restrict__ levels, const uint32
 }
 
 // ------------------------------------------------------------- drivers
-static int persistent_grid(int *grid) {
-  int per_cu = 0;
-  OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_persistent_kernel, kBlock, 0));
-  if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "persistent BFS kernel does not fit on a CU");
-  per_cu = std::min(per_cu, 2);
-  *grid = per_cu * cu_count();
-  return OLV_OK;
-}
-
 static BfsArgs make_args(const olv_graph *g, BfsWork *w, int32_t *levels) {
   BfsArgs a;
   a.row_ptr = g->row_ptr;
@@ -227,120 +289,26 @@ static BfsArgs make_args(const olv_graph *g, BfsWork *w, int32_t *levels) {
   a.levels = levels;
   a.visited = w->visited;
   a.queue = w->queue;
+  a.hubs = w->hubs;
   a.n = g->n;
   a.wave_thr = g->tune.td_wave_thr;
   a.block_thr = g->tune.td_block_thr;
+  a.hub_thr = g->tune.hub_thr;
   return a;
 }
 
-static unsigned grid_for(size_t items, size_t per_block) {
+static unsigned grid_for(size_t items, size_t per_block, size_t per_cu) {
   size_t b = (items + per_block - 1) / per_block;
-  size_t cap = (size_t)cu_count() * 8;
+  size_t cap = (size_t)cu_count() * per_cu;
   return (unsigned)std::max<size_t>(1, std::min(b, cap));
 }
 
-// Enqueue reset + seeding (stream-ordered, capturable: no memset nodes).
-static int enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels) {
-  hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 3), 256)), dim3(256),
-                     0, stream(), levels, g->n, w->visited, w->bms, w->nwp, w->ctr, w->bar);
-  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, levels, w->visited, w->queue,
-                     w->ctr);
-  OLV_HIP(hipGetLastError());
-  return OLV_OK;
-}
-
-static int enqueue_persistent(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu) {
-  int grid = 0;
-  OLV_TRY(persistent_grid(&grid));
-  BfsPersist p;
-  p.a = make_args(g, w, levels);
-  p.bms = w->bms;
-  p.ctr = w->ctr;
-  p.bar = w->bar;
-  p.m = g->m;
-  p.nw = w->nw;
-  p.nwp = w->nwp;
-  p.can_bu = can_bu ? 1u : 0u;
-  p.alpha = g->tune.alpha;
-  p.beta = g->tune.beta;
-  void *args[] = {&p};
-  OLV_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(bfs_persistent_kernel), dim3(grid),
-                                     dim3(kBlock), args, 0, stream()));
-  return OLV_OK;
-}
-
-// Host-driven levels: one launch group per level, counters read back.
-static int run_host_loop(const olv_graph *g, BfsWork *w, int32_t *levels, const std::vector<uint32_t> &h_seeds,
-                         bool can_bu, uint32_t *levels_run, uint32_t *bu_levels, unsigned long long *inspected) {
-  hipStream_t s = stream();
-  BfsArgs a = make_args(g, w, levels);
-  // seed statistics from the host copy of the seeds
-  OLV_HIP(hipMemcpyAsync(w->h_ctr, w->ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
-  OLV_HIP(hipStreamSynchronize(s));
-  uint32_t qsize = w->h_ctr[0].qsize;
-  unsigned long long mu = g->m;
-  {
-    std::vector<uint32_t> rp(2);
-    for (uint32_t sd : h_seeds) {
-      OLV_HIP(hipMemcpy(rp.data(), g->row_ptr + sd, 8, hipMemcpyDeviceToHost));
-      mu -= rp[1] - rp[0];
-    }
-  }
-  uint32_t *bm[3] = {w->bms, w->bms + w->nwp, w->bms + 2 * (size_t)w->nwp};
-  uint32_t *front = bm[2], *next = bm[0], *spare = bm[1];
-  bool bu = false;
-  uint32_t L = 0, nbu = 0;
-  unsigned long long insp_total = 0;
-  for (;;) {
-    hipLaunchKernelGGL(clear_ctr_kernel, dim3(1), dim3(64), 0, s, w->ctr);
-    if (!bu)
-      hipLaunchKernelGGL(td_kernel, dim3(grid_for(qsize, kBlock)), dim3(kBlock), 0, s, a, qsize, (int32_t)(L + 1),
-                         next, w->ctr);
-    else
-      hipLaunchKernelGGL(bu_kernel, dim3(grid_for(g->n, kBlock)), dim3(kBlock), 0, s, a, (const uint32_t *)front,
-                         (int32_t)(L + 1), next, w->ctr);
-    OLV_HIP(hipGetLastError());
-    OLV_HIP(hipMemcpyAsync(w->h_ctr, w->ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
-    OLV_HIP(hipStreamSynchronize(s));
-    const uint32_t nf = w->h_ctr[0].nf;
-    const unsigned long long mf = w->h_ctr[0].mf;
-    insp_total += w->h_ctr[0].inspected;
-    if (bu) ++nbu;
-    ++L;
-    if (nf == 0 || L > g->n) break;
-    mu = mu > mf ? mu - mf : 0ull;
-    const bool next_bu = choose_bottom_up(bu, can_bu, nf, mf, mu, g->n, g->tune.alpha, g->tune.beta);
-    if (!next_bu) {
-      hipLaunchKernelGGL(clear_ctr_kernel, dim3(1), dim3(64), 0, s, w->ctr);
-      hipLaunchKernelGGL(compact_kernel, dim3(grid_for(w->nw, kBlock)), dim3(kBlock), 0, s, (const uint32_t *)next,
-                         w->nw, w->queue, &w->ctr[0].qsize);
-      qsize = nf;
-    }
-    // rotate: next becomes the frontier bitmap, the old frontier is cleared
-    uint32_t *old_front = front;
-    front = next;
-    next = spare;
-    spare = old_front;
-    hipLaunchKernelGGL(clear_words_kernel, dim3(grid_for(w->nwp, 256)), dim3(256), 0, s, next, w->nwp);
-    bu = next_bu;
-  }
-  *levels_run = L;
-  *bu_levels = nbu;
-  *inspected = insp_total;
-  return OLV_OK;
-}
-
-static int collect_stats(const olv_graph *g, BfsWork *w, const int32_t *levels, olv_stats *st) {
-  if (!st) return OLV_OK;
-  OLV_HIP(hipMemsetAsync(w->stat, 0, 2 * sizeof(unsigned long long), stream()));
-  hipLaunchKernelGGL(bfs_stat_kernel, dim3(grid_for(g->n, kBlock)), dim3(kBlock), 0, stream(), levels, g->row_ptr,
-                     g->n, w->stat);
-  unsigned long long h[2];
-  OLV_HIP(hipMemcpyAsync(h, w->stat, sizeof(h), hipMemcpyDeviceToHost, stream()));
-  OLV_HIP(hipStreamSynchronize(stream()));
-  st->vertices_reached = h[0];
-  st->edges_traversed = h[1];
-  return OLV_OK;
+static BfsDir make_dir(const olv_graph *g, bool can_bu) {
+  BfsDir d;
+  d.can_bu = (can_bu && g->in_row_ptr) ? 1u : 0u;
+  d.alpha = g->tune.alpha;
+  d.beta = g->tune.beta;
+  return d;
 }
 
 // Transpose (for bottom-up steps) and work buffers of g.
@@ -349,23 +317,91 @@ int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w) {
   return get_work(g, w);
 }
 
-// Stream-ordered, capturable: reset, seeding and the persistent kernel.
-int bfs_enqueue(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels, bool can_bu) {
-  OLV_TRY(enqueue_init(g, w, d_seeds, k, levels));
-  return enqueue_persistent(g, w, levels, can_bu && g->in_row_ptr != nullptr);
-}
-
-// Wait for the stream, then read the level count of the last persistent run.
-int bfs_result(BfsWork *w, uint32_t *levels_run, uint32_t *bu_levels) {
-  OLV_HIP(hipMemcpyAsync(w->h_res, w->bar, 16 * 4, hipMemcpyDeviceToHost, stream()));
-  OLV_HIP(hipStreamSynchronize(stream()));
-  if (w->h_res[2]) return fail(OLV_EINTERNAL, "olv_bfs: grid barrier timed out (co-residency lost)");
-  *levels_run = w->h_res[4];
-  *bu_levels = w->h_res[5];
+// Stream-ordered, capturable: reset and seeding.
+int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels) {
+  hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 3), 256, 8)), dim3(256),
+                     0, stream(), levels, g->n, w->visited, w->bms, w->nwp, w->ctl, w->bar);
+  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, make_args(g, w, levels),
+                     (unsigned long long)g->m, w->ctl);
+  OLV_HIP(hipGetLastError());
   return OLV_OK;
 }
 
-uint32_t *bfs_result_words(BfsWork *w) { return w->bar; }
+// Stream-ordered, capturable: `count` levels (expand + compact each).
+int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count) {
+  const BfsArgs a = make_args(g, w, levels);
+  const BfsDir dir = make_dir(g, can_bu);
+  const unsigned ge = (unsigned)cu_count() * 8;
+  const unsigned gc = grid_for(w->nw, kBlock, 8);
+  for (uint32_t i = 0; i < count; ++i) {
+    hipLaunchKernelGGL(bfs_expand_kernel, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);
+    hipLaunchKernelGGL(bfs_compact_kernel, dim3(gc), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nw, w->nwp,
+                       (unsigned long long)g->m, dir);
+  }
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
+static int enqueue_persistent(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu) {
+  int per_cu = 0;
+  OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_persistent_kernel, kBlock, 0));
+  if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "persistent BFS kernel does not fit on a CU");
+  const int grid = cu_count();  // one workgroup per CU: cheapest barrier (measured)
+  BfsArgs a = make_args(g, w, levels);
+  BfsCtl *ctl = w->ctl;
+  uint32_t *bms = w->bms;
+  uint32_t nw = w->nw, nwp = w->nwp;
+  BfsDir dir = make_dir(g, can_bu);
+  GridBar *bar = w->bar;
+  void *args[] = {&a, &ctl, &bms, &nw, &nwp, &dir, &bar};
+  OLV_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(bfs_persistent_kernel), dim3(grid), dim3(kBlock),
+                                     args, 0, stream()));
+  return OLV_OK;
+}
+
+// Wait for the stream, then read the control block.
+int bfs_result(BfsWork *w, uint32_t *levels_run, uint32_t *bu_levels) {
+  OLV_HIP(hipMemcpyAsync(w->h_ctl, w->ctl, sizeof(BfsCtl), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  if (levels_run) *levels_run = w->h_ctl->levels;
+  if (bu_levels) *bu_levels = w->h_ctl->nbu;
+  return OLV_OK;
+}
+
+bool bfs_done(BfsWork *w) { return w->h_ctl->done != 0; }
+uint32_t bfs_depth_hint(BfsWork *w) { return w->last_levels; }
+void bfs_set_depth_hint(BfsWork *w, uint32_t d) { w->last_levels = d; }
+uint32_t *bfs_ctl_words(BfsWork *w) { return reinterpret_cast<uint32_t *>(w->ctl); }
+GridBar *bfs_bar(BfsWork *w) { return w->bar; }
+
+// Enqueue levels in growing chunks until the device reports done.
+// chunk0 == 1 gives one host round trip per level (OLV_HOST_LOOP).
+static int run_pipeline(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t chunk0) {
+  uint32_t chunk = std::max(1u, chunk0);
+  uint64_t enq = 0;
+  for (;;) {
+    OLV_TRY(bfs_enqueue_levels(g, w, levels, can_bu, chunk));
+    enq += chunk;
+    OLV_TRY(bfs_result(w, nullptr, nullptr));
+    if (w->h_ctl->done) break;
+    if (enq > (uint64_t)g->n + 2) return fail(OLV_EINTERNAL, "olv_bfs: level loop did not terminate");
+    if (chunk0 > 1) chunk = std::min(chunk * 2, 256u);
+  }
+  return OLV_OK;
+}
+
+static int collect_stats(const olv_graph *g, BfsWork *w, const int32_t *levels, olv_stats *st) {
+  if (!st) return OLV_OK;
+  OLV_HIP(hipMemsetAsync(w->stat, 0, 2 * sizeof(unsigned long long), stream()));
+  hipLaunchKernelGGL(bfs_stat_kernel, dim3(grid_for(g->n, kBlock, 8)), dim3(kBlock), 0, stream(), levels, g->row_ptr,
+                     g->n, w->stat);
+  unsigned long long h[2];
+  OLV_HIP(hipMemcpyAsync(h, w->stat, sizeof(h), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  st->vertices_reached = h[0];
+  st->edges_traversed = h[1];
+  return OLV_OK;
+}
 
 // Multi-source BFS on g (levels = distance to the nearest seed).
 // seeds_host: seeds is a host array (levels follow OLV_PTR_HOST in flags).
@@ -392,25 +428,31 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
   }
   Timer t;
   t.start();
-  OLV_TRY(enqueue_init(g, w, d_seeds.as<uint32_t>(), k, out));
-  uint32_t levels_run = 0, bu_levels = 0;
-  unsigned long long insp = 0;
-  if (flags & OLV_HOST_LOOP) {
-    OLV_TRY(run_host_loop(g, w, out, h_seeds, can_bu, &levels_run, &bu_levels, &insp));
-    t.stop();
-  } else {
+  OLV_TRY(bfs_enqueue_init(g, w, d_seeds.as<uint32_t>(), k, out));
+  if (flags & OLV_PERSISTENT) {
     OLV_TRY(enqueue_persistent(g, w, out, can_bu));
-    t.stop();
-    OLV_TRY(bfs_result(w, &levels_run, &bu_levels));
+  } else {
+    const uint32_t chunk0 = (flags & OLV_HOST_LOOP) ? 1u : std::max(4u, w->last_levels + 1);
+    OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0));
   }
+  t.stop();
   const double ms = t.ms();
+  uint32_t levels_run = 0, bu_levels = 0;
+  OLV_TRY(bfs_result(w, &levels_run, &bu_levels));
+  if (flags & OLV_PERSISTENT) {
+    GridBar hb;
+    OLV_HIP(hipMemcpy(&hb, w->bar, sizeof(hb), hipMemcpyDeviceToHost));
+    if (hb.err || !w->h_ctl->done) return fail(OLV_EINTERNAL, "olv_bfs: grid barrier timed out (co-residency lost)");
+  }
+  if (w->h_ctl->overflow) return fail(OLV_EINTERNAL, "olv_bfs: more than n levels");
+  w->last_levels = levels_run;
   if (host) OLV_HIP(hipMemcpy(levels, out, (size_t)g->n * 4, hipMemcpyDeviceToHost));
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
     st->iterations = levels_run;
     st->bottom_up_levels = bu_levels;
-    if (flags & OLV_HOST_LOOP) st->edges_inspected = insp;
+    st->edges_inspected = w->h_ctl->inspected;
     OLV_TRY(collect_stats(g, w, out, st));
   }
   return OLV_OK;

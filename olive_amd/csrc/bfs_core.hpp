@@ -1,23 +1,28 @@
-// Frontier steps shared by the BFS/SSSP drivers (host-driven, persistent and
-// multi-graph).  Each step is a __device__ function over a virtual block
-// range [vb, vb + nvb) so the same code runs as a standalone kernel per level
-// or inside one persistent cooperative kernel.
+// Frontier steps shared by the BFS, SSSP and batched drivers.  Each step is
+// a __device__ function over a virtual block range [vb, vb + nvb) so the same
+// code runs in the per-level kernels or inside one persistent kernel.
 //
 // Reference semantics (BFS.cu:38-79 over oliver.h:56-113): a vertex first
 // reached while expanding level L gets level L+1; INF = 0x7fffffff.  Levels
 // are unique, so push (top-down) and pull (bottom-up) steps may be mixed per
 // level without changing any output.
+//
+// A frontier handed to a push step has two parts: a queue of vertices with
+// out-degree below hub_thr, and a list of hub items (vertex, chunk) that cut
+// each higher-degree out-list into kHubChunk-edge pieces, one workgroup per
+// piece -- so no single workgroup walks a 10^5-edge list alone.
 #pragma once
 #include "device.hpp"
 
 namespace olv {
 
+constexpr uint32_t kHubChunk = 4096;
+
 struct BfsCtr {
   unsigned long long mf;         // sum of out-degrees of vertices claimed
   unsigned long long inspected;  // edges read
   uint32_t nf;                   // vertices claimed
-  uint32_t qsize;                // length of the queue the TD step reads
-  uint32_t pad[2];
+  uint32_t pad[3];
 };
 
 struct BfsArgs {
@@ -28,24 +33,30 @@ struct BfsArgs {
   int32_t *levels;
   uint32_t *visited;
   uint32_t *queue;
+  uint2 *hubs;
   uint32_t n;
   uint32_t wave_thr;   // degree >= wave_thr: one wave per vertex
   uint32_t block_thr;  // degree >= block_thr: whole workgroup per vertex
+  uint32_t hub_thr;    // degree >= hub_thr: kHubChunk-edge hub items
 };
 
-// Workload-balanced push expansion of queue[0, qsize).  Each out-list is
-// read by the whole workgroup (degree >= block_thr), by one wave
-// (>= wave_thr), or -- for the remaining short lists -- by a block-wide scan
-// that deals the edges of many vertices to consecutive lanes, so a wave
-// always reads 64 consecutive column entries.
+// Hub items needed for the worst case of m edges.
+inline size_t hub_capacity(uint64_t m) { return 2 * (m / kHubChunk) + 64; }
+
+// Workload-balanced push expansion of a frontier (queue[0, qsize) plus hub
+// items hubs[0, nhubs)).  Each queued out-list is read by the whole
+// workgroup (degree >= block_thr), by one wave (>= wave_thr), or -- for the
+// remaining short lists -- by a block-wide scan that deals the edges of many
+// vertices to consecutive lanes, so a wave always reads 64 consecutive
+// column entries.
 //
-// Visitor: `uint32_t row(uint32_t q)` maps a queue entry to its CSR row,
+// Visitor: `uint32_t row(uint32_t q)` maps a frontier entry to its CSR row,
 // `P load(uint32_t q)` gives the per-source payload read once per frontier
 // entry, `void visit(uint32_t e, P payload)` handles edge e.
 template <int BLOCK, class Visitor>
-__device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint32_t *row_ptr, uint32_t wave_thr,
-                            uint32_t block_thr, uint32_t vb, uint32_t nvb, Visitor &vis,
-                            unsigned long long &insp) {
+__device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *hubs, uint32_t nhubs,
+                            const uint32_t *row_ptr, uint32_t wave_thr, uint32_t block_thr, uint32_t vb, uint32_t nvb,
+                            Visitor &vis, unsigned long long &insp) {
   using P = decltype(vis.load(0u));
   __shared__ uint32_t s_owner, s_beg, s_end;
   __shared__ P s_own_payload;
@@ -53,6 +64,17 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint32_
   __shared__ uint32_t s_edges[BLOCK];
   __shared__ P s_payload[BLOCK];
   const uint32_t lane = lane_id();
+  // hub pieces: one workgroup per item
+  for (uint32_t h = vb; h < nhubs; h += nvb) {
+    const uint2 it = hubs[h];
+    const uint32_t u = vis.row(it.x);
+    const uint32_t beg = row_ptr[u] + it.y * kHubChunk;
+    const uint32_t e1 = row_ptr[u + 1];
+    const uint32_t end = e1 < beg + kHubChunk ? e1 : beg + kHubChunk;
+    const P pl = vis.load(it.x);
+    if (threadIdx.x == 0) insp += end - beg;
+    for (uint32_t e = beg + threadIdx.x; e < end; e += BLOCK) vis.visit(e, pl);
+  }
   for (uint32_t base = vb * BLOCK; base < qsize; base += nvb * BLOCK) {
     const uint32_t i = base + threadIdx.x;
     uint32_t beg = 0, end = 0;
@@ -138,10 +160,10 @@ struct BfsVisitor {
 };
 
 template <int BLOCK>
-__device__ void td_step(const BfsArgs &a, uint32_t qsize, int32_t nlevel, uint32_t *next_bm, uint32_t vb,
-                        uint32_t nvb, uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
+__device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_t nlevel, uint32_t *next_bm,
+                        uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
   BfsVisitor vis(a, nlevel, next_bm);
-  push_expand<BLOCK>(a.queue, qsize, a.row_ptr, a.wave_thr, a.block_thr, vb, nvb, vis, insp);
+  push_expand<BLOCK>(a.queue, qsize, a.hubs, nhubs, a.row_ptr, a.wave_thr, a.block_thr, vb, nvb, vis, insp);
   nf += vis.nf;
   mf += vis.mf;
 }
@@ -193,29 +215,53 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
   }
 }
 
-// Bitmap -> queue.  One lane per word, block scan of popcounts, one global
-// atomic per block for the output base (ids ascend within a block's range).
-template <int BLOCK>
-__device__ void compact_step(const uint32_t *bm, uint32_t nwords, uint32_t *queue, uint32_t *count, uint32_t vb,
-                             uint32_t nvb) {
+// Bitmap -> frontier.  One lane per word; bits whose row has out-degree >=
+// hub_thr become hub items (vertex, chunk) appended with one atomic per hub,
+// the rest are placed in the queue by a block scan of per-lane counts with
+// one global atomic per block.  row(v) maps a bit to its CSR row.
+template <int BLOCK, class RowFn>
+__device__ void compact_step(const uint32_t *bm, uint32_t nwords, const uint32_t *row_ptr, uint32_t hub_thr,
+                             uint32_t *queue, uint32_t *qcount, uint2 *hubs, uint32_t *hcount, uint32_t vb,
+                             uint32_t nvb, RowFn row) {
   __shared__ uint32_t s_scan[BLOCK / kWave + 1];
   __shared__ uint32_t s_base;
   for (uint32_t w0 = vb * BLOCK; w0 < nwords; w0 += nvb * BLOCK) {
     const uint32_t w = w0 + threadIdx.x;
-    uint32_t word = w < nwords ? bm[w] : 0u;
+    const uint32_t word = w < nwords ? bm[w] : 0u;
+    uint32_t small = 0;
+    for (uint32_t x = word; x;) {
+      const uint32_t b = __ffs(x) - 1;
+      x &= x - 1;
+      const uint32_t v = w * 32u + b;
+      const uint32_t u = row(v);
+      const uint32_t d = row_ptr[u + 1] - row_ptr[u];
+      if (d >= hub_thr) {
+        const uint32_t k = (d + kHubChunk - 1) / kHubChunk;
+        const uint32_t hb = atomicAdd(hcount, k);
+        for (uint32_t j = 0; j < k; ++j) hubs[hb + j] = make_uint2(v, j);
+      } else {
+        ++small;
+      }
+    }
     uint32_t total;
-    const uint32_t off = block_exclusive_scan<BLOCK>((uint32_t)__popc(word), s_scan, &total);
-    if (threadIdx.x == 0) s_base = total ? atomicAdd(count, total) : 0u;
+    const uint32_t off = block_exclusive_scan<BLOCK>(small, s_scan, &total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(qcount, total) : 0u;
     __syncthreads();
     uint32_t pos = s_base + off;
-    while (word) {
-      const uint32_t b = __ffs(word) - 1;
-      queue[pos++] = w * 32u + b;
-      word &= word - 1;
+    for (uint32_t x = word; x;) {
+      const uint32_t b = __ffs(x) - 1;
+      x &= x - 1;
+      const uint32_t v = w * 32u + b;
+      const uint32_t u = row(v);
+      if (row_ptr[u + 1] - row_ptr[u] < hub_thr) queue[pos++] = v;
     }
     __syncthreads();
   }
 }
+
+struct IdRow {
+  __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v; }
+};
 
 template <int BLOCK>
 __device__ __forceinline__ void flush_counters(BfsCtr *c, uint32_t nf, unsigned long long mf,
@@ -239,6 +285,34 @@ __host__ __device__ __forceinline__ bool choose_bottom_up(bool cur_bu, bool can_
   if (!can_bu) return false;
   if (!cur_bu) return (double)mf * alpha > (double)mu;
   return (double)nf * beta >= (double)n;
+}
+
+// Seeds -> level-0 frontier (one workgroup): distinct seeds with out-degree
+// below hub_thr go to the queue, the others become hub items.  Returns the
+// out-degree sum of the distinct seeds in *deg (thread 0).
+template <class Claim, class RowFn>
+__device__ void seed_frontier(const uint32_t *seeds, uint32_t k, const uint32_t *row_ptr, uint32_t hub_thr,
+                              uint32_t *queue, uint32_t *qcount, uint2 *hubs, uint32_t *hcount, Claim claim,
+                              RowFn row, unsigned long long *deg) {
+  __shared__ unsigned long long s_deg;
+  if (threadIdx.x == 0) s_deg = 0;
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+    const uint32_t q = claim(i);  // frontier entry, or ~0u if not new
+    if (q == 0xffffffffu) continue;
+    const uint32_t u = row(q);
+    const uint32_t d = row_ptr[u + 1] - row_ptr[u];
+    atomicAdd(&s_deg, (unsigned long long)d);
+    if (d >= hub_thr) {
+      const uint32_t kk = (d + kHubChunk - 1) / kHubChunk;
+      const uint32_t hb = atomicAdd(hcount, kk);
+      for (uint32_t j = 0; j < kk; ++j) hubs[hb + j] = make_uint2(q, j);
+    } else {
+      queue[atomicAdd(qcount, 1u)] = q;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) *deg = s_deg;
 }
 
 }  // namespace olv

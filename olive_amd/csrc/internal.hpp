@@ -37,6 +37,7 @@ struct SegPlan {
 struct Tuning {
   uint32_t td_block_thr = 256;  // degree >= this: whole workgroup per vertex
   uint32_t td_wave_thr = 64;    // degree >= this: one wave per vertex
+  uint32_t hub_thr = 4096;      // degree >= this: split into 4096-edge items
   uint32_t block_threads = 256;
   float alpha = 14.0f;          // top-down -> bottom-up when m_f > m_u / alpha
   float beta = 24.0f;           // bottom-up -> top-down when n_f < n / beta

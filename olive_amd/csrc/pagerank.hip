@@ -6,9 +6,11 @@
 // (PageRank.cu:63-75) and a host-side vertexReduce (full D2H copy,
 // oliver.h:176-183) per iteration.  Here one fused launch per iteration
 // computes every vertex's in-sum with a fixed summation order, applies the
-// update, writes the next contribution vector and a per-block residual; a
-// one-workgroup finalize kernel totals the residual and raises a device
-// flag that turns the remaining pre-enqueued iterations into no-ops.
+// update, writes the next contribution vector and a per-block residual; the
+// launch's last workgroup totals the residual and raises a device flag that
+// turns the remaining pre-enqueued iterations into no-ops.  Index and weight
+// arrays are streamed with non-temporal loads so the gathered vector keeps
+// its place in L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,7 +92,9 @@ struct PrOp {
   float *r;
   const float *inv_outdeg;
   float d, base;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return contrib_cur[ci[k]]; }
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
+    return contrib_cur[ld_stream(&ci[k])];
+  }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + base;
     err += fabsf(rn - r[v]);
@@ -108,7 +112,9 @@ struct WprOp {
   float *r_next;
   const float *t;
   float d, omd;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return w[k] * r_cur[ci[k]]; }
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
+    return ld_stream(&w[k]) * r_cur[ld_stream(&ci[k])];
+  }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + omd * t[v];
     err += fabsf(rn - r_cur[v]);
@@ -124,7 +130,9 @@ struct AdjOp {
   float *l_next;
   const float *gbar;
   float d;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const { return w[k] * l_cur[ci[k]]; }
+  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
+    return ld_stream(&w[k]) * l_cur[ld_stream(&ci[k])];
+  }
   __device__ __forceinline__ void finish(uint32_t u, float s, float &err) const {
     const float ln = gbar[u] + d * s;
     err += fabsf(ln - l_cur[u]);
@@ -140,7 +148,7 @@ struct GradOp {
   float *grad_w;
   float d;
   __device__ __forceinline__ float term(uint32_t u, uint32_t k) const {
-    grad_w[k] = d * r[u] * lam[ci[k]];
+    __builtin_nontemporal_store(d * r[u] * lam[ld_stream(&ci[k])], &grad_w[k]);
     return 0.f;
   }
   __device__ __forceinline__ void finish(uint32_t, float, float &) const {}
@@ -159,6 +167,7 @@ __global__ void pr_init_kernel(uint32_t n, float r0, const float *__restrict__ i
     ctl->iters = 0;
     ctl->eps = eps;
     ctl->residual = 0.0;
+    ctl->arrived = 0;
   }
 }
 
@@ -167,6 +176,7 @@ __global__ void ctl_init_kernel(IterCtl *ctl, double eps) {
   ctl->iters = 0;
   ctl->eps = eps;
   ctl->residual = 0.0;
+  ctl->arrived = 0;
 }
 
 // eps = tol * sum |gbar| (single workgroup)
@@ -181,6 +191,7 @@ __global__ void __launch_bounds__(1024) ctl_init_l1_kernel(IterCtl *ctl, const f
     ctl->iters = 0;
     ctl->eps = tol * s;
     ctl->residual = 0.0;
+    ctl->arrived = 0;
   }
 }
 
@@ -230,14 +241,12 @@ struct KernelProfile {
 template <class Op>
 static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, PrWork *w,
                              KernelProfile *prof = nullptr) {
-  const uint32_t blocks = d.role_start[5];
-  if (blocks) {
-    hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
-    if (a) OLV_HIP(hipEventRecord(a, stream()));
-    hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(blocks), dim3(kSegBlock), 0, stream(), d, rp, op,
-                       (const uint32_t *)&w->ctl->done, w->err_part);
-    if (b) OLV_HIP(hipEventRecord(b, stream()));
-  }
+  const uint32_t blocks = seg_grid(d);
+  hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
+  if (a) OLV_HIP(hipEventRecord(a, stream()));
+  hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(blocks), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl,
+                     w->err_part);
+  if (b) OLV_HIP(hipEventRecord(b, stream()));
   hipLaunchKernelGGL(seg_finalize_kernel<Op>, dim3(1), dim3(1024), 0, stream(), d, op, (const float *)w->err_part,
                      blocks, w->ctl);
   OLV_HIP(hipGetLastError());
@@ -279,7 +288,7 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   OLV_TRY(ensure_plan(g, true));
   const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, std::max(1u, d.role_start[5]), &w));
+  OLV_TRY(get_pr_work(g, false, seg_grid(d), &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage so;
@@ -324,7 +333,7 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   OLV_TRY(ensure_plan(g, true));
   const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, true, std::max(1u, d.role_start[5]), &w));
+  OLV_TRY(get_pr_work(g, true, seg_grid(d), &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, st_t, so;
@@ -379,7 +388,7 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(ensure_plan(g, false));
   const SegDev d = make_seg_dev(g->plan_out);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, std::max(1u, d.role_start[5]), &w));
+  OLV_TRY(get_pr_work(g, false, seg_grid(d), &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, sr, sg, sgw, sgt, sl;
@@ -416,9 +425,8 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   const float *lf = lam[iters & 1];
   if (dgw && g->m) {
     GradOp gop{g->col, dr, lf, dgw, damping};
-    if (d.role_start[5])
-      hipLaunchKernelGGL(seg_iter_kernel<GradOp>, dim3(d.role_start[5]), dim3(kSegBlock), 0, stream(), d, g->row_ptr,
-                         gop, (const uint32_t *)nullptr, w->err_part);
+    hipLaunchKernelGGL(seg_iter_kernel<GradOp>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, g->row_ptr,
+                       gop, (IterCtl *)nullptr, w->err_part);
     // chunked hub rows wrote their edges inside the chunk blocks already
   }
   if (dgt) hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf, 1.0f - damping, g->n, dgt);

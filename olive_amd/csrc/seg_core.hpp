@@ -1,16 +1,18 @@
 // Bucketed segment sums over a CSR: one fused launch covers every vertex of
-// a SegPlan.  Block roles by blockIdx (heavy roles first so they start
-// first):
+// a SegPlan.  Work is cut into virtual blocks by role (heavy roles first so
+// they start first); a grid of a few thousand workgroups loops over them:
 //   [hub chunks]  one workgroup per chunk of a row longer than plan.chunk,
-//                 partial sum to chunk_partial[c] (combined later in a fixed
-//                 order by the finalize kernel)
+//                 partial sum to chunk_partial[c] (combined later in chunk
+//                 order)
 //   [bucket 3]    one workgroup per vertex
 //   [bucket 2]    one wave per vertex
 //   [bucket 1]    8 lanes per vertex
 //   [bucket 0]    one lane per vertex
-// Within a vertex, lane j sums terms j, j+G, j+2G, ... sequentially and the
-// G partials are combined by a shuffle tree, so the fp32 summation order is
-// fixed (bit-reproducible) and its error grows with deg/G + log2(G).
+// Within a vertex, lane j owns terms j, j+G, j+2G, ... and sums them into
+// four interleaved accumulators (four independent gathers in flight), which
+// are added pairwise, then the G lane partials are combined by a shuffle
+// tree.  The fp32 summation order is therefore fixed (bit-reproducible) and
+// its error grows with deg/(4G) + log2(4G).
 #pragma once
 #include "device.hpp"
 #include "internal.hpp"
@@ -25,8 +27,8 @@ struct SegDev {
   float *chunk_partial;
   uint32_t off[SegPlan::kBuckets + 1];
   uint32_t nhub, nchunks, chunk;
-  // first block of each role: [0]=hub chunks, [1]=bucket3, [2]=bucket2,
-  // [3]=bucket1, [4]=bucket0, [5]=end
+  // first virtual block of each role: [0]=hub chunks, [1]=bucket3,
+  // [2]=bucket2, [3]=bucket1, [4]=bucket0, [5]=end
   uint32_t role_start[6];
 };
 
@@ -56,6 +58,23 @@ inline SegDev make_seg_dev(const SegPlan &p) {
   return d;
 }
 
+// Workgroups per CU of a segment launch (0: one workgroup per virtual
+// block); never more workgroups than there are virtual blocks.
+#ifndef OLV_SEG_GRID_PER_CU
+#define OLV_SEG_GRID_PER_CU 16
+#endif
+// Independent gathers in flight per lane (1 or 4).
+#ifndef OLV_SEG_UNROLL
+#define OLV_SEG_UNROLL 4
+#endif
+
+inline uint32_t seg_grid(const SegDev &d) {
+  const uint32_t nvb = d.role_start[5] ? d.role_start[5] : 1;
+  if (OLV_SEG_GRID_PER_CU == 0) return nvb;
+  const uint32_t cap = (uint32_t)cu_count() * OLV_SEG_GRID_PER_CU;
+  return nvb < cap ? nvb : cap;
+}
+
 template <int G>
 __device__ __forceinline__ float group_sum(float v) {
 #pragma unroll
@@ -63,70 +82,23 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
-// Op: float term(uint32_t v, uint32_t k)   -- summand of in/out slot k of v
-//     void finish(uint32_t v, float sum, float &err)
-// err is accumulated per thread and written to err_part[blockIdx.x].
-template <class Op>
-__global__ void __launch_bounds__(kSegBlock) seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op,
-                                                             const uint32_t *__restrict__ done,
-                                                             float *__restrict__ err_part) {
-  if (done && __hip_atomic_load(done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  __shared__ float s_red[kSegBlock / kWave];
-  const uint32_t b = blockIdx.x;
-  float err = 0.f;
-  if (b < d.role_start[1]) {
-    // hub chunk: partial sum only
-    const uint32_t c = b;
-    const uint32_t h = d.chunk_hub[c];
-    const uint32_t v = d.hub_vertex[h];
-    const uint32_t k0 = c - d.hub_first[h];
-    const uint32_t beg = rp[v] + k0 * d.chunk;
-    const uint32_t end = min(rp[v + 1], beg + d.chunk);
-    float s = 0.f;
-    for (uint32_t k = beg + threadIdx.x; k < end; k += kSegBlock) s += op.term(v, k);
-    s = block_sum<kSegBlock, float>(s, s_red);
-    if (threadIdx.x == 0) d.chunk_partial[c] = s;
-    if (threadIdx.x == 0) err_part[b] = 0.f;
-    return;
-  } else if (b < d.role_start[2]) {
-    const uint32_t v = d.list[d.off[3] + (b - d.role_start[1])];
-    float s = 0.f;
-    for (uint32_t k = rp[v] + threadIdx.x; k < rp[v + 1]; k += kSegBlock) s += op.term(v, k);
-    s = block_sum<kSegBlock, float>(s, s_red);
-    if (threadIdx.x == 0) op.finish(v, s, err);
-  } else if (b < d.role_start[3]) {
-    const uint32_t idx = d.off[2] + (b - d.role_start[2]) * (kSegBlock / 64) + threadIdx.x / 64;
-    const uint32_t j = threadIdx.x % 64;
-    if (idx < d.off[3]) {
-      const uint32_t v = d.list[idx];
-      float s = 0.f;
-      for (uint32_t k = rp[v] + j; k < rp[v + 1]; k += 64) s += op.term(v, k);
-      s = group_sum<64>(s);
-      if (j == 0) op.finish(v, s, err);
-    }
-  } else if (b < d.role_start[4]) {
-    const uint32_t idx = d.off[1] + (b - d.role_start[3]) * (kSegBlock / 8) + threadIdx.x / 8;
-    const uint32_t j = threadIdx.x % 8;
-    const bool ok = idx < d.off[2];
-    float s = 0.f;
-    uint32_t v = 0;
-    if (ok) {
-      v = d.list[idx];
-      for (uint32_t k = rp[v] + j; k < rp[v + 1]; k += 8) s += op.term(v, k);
-    }
-    s = group_sum<8>(s);
-    if (ok && j == 0) op.finish(v, s, err);
-  } else {
-    const uint32_t idx = d.off[0] + (b - d.role_start[4]) * kSegBlock + threadIdx.x;
-    if (idx < d.off[1]) {
-      const uint32_t v = d.list[idx];
-      float s = 0.f;
-      for (uint32_t k = rp[v]; k < rp[v + 1]; ++k) s += op.term(v, k);
-      op.finish(v, s, err);
-    }
+// Sum of op.term(v, k) for k = beg + j, beg + j + G, ... < end.
+template <int G, class Op>
+__device__ __forceinline__ float lane_sum(const Op &op, uint32_t v, uint32_t beg, uint32_t end, uint32_t j) {
+  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+  uint32_t k = beg + j;
+  for (; OLV_SEG_UNROLL == 4 && k + 3u * G < end; k += 4u * G) {
+    const float x0 = op.term(v, k);
+    const float x1 = op.term(v, k + G);
+    const float x2 = op.term(v, k + 2u * G);
+    const float x3 = op.term(v, k + 3u * G);
+    s0 += x0;
+    s1 += x1;
+    s2 += x2;
+    s3 += x3;
   }
-  err = block_sum<kSegBlock, float>(err, s_red);
-  if (threadIdx.x == 0) err_part[b] = err;
+  for (; k < end; k += G) s0 += op.term(v, k);
+  return (s0 + s1) + (s2 + s3);
 }
 
 // Iteration control block (device).
@@ -135,29 +107,106 @@ struct IterCtl {
   uint32_t iters;     // updates performed
   double eps;         // stop threshold on the L1 residual
   double residual;    // last residual
+  uint32_t arrived;   // workgroups finished in the current launch
+  uint32_t pad;
 };
 
-// Single-workgroup epilogue of one iteration: finish the chunked hub rows
-// (partials summed in chunk order), total the residual, set `done`.
-template <class Op>
-__global__ void __launch_bounds__(1024) seg_finalize_kernel(SegDev d, Op op, const float *__restrict__ err_part,
-                                                            uint32_t nparts, IterCtl *__restrict__ ctl) {
-  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  __shared__ double s_red[1024 / kWave];
+// Finish the chunked hub rows (partials in chunk order), total the
+// residual, set `done`, re-arm the arrival counter.  Called by the last
+// workgroup of a launch (fused) or by a one-workgroup kernel.
+template <int BLOCK, class Op>
+__device__ void seg_finalize(const SegDev &d, const Op &op, const float *err_part, uint32_t nparts, IterCtl *ctl) {
+  __shared__ double s_red[BLOCK / kWave];
   float err = 0.f;
-  for (uint32_t h = threadIdx.x; h < d.nhub; h += blockDim.x) {
+  for (uint32_t h = threadIdx.x; h < d.nhub; h += BLOCK) {
     float s = 0.f;
-    for (uint32_t c = d.hub_first[h]; c < d.hub_first[h + 1]; ++c) s += d.chunk_partial[c];
+    for (uint32_t c = d.hub_first[h]; c < d.hub_first[h + 1]; ++c)
+      s += __hip_atomic_load(&d.chunk_partial[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     op.finish(d.hub_vertex[h], s, err);
   }
   double tot = err;
-  for (uint32_t i = threadIdx.x; i < nparts; i += blockDim.x) tot += err_part[i];
-  tot = block_sum<1024, double>(tot, s_red);
+  for (uint32_t i = threadIdx.x; i < nparts; i += BLOCK)
+    tot += __hip_atomic_load(&err_part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  tot = block_sum<BLOCK, double>(tot, s_red);
   if (threadIdx.x == 0) {
     ctl->residual = tot;
     ctl->iters += 1;
     if (tot < ctl->eps) ctl->done = 1u;
+    ctl->arrived = 0u;
   }
+}
+
+// Op: float term(uint32_t v, uint32_t k)   -- summand of slot k of row v
+//     void finish(uint32_t v, float sum, float &err)
+// ctl == nullptr: no residual (pure edge pass, e.g. gradients).  The
+// residual partials are totalled by seg_finalize_kernel in a second launch:
+// a last-workgroup finalize inside this kernel would need an agent-scope
+// release fence per workgroup, which on gfx950 writes back the XCD's whole
+// L2 each time.
+template <class Op>
+__global__ void __launch_bounds__(kSegBlock) seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op,
+                                                             IterCtl *ctl, float *__restrict__ err_part) {
+  if (ctl && __hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  __shared__ float s_red[kSegBlock / kWave];
+  float err = 0.f;
+  const uint32_t nvb = d.role_start[5];
+  for (uint32_t b = blockIdx.x; b < nvb; b += gridDim.x) {
+    if (b < d.role_start[1]) {
+      // hub chunk: partial sum only
+      const uint32_t c = b;
+      const uint32_t h = d.chunk_hub[c];
+      const uint32_t v = d.hub_vertex[h];
+      const uint32_t k0 = c - d.hub_first[h];
+      const uint32_t beg = rp[v] + k0 * d.chunk;
+      const uint32_t end = min(rp[v + 1], beg + d.chunk);
+      float s = lane_sum<kSegBlock>(op, v, beg, end, threadIdx.x);
+      s = block_sum<kSegBlock, float>(s, s_red);
+      if (threadIdx.x == 0) d.chunk_partial[c] = s;
+    } else if (b < d.role_start[2]) {
+      const uint32_t v = d.list[d.off[3] + (b - d.role_start[1])];
+      float s = lane_sum<kSegBlock>(op, v, rp[v], rp[v + 1], threadIdx.x);
+      s = block_sum<kSegBlock, float>(s, s_red);
+      if (threadIdx.x == 0) op.finish(v, s, err);
+    } else if (b < d.role_start[3]) {
+      const uint32_t idx = d.off[2] + (b - d.role_start[2]) * (kSegBlock / 64) + threadIdx.x / 64;
+      const uint32_t j = threadIdx.x % 64;
+      if (idx < d.off[3]) {
+        const uint32_t v = d.list[idx];
+        float s = lane_sum<64>(op, v, rp[v], rp[v + 1], j);
+        s = group_sum<64>(s);
+        if (j == 0) op.finish(v, s, err);
+      }
+    } else if (b < d.role_start[4]) {
+      const uint32_t idx = d.off[1] + (b - d.role_start[3]) * (kSegBlock / 8) + threadIdx.x / 8;
+      const uint32_t j = threadIdx.x % 8;
+      const bool ok = idx < d.off[2];
+      float s = 0.f;
+      uint32_t v = 0;
+      if (ok) {
+        v = d.list[idx];
+        s = lane_sum<8>(op, v, rp[v], rp[v + 1], j);
+      }
+      s = group_sum<8>(s);
+      if (ok && j == 0) op.finish(v, s, err);
+    } else {
+      const uint32_t idx = d.off[0] + (b - d.role_start[4]) * kSegBlock + threadIdx.x;
+      if (idx < d.off[1]) {
+        const uint32_t v = d.list[idx];
+        const float s = lane_sum<1>(op, v, rp[v], rp[v + 1], 0u);
+        op.finish(v, s, err);
+      }
+    }
+  }
+  if (!ctl) return;
+  err = block_sum<kSegBlock, float>(err, s_red);
+  if (threadIdx.x == 0) err_part[blockIdx.x] = err;
+}
+
+template <class Op>
+__global__ void __launch_bounds__(1024) seg_finalize_kernel(SegDev d, Op op, const float *__restrict__ err_part,
+                                                            uint32_t nparts, IterCtl *__restrict__ ctl) {
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  seg_finalize<1024>(d, op, err_part, nparts, ctl);
 }
 
 }  // namespace olv

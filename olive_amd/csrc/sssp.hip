@@ -11,13 +11,15 @@
 // distance dropped this round is marked in a bitmap and expanded next round.
 //
 // k sources run as k instances over one shared CSR: instance s owns virtual
-// vertices s*n .. s*n+n-1 (dist row s), so one persistent launch relaxes all
-// k frontiers together and each CSR row is read once per round for every
-// instance that has it on its frontier.
+// vertices s*n .. s*n+n-1 (dist row s), so one launch per round relaxes all
+// k frontiers together.  Rounds are a relax kernel and a compact kernel
+// whose control lives on the device (as for BFS); OLV_PERSISTENT runs all
+// rounds in one cooperative kernel instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "bfs_core.hpp"
@@ -26,7 +28,7 @@
 namespace olv {
 
 constexpr int kSBlock = 256;
-constexpr uint32_t kSSpin = 1u << 26;
+constexpr uint32_t kSSpin = 1u << 24;
 
 template <bool Multi>
 struct SsspVisitor {
@@ -66,130 +68,178 @@ struct SsspVisitor {
   }
 };
 
+struct SsspCtl {
+  uint32_t round;     // next round to relax (written by compact)
+  uint32_t round_e;   // round being compacted (written by relax)
+  uint32_t done;
+  uint32_t rounds;    // result
+  uint32_t capped;    // result: n rounds without convergence
+  uint32_t pad;
+  uint32_t qfill[2];  // frontier queue length of round R, slot R & 1
+  uint32_t hfill[2];  // hub items of round R, slot R & 1
+  unsigned long long relax;  // result: edges read
+  BfsCtr cnt[3];
+};
+
 struct SsspParams {
   const uint32_t *row_ptr;
   const uint32_t *col;
   const int32_t *w;
   int32_t *dist;     // ninst x n
   uint32_t *queue;   // virtual ids
+  uint2 *hubs;       // hub items (virtual id, chunk)
   uint32_t *bms;     // 2 x nwp
-  BfsCtr *ctr;       // 3 slots
-  uint32_t *bar;     // [0..2] barrier, [4] rounds, [6] hit cap
-  unsigned long long *relax;  // edges read
+  SsspCtl *ctl;
+  GridBar *bar;
   uint32_t n, ninst, nw, nwp;
-  uint32_t wave_thr, block_thr;
+  uint32_t wave_thr, block_thr, hub_thr;
   uint32_t max_rounds;
 };
 
+// Virtual id -> CSR row.
+struct ModRow {
+  uint32_t n;
+  __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v % n; }
+};
+
 template <bool Multi>
-__device__ __forceinline__ void sssp_round(const SsspParams &p, uint32_t qsize, uint32_t *nextb, uint32_t vb,
-                                           uint32_t nvb, uint32_t &nf, unsigned long long &insp) {
+__device__ __forceinline__ void sssp_round(const SsspParams &p, uint32_t qsize, uint32_t nhubs, uint32_t *nextb,
+                                           uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &insp) {
   SsspVisitor<Multi> vis(p.col, p.w, p.dist, nextb, p.n);
-  push_expand<kSBlock>(p.queue, qsize, p.row_ptr, p.wave_thr, p.block_thr, vb, nvb, vis, insp);
+  push_expand<kSBlock>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, vb, nvb, vis, insp);
   nf += vis.nf;
 }
 
 template <bool Multi>
-__global__ void __launch_bounds__(kSBlock) sssp_persistent_kernel(SsspParams p) {
-  uint32_t gen = 0;
-  const uint32_t vb = blockIdx.x, nvb = gridDim.x;
-  uint32_t R = 0;
-  unsigned long long insp_all = 0;
-  for (;;) {
-    BfsCtr *cur = &p.ctr[R % 3];
-    BfsCtr *nxt = &p.ctr[(R + 1) % 3];
-    uint32_t *nextb = p.bms + (size_t)(R & 1) * p.nwp;
-    uint32_t *clearb = p.bms + (size_t)((R + 1) & 1) * p.nwp;
-    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) reinterpret_cast<uint32_t *>(nxt)[threadIdx.x] = 0u;
-    for (uint32_t i = vb * kSBlock + threadIdx.x; i < p.nwp; i += nvb * kSBlock) clearb[i] = 0u;
-    uint32_t nf = 0;
-    unsigned long long insp = 0;
-    const uint32_t qsize = __hip_atomic_load(&cur->qsize, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    sssp_round<Multi>(p, qsize, nextb, vb, nvb, nf, insp);
-    insp_all += insp;
-    flush_counters<kSBlock>(cur, nf, 0ull, 0ull);
-    if (!grid_barrier(p.bar, nvb, gen, kSSpin)) break;
-    const uint32_t tnf = __hip_atomic_load(&cur->nf, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ++R;
-    if (tnf == 0) break;
-    if (R >= p.max_rounds) {
-      if (vb == 0 && threadIdx.x == 0) p.bar[6] = 1u;
-      break;
+__global__ void __launch_bounds__(kSBlock) sssp_relax_kernel(SsspParams p) {
+  SsspCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t R = ctl->round;
+  const uint32_t qsize = ctl->qfill[R & 1], nhubs = ctl->hfill[R & 1];
+  uint32_t *nextb = p.bms + (size_t)(R & 1) * p.nwp;
+  uint32_t *clearb = p.bms + (size_t)((R + 1) & 1) * p.nwp;
+  for (uint32_t i = blockIdx.x * kSBlock + threadIdx.x; i < p.nwp; i += gridDim.x * kSBlock) clearb[i] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+    if (threadIdx.x == 0) {
+      ctl->round_e = R;
+      ctl->qfill[(R + 1) & 1] = 0;
+      ctl->hfill[(R + 1) & 1] = 0;
     }
-    compact_step<kSBlock>(nextb, p.nw, p.queue, &nxt->qsize, vb, nvb);
-    if (!grid_barrier(p.bar, nvb, gen, kSSpin)) break;
+    reinterpret_cast<uint32_t *>(&ctl->cnt[(R + 1) % 3])[threadIdx.x] = 0u;
   }
-  __shared__ unsigned long long s_red[kSBlock / kWave];
-  insp_all = block_sum<kSBlock, unsigned long long>(insp_all, s_red);
-  if (threadIdx.x == 0 && insp_all) atomicAdd(p.relax, insp_all);
-  if (vb == 0 && threadIdx.x == 0) p.bar[4] = R;
+  uint32_t nf = 0;
+  unsigned long long insp = 0;
+  sssp_round<Multi>(p, qsize, nhubs, nextb, blockIdx.x, gridDim.x, nf, insp);
+  flush_counters<kSBlock>(&ctl->cnt[R % 3], nf, 0ull, insp);
+}
+
+__global__ void __launch_bounds__(kSBlock) sssp_compact_kernel(SsspParams p) {
+  SsspCtl *ctl = p.ctl;
+  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  const uint32_t R = ctl->round_e;
+  const BfsCtr c = ctl->cnt[R % 3];
+  const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+  if (lead) ctl->relax += c.inspected;
+  if (c.nf == 0 || R + 1 >= p.max_rounds) {
+    if (lead) {
+      ctl->rounds = R + 1;
+      ctl->capped = c.nf != 0;
+      ctl->done = 1u;
+    }
+    return;
+  }
+  if (lead) ctl->round = R + 1;
+  compact_step<kSBlock>(p.bms + (size_t)(R & 1) * p.nwp, p.nw, p.row_ptr, p.hub_thr, p.queue, &ctl->qfill[(R + 1) & 1],
+                        p.hubs, &ctl->hfill[(R + 1) & 1], blockIdx.x, gridDim.x, ModRow{p.n});
 }
 
 template <bool Multi>
-__global__ void __launch_bounds__(kSBlock) sssp_round_kernel(SsspParams p, uint32_t qsize, uint32_t *nextb,
-                                                             BfsCtr *ctr) {
-  uint32_t nf = 0;
-  unsigned long long insp = 0;
-  sssp_round<Multi>(p, qsize, nextb, blockIdx.x, gridDim.x, nf, insp);
-  flush_counters<kSBlock>(ctr, nf, 0ull, insp);
+__global__ void __launch_bounds__(kSBlock) sssp_persistent_kernel(SsspParams p) {
+  GridBarLocal gb;
+  if (!gbar_init(p.bar, gb, gridDim.x, kSSpin)) return;
+  SsspCtl *ctl = p.ctl;
+  const uint32_t vb = blockIdx.x, nvb = gridDim.x;
+  const bool lead = vb == 0 && threadIdx.x == 0;
+  unsigned long long insp_all = 0;
+  for (uint32_t R = 0;; ++R) {
+    uint32_t *nextb = p.bms + (size_t)(R & 1) * p.nwp;
+    uint32_t *clearb = p.bms + (size_t)((R + 1) & 1) * p.nwp;
+    const uint32_t qsize = __hip_atomic_load(&ctl->qfill[R & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t nhubs = __hip_atomic_load(&ctl->hfill[R & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t i = vb * kSBlock + threadIdx.x; i < p.nwp; i += nvb * kSBlock) clearb[i] = 0u;
+    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+      if (threadIdx.x == 0) {
+        ctl->qfill[(R + 1) & 1] = 0;
+        ctl->hfill[(R + 1) & 1] = 0;
+      }
+      reinterpret_cast<uint32_t *>(&ctl->cnt[(R + 1) % 3])[threadIdx.x] = 0u;
+    }
+    uint32_t nf = 0;
+    unsigned long long insp = 0;
+    sssp_round<Multi>(p, qsize, nhubs, nextb, vb, nvb, nf, insp);
+    flush_counters<kSBlock>(&ctl->cnt[R % 3], nf, 0ull, insp);
+    if (!grid_barrier(p.bar, gb, kSSpin)) break;
+    const BfsCtr c = ctl->cnt[R % 3];
+    insp_all += c.inspected;
+    if (c.nf == 0 || R + 1 >= p.max_rounds) {
+      if (lead) {
+        ctl->rounds = R + 1;
+        ctl->capped = c.nf != 0;
+        ctl->relax = insp_all;
+        ctl->done = 1u;
+      }
+      break;
+    }
+    compact_step<kSBlock>(nextb, p.nw, p.row_ptr, p.hub_thr, p.queue, &ctl->qfill[(R + 1) & 1], p.hubs,
+                          &ctl->hfill[(R + 1) & 1], vb, nvb, ModRow{p.n});
+    if (!grid_barrier(p.bar, gb, kSSpin)) break;
+  }
 }
 
-__global__ void __launch_bounds__(kSBlock) sssp_compact_kernel(const uint32_t *bm, uint32_t nw, uint32_t *queue,
-                                                               uint32_t *count) {
-  compact_step<kSBlock>(bm, nw, queue, count, blockIdx.x, gridDim.x);
-}
-
-// dist = INF except the sources, bitmaps/counters/barrier = 0, queue =
-// the distinct virtual source ids.
+// dist = INF, bitmaps/control/barrier = 0.
 __global__ void sssp_reset_kernel(int32_t *__restrict__ dist, size_t total, uint32_t *__restrict__ bms,
-                                  uint32_t nwp, BfsCtr *__restrict__ ctr, uint32_t *__restrict__ bar,
-                                  unsigned long long *__restrict__ relax) {
+                                  uint32_t nwp, SsspCtl *__restrict__ ctl, GridBar *__restrict__ bar) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = tid; i < total; i += stride) dist[i] = kInf;
   for (size_t i = tid; i < (size_t)nwp * 2; i += stride) bms[i] = 0u;
-  if (tid < sizeof(BfsCtr) * 3 / 4) reinterpret_cast<uint32_t *>(ctr)[tid] = 0u;
-  if (tid < 16) bar[tid] = 0u;
-  if (tid == 0) *relax = 0ull;
+  for (size_t i = tid; i < sizeof(SsspCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
+  for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
 }
 
-__global__ void sssp_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k, uint32_t n, int32_t *__restrict__ dist,
-                                 uint32_t *__restrict__ queue, BfsCtr *__restrict__ ctr) {
-  for (uint32_t s = threadIdx.x; s < k; s += blockDim.x) {
-    const uint32_t x = s * n + srcs[s];
-    dist[x] = 0;
-    queue[s] = x;
-  }
-  if (threadIdx.x == 0) ctr[0].qsize = k;
-}
-
-__global__ void sssp_clear_kernel(BfsCtr *c, uint32_t *bm, uint32_t nwp) {
-  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) reinterpret_cast<uint32_t *>(c)[threadIdx.x] = 0u;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nwp; i += gridDim.x * blockDim.x) bm[i] = 0u;
+// One workgroup: instance s starts from virtual vertex s*n + srcs[s].
+__global__ void sssp_seed_kernel(const uint32_t *__restrict__ srcs, uint32_t k, SsspParams p) {
+  auto claim = [&](uint32_t s) -> uint32_t {
+    const uint32_t x = s * p.n + srcs[s];
+    p.dist[x] = 0;
+    return x;
+  };
+  unsigned long long deg;
+  seed_frontier(srcs, k, p.row_ptr, p.hub_thr, p.queue, &p.ctl->qfill[0], p.hubs, &p.ctl->hfill[0], claim,
+                ModRow{p.n}, &deg);
 }
 
 struct SsspWork {
   uint32_t nw = 0, nwp = 0;
-  uint32_t *queue = nullptr, *bms = nullptr, *bar = nullptr, *h_bar = nullptr;
-  BfsCtr *ctr = nullptr, *h_ctr = nullptr;
-  unsigned long long *relax = nullptr;
+  uint32_t *queue = nullptr, *bms = nullptr;
+  uint2 *hubs = nullptr;
+  SsspCtl *ctl = nullptr, *h_ctl = nullptr;
+  GridBar *bar = nullptr;
   ~SsspWork() {
-    void *ps[] = {queue, bms, bar, ctr, relax};
+    void *ps[] = {queue, bms, hubs, ctl, bar};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    if (h_bar) (void)hipHostFree(h_bar);
-    if (h_ctr) (void)hipHostFree(h_ctr);
+    if (h_ctl) (void)hipHostFree(h_ctl);
   }
-  int alloc(uint64_t nv) {
+  int alloc(uint64_t nv, uint32_t k, uint64_t m) {
     nw = (uint32_t)((nv + 31) / 32);
     nwp = (nw + 63) / 64 * 64;
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&queue), (size_t)nv * 4));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&hubs), hub_capacity(m) * k * sizeof(uint2)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bms), (size_t)nwp * 8));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bar), 64));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&ctr), sizeof(BfsCtr) * 3));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&relax), 8));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_bar), 64, hipHostMallocDefault));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctr), sizeof(BfsCtr) * 3, hipHostMallocDefault));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&ctl), sizeof(SsspCtl)));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bar), sizeof(GridBar)));
+    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctl), sizeof(SsspCtl), hipHostMallocDefault));
     return OLV_OK;
   }
 };
@@ -199,6 +249,12 @@ static unsigned sgrid(size_t items, size_t per) {
   return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * 8));
 }
 
+static int sssp_read(SsspWork &w) {
+  OLV_HIP(hipMemcpyAsync(w.h_ctl, w.ctl, sizeof(SsspCtl), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  return OLV_OK;
+}
+
 // k instances from device sources d_srcs[0..k); dist is k x n.
 static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, const int32_t *weights, int32_t *dist,
                     unsigned flags, olv_stats *st) {
@@ -206,88 +262,74 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
   if (nv >= (1ull << 32)) return fail(OLV_ERANGE, "sssp: %u sources x %u vertices exceed 2^32 ids", k, g->n);
   const bool multi = k > 1;
   SsspWork w;
-  OLV_TRY(w.alloc(nv));
+  OLV_TRY(w.alloc(nv, k, g->m));
   SsspParams p;
   p.row_ptr = g->row_ptr;
   p.col = g->col;
   p.w = weights;
   p.dist = dist;
   p.queue = w.queue;
+  p.hubs = w.hubs;
   p.bms = w.bms;
-  p.ctr = w.ctr;
+  p.ctl = w.ctl;
   p.bar = w.bar;
-  p.relax = w.relax;
   p.n = g->n;
   p.ninst = k;
   p.nw = w.nw;
   p.nwp = w.nwp;
   p.wave_thr = g->tune.td_wave_thr;
   p.block_thr = g->tune.td_block_thr;
+  p.hub_thr = g->tune.hub_thr;
   p.max_rounds = g->n;
   hipStream_t s = stream();
   Timer t;
   t.start();
   hipLaunchKernelGGL(sssp_reset_kernel, dim3(sgrid(std::max<size_t>(nv, (size_t)w.nwp * 2), 256)), dim3(256), 0, s,
-                     dist, (size_t)nv, w.bms, w.nwp, w.ctr, w.bar, w.relax);
-  hipLaunchKernelGGL(sssp_seed_kernel, dim3(1), dim3(256), 0, s, d_srcs, k, g->n, dist, w.queue, w.ctr);
+                     dist, (size_t)nv, w.bms, w.nwp, w.ctl, w.bar);
+  hipLaunchKernelGGL(sssp_seed_kernel, dim3(1), dim3(256), 0, s, d_srcs, k, p);
   OLV_HIP(hipGetLastError());
-  uint32_t rounds = 0;
-  bool capped = false;
-  unsigned long long relax = 0;
-  if (flags & OLV_HOST_LOOP) {
-    uint32_t qsize = k;
-    for (;;) {
-      uint32_t *nextb = w.bms + (size_t)(rounds & 1) * w.nwp;
-      hipLaunchKernelGGL(sssp_clear_kernel, dim3(sgrid(w.nwp, 256)), dim3(256), 0, s, w.ctr,
-                         w.bms + (size_t)((rounds + 1) & 1) * w.nwp, w.nwp);
-      if (multi)
-        hipLaunchKernelGGL(sssp_round_kernel<true>, dim3(sgrid(qsize, kSBlock)), dim3(kSBlock), 0, s, p, qsize, nextb,
-                           w.ctr);
-      else
-        hipLaunchKernelGGL(sssp_round_kernel<false>, dim3(sgrid(qsize, kSBlock)), dim3(kSBlock), 0, s, p, qsize,
-                           nextb, w.ctr);
-      OLV_HIP(hipGetLastError());
-      OLV_HIP(hipMemcpyAsync(w.h_ctr, w.ctr, sizeof(BfsCtr), hipMemcpyDeviceToHost, s));
-      OLV_HIP(hipStreamSynchronize(s));
-      const uint32_t nf = w.h_ctr[0].nf;
-      relax += w.h_ctr[0].inspected;
-      ++rounds;
-      if (nf == 0) break;
-      if (rounds >= p.max_rounds) {
-        capped = true;
-        break;
-      }
-      hipLaunchKernelGGL(sssp_clear_kernel, dim3(1), dim3(64), 0, s, w.ctr, (uint32_t *)nullptr, 0u);
-      hipLaunchKernelGGL(sssp_compact_kernel, dim3(sgrid(w.nw, kSBlock)), dim3(kSBlock), 0, s, (const uint32_t *)nextb,
-                         w.nw, w.queue, &w.ctr[0].qsize);
-      qsize = nf;
-    }
-    t.stop();
-  } else {
+  if (flags & OLV_PERSISTENT) {
     const void *fn = multi ? reinterpret_cast<const void *>(sssp_persistent_kernel<true>)
                            : reinterpret_cast<const void *>(sssp_persistent_kernel<false>);
     int per_cu = 0;
     OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, kSBlock, 0));
     if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "persistent SSSP kernel does not fit on a CU");
-    const int grid = std::min(per_cu, 2) * cu_count();
     void *args[] = {&p};
-    OLV_HIP(hipLaunchCooperativeKernel(fn, dim3(grid), dim3(kSBlock), args, 0, s));
-    t.stop();
-    OLV_HIP(hipMemcpyAsync(w.h_bar, w.bar, 64, hipMemcpyDeviceToHost, s));
-    OLV_HIP(hipMemcpyAsync(&relax, w.relax, 8, hipMemcpyDeviceToHost, s));
-    OLV_HIP(hipStreamSynchronize(s));
-    if (w.h_bar[2]) return fail(OLV_EINTERNAL, "olv_sssp: grid barrier timed out (co-residency lost)");
-    rounds = w.h_bar[4];
-    capped = w.h_bar[6] != 0;
+    OLV_HIP(hipLaunchCooperativeKernel(fn, dim3(cu_count()), dim3(kSBlock), args, 0, s));
+    OLV_TRY(sssp_read(w));
+    GridBar hb;
+    OLV_HIP(hipMemcpy(&hb, w.bar, sizeof(hb), hipMemcpyDeviceToHost));
+    if (hb.err || !w.h_ctl->done) return fail(OLV_EINTERNAL, "olv_sssp: grid barrier timed out (co-residency lost)");
+  } else {
+    const unsigned gr = (unsigned)cu_count() * 8;
+    const unsigned gc = sgrid(w.nw, kSBlock);
+    uint32_t chunk = (flags & OLV_HOST_LOOP) ? 1u : 8u;
+    uint64_t enq = 0;
+    for (;;) {
+      for (uint32_t i = 0; i < chunk; ++i) {
+        if (multi)
+          hipLaunchKernelGGL(sssp_relax_kernel<true>, dim3(gr), dim3(kSBlock), 0, s, p);
+        else
+          hipLaunchKernelGGL(sssp_relax_kernel<false>, dim3(gr), dim3(kSBlock), 0, s, p);
+        hipLaunchKernelGGL(sssp_compact_kernel, dim3(gc), dim3(kSBlock), 0, s, p);
+      }
+      OLV_HIP(hipGetLastError());
+      enq += chunk;
+      OLV_TRY(sssp_read(w));
+      if (w.h_ctl->done) break;
+      if (enq > (uint64_t)g->n + 2) return fail(OLV_EINTERNAL, "olv_sssp: round loop did not terminate");
+      if (!(flags & OLV_HOST_LOOP)) chunk = std::min(chunk * 2, 256u);
+    }
   }
+  t.stop();
   const double ms = t.ms();
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
-    st->iterations = rounds;
-    st->edges_inspected = relax;
+    st->iterations = w.h_ctl->rounds;
+    st->edges_inspected = w.h_ctl->relax;
   }
-  if (capped) return fail(OLV_ENOCONV, "olv_sssp: still relaxing after n=%u rounds (negative cycle)", g->n);
+  if (w.h_ctl->capped) return fail(OLV_ENOCONV, "olv_sssp: still relaxing after n=%u rounds (negative cycle)", g->n);
   return OLV_OK;
 }
 
@@ -325,16 +367,13 @@ int reach_stats(const olv_graph *g, const int32_t *d, uint32_t k, olv_stats *st)
   return OLV_OK;
 }
 
-static int sssp_entry(const olv_graph *g, const uint32_t *srcs, uint32_t k, bool srcs_host, const int32_t *weights,
-                      int32_t *dist, unsigned flags, olv_stats *st) {
+static int sssp_entry(const olv_graph *g, const uint32_t *srcs, uint32_t k, const int32_t *weights, int32_t *dist,
+                      unsigned flags, olv_stats *st) {
   if (!g || !dist || !srcs) return fail(OLV_EINVAL, "olv_sssp: NULL argument");
   if (k == 0) return fail(OLV_EINVAL, "olv_sssp: no source");
   const bool host = (flags & OLV_PTR_HOST) != 0;
-  std::vector<uint32_t> hs(k);
-  if (srcs_host) std::memcpy(hs.data(), srcs, (size_t)k * 4);
-  else OLV_HIP(hipMemcpy(hs.data(), srcs, (size_t)k * 4, hipMemcpyDeviceToHost));
-  for (uint32_t s : hs)
-    if (s >= g->n) return fail(OLV_EINVAL, "olv_sssp: source %u out of range (n=%u)", s, g->n);
+  for (uint32_t i = 0; i < k; ++i)
+    if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_sssp: source %u out of range (n=%u)", srcs[i], g->n);
   DevBuf dw, dd, ds;
   const int32_t *w = weights;
   if (!w) {
@@ -352,7 +391,7 @@ static int sssp_entry(const olv_graph *g, const uint32_t *srcs, uint32_t k, bool
     out = dd.as<int32_t>();
   }
   OLV_TRY(ds.alloc((size_t)k * 4));
-  OLV_HIP(hipMemcpy(ds.p, hs.data(), (size_t)k * 4, hipMemcpyHostToDevice));
+  OLV_HIP(hipMemcpy(ds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice));
   OLV_TRY(sssp_run(g, ds.as<uint32_t>(), k, w, out, flags, st));
   if (st) OLV_TRY(reach_stats(g, out, k, st));
   if (host) OLV_HIP(hipMemcpy(dist, out, total * 4, hipMemcpyDeviceToHost));
@@ -366,10 +405,10 @@ using namespace olv;
 extern "C" int olv_sssp(const olv_graph *g, uint32_t src, const int32_t *weights, int32_t *dist, unsigned flags,
                         olv_stats *st) {
   if (!g) return fail(OLV_EINVAL, "olv_sssp: NULL graph");
-  return sssp_entry(g, &src, 1, true, weights, dist, flags, st);
+  return sssp_entry(g, &src, 1, weights, dist, flags, st);
 }
 
 extern "C" int olv_sssp_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k, const int32_t *weights,
                                 int32_t *dist, unsigned flags, olv_stats *st) {
-  return sssp_entry(g, srcs, k, true, weights, dist, flags, st);
+  return sssp_entry(g, srcs, k, weights, dist, flags, st);
 }

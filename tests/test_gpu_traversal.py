@@ -15,10 +15,12 @@ INF = 0x7FFFFFFF
 G = np.load(os.path.join(GOLDEN, "golden.npz"))
 
 MODES = {
-    "persistent": 0,
-    "host_loop": 0x8,
-    "push_only": 0x2,
+    "pipeline": 0,                      # device-controlled level kernels
+    "host_loop": 0x8,                   # host waits for every level
+    "push_only": 0x2,                   # no bottom-up steps
     "push_only_host_loop": 0x2 | 0x8,
+    "persistent": 0x20,                 # one cooperative kernel, grid barrier
+    "persistent_push_only": 0x20 | 0x2,
 }
 
 
@@ -78,9 +80,9 @@ def test_bfs_rmat22_full_size(olv):
     g = olv.Graph(csr)
     s = int(np.argmax(csr.outdeg()))
     ref = oracle.bfs(csr.row_ptr, csr.col, s)
-    for fl in (0, 0x8):
+    for fl in (0x8, 0x20, 0):
         out, st = g.bfs(s, flags=fl)
-        assert np.array_equal(out.numpy(), ref)
+        assert np.array_equal(out.numpy(), ref), fl
     assert st.bottom_up_levels > 0
 
 
@@ -109,7 +111,7 @@ def test_bfs_errors(olv):
 
 
 # -------------------------------------------------------------------- SSSP
-@pytest.mark.parametrize("mode", ["persistent", "host_loop"])
+@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent"])
 def test_sssp_weighted_data_files(olv, mode):
     for name in DATA_FILES:
         csr = olv.load_edge_list(data_file(name))
@@ -122,7 +124,7 @@ def test_sssp_weighted_data_files(olv, mode):
             assert np.array_equal(out.numpy(), G[f"{name}/sssp/{s}"]), (name, s)
 
 
-@pytest.mark.parametrize("mode", ["persistent", "host_loop"])
+@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent"])
 def test_sssp_uniform(olv, mode):
     say(f"sssp uniform {mode}")
     csr = olv.with_weights_i32(olv.gen_uniform(1 << 16, 1 << 19, seed=7), 1, 64, seed=7)
@@ -159,7 +161,7 @@ def test_sssp_rmat_heavy_weights(olv):
 def test_sssp_negative_cycle_reports_enoconv(olv):
     csr = olv.CsrGraph(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.array([-1, -1], np.int32))
     g = olv.Graph(csr)
-    for fl in (0, 0x8):
+    for fl in (0, 0x8, 0x20):
         with pytest.raises(olv.OliveError) as e:
             g.sssp(0, flags=fl)
         assert e.value.code == 6
