@@ -1,0 +1,235 @@
+// CSR graph and file loaders of the Olive C++ API (csrGraph.h:54-287 of the
+// reference).  Layout as the reference: `vertices[n+1]` row offsets and
+// `edges[m]` column ids (csrGraph.h:61-63).
+//
+// Differences, all deliberate:
+//  * edge-list input need not be sorted: tuples are stable-sorted by source
+//    (the sort is commented out at csrGraph.h:149, which corrupts unsorted
+//    files such as data/disorderedGraph_9);
+//  * a third column is kept as the edge value (dropped at csrGraph.h:139);
+//  * arrays are released with delete[] (csrGraph.h:70-75 uses delete).
+#ifndef OLIVE_CSR_GRAPH_H
+#define OLIVE_CSR_GRAPH_H
+
+#include <algorithm>
+#include <iostream>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "common.h"
+#include "edgeTuple.h"
+#include "logging.h"
+#include "timer.h"
+#include "utils.h"
+
+template <typename VertexValue, typename EdgeValue>
+class CsrGraph {
+ public:
+  VertexId vertexCount;
+  EdgeId edgeCount;
+  EdgeId *vertices;
+  VertexId *edges;
+  EdgeValue *edgeValues;
+  VertexValue *vertexValues;
+  bool hasEdgeValues;
+
+  CsrGraph()
+      : vertexCount(0), edgeCount(0), vertices(NULL), edges(NULL), edgeValues(NULL), vertexValues(NULL),
+        hasEdgeValues(false) {}
+  CsrGraph(const CsrGraph &) = delete;
+  CsrGraph &operator=(const CsrGraph &) = delete;
+  ~CsrGraph() { release(); }
+
+  void release() {
+    delete[] vertices;
+    delete[] edges;
+    delete[] edgeValues;
+    delete[] vertexValues;
+    vertices = NULL;
+    edges = NULL;
+    edgeValues = NULL;
+    vertexValues = NULL;
+    vertexCount = 0;
+    edgeCount = 0;
+    hasEdgeValues = false;
+  }
+
+  void initGraph(VertexId n, EdgeId m) {
+    release();
+    vertexCount = n;
+    edgeCount = m;
+    vertices = new EdgeId[(size_t)n + 1];
+    edges = new VertexId[m ? m : 1];
+    edgeValues = new EdgeValue[m ? m : 1]();
+    vertexValues = new VertexValue[n ? n : 1]();
+  }
+
+  /**
+   * Edge-list file: lines starting with '#' are comments; the first other
+   * line is "n m"; then m lines "src dst [value]".
+   */
+  void fromEdgeListFile(const char *path) {
+    FILE *file = fopen(path, "r");
+    if (file == NULL) {
+      LOG(ERROR) << "Can not open graph file: " << path;
+      return;
+    }
+    Stopwatch sw;
+    sw.start();
+    std::vector<char> buf;
+    {
+      char chunk[1 << 16];
+      size_t r;
+      while ((r = fread(chunk, 1, sizeof(chunk), file)) > 0) buf.insert(buf.end(), chunk, chunk + r);
+    }
+    fclose(file);
+    buf.push_back('\0');
+    long long n = -1, m = -1;
+    std::vector<VertexId> src, dst;
+    std::vector<long long> val;
+    bool allValues = true;
+    char *p = buf.data();
+    while (*p) {
+      char *eol = strchr(p, '\n');
+      if (eol) *eol = '\0';
+      char *s = p;
+      while (*s == ' ' || *s == '\t' || *s == '\r') ++s;
+      if (*s && *s != '#') {
+        char *q = s;
+        long long a = strtoll(q, &q, 10), b = strtoll(q, &q, 10);
+        if (n < 0) {
+          n = a;
+          m = b;
+          src.reserve((size_t)m);
+          dst.reserve((size_t)m);
+        } else {
+          char *before = q;
+          long long c = strtoll(q, &q, 10);
+          const bool has = q != before;
+          src.push_back((VertexId)a);
+          dst.push_back((VertexId)b);
+          val.push_back(has ? c : 0);
+          allValues = allValues && has;
+        }
+      }
+      if (!eol) break;
+      p = eol + 1;
+    }
+    if (n <= 0 || (long long)src.size() != m) {
+      LOG(ERROR) << "Malformed edge list " << path << ": header n=" << n << " m=" << m << ", " << src.size()
+                 << " edges";
+      return;
+    }
+    initGraph((VertexId)n, (EdgeId)m);
+    // stable counting sort by source
+    std::vector<EdgeId> count((size_t)n + 1, 0);
+    for (size_t e = 0; e < src.size(); ++e) {
+      if (src[e] >= (VertexId)n || dst[e] >= (VertexId)n) {
+        LOG(ERROR) << "Vertex id out of range in " << path;
+        release();
+        return;
+      }
+      count[src[e] + 1]++;
+    }
+    for (size_t v = 0; v < (size_t)n; ++v) count[v + 1] += count[v];
+    for (size_t v = 0; v <= (size_t)n; ++v) vertices[v] = count[v];
+    for (size_t e = 0; e < src.size(); ++e) {
+      const EdgeId pos = count[src[e]]++;
+      edges[pos] = dst[e];
+      if constexpr (std::is_arithmetic<EdgeValue>::value)
+        if (allValues) edgeValues[pos] = (EdgeValue)val[e];
+    }
+    hasEdgeValues = std::is_arithmetic<EdgeValue>::value && allValues && m > 0;
+    LOG(INFO) << "Parsed " << path << ": " << n << " vertices, " << m << " edges in " << sw.getElapsedMillis()
+              << "ms";
+  }
+
+  /**
+   * DIMACS/METIS adjacency file: header "n m" (m undirected edges, 2m
+   * arcs), then line i lists the 1-based neighbours of vertex i-1.
+   */
+  void fromDimacsFile(const char *path) {
+    FILE *file = fopen(path, "r");
+    if (file == NULL) {
+      LOG(ERROR) << "Can not open graph file: " << path;
+      return;
+    }
+    std::vector<std::vector<VertexId>> rows;
+    long long n = -1, m = -1;
+    char *line = NULL;
+    size_t cap = 0;
+    ssize_t len;
+    while ((len = getline(&line, &cap, file)) >= 0) {
+      char *s = line;
+      while (*s == ' ' || *s == '\t') ++s;
+      if (*s == '#') continue;
+      if (n < 0) {
+        if (*s == '\n' || *s == '\0') continue;
+        char *q = s;
+        n = strtoll(q, &q, 10);
+        m = strtoll(q, &q, 10);
+        continue;
+      }
+      std::vector<VertexId> r;
+      char *q = s;
+      for (;;) {
+        char *before = q;
+        long long x = strtoll(q, &q, 10);
+        if (q == before) break;
+        r.push_back((VertexId)(x - 1));
+      }
+      rows.push_back(r);
+      if ((long long)rows.size() == n) break;
+    }
+    free(line);
+    fclose(file);
+    if (n <= 0) {
+      LOG(ERROR) << "Malformed DIMACS file " << path;
+      return;
+    }
+    rows.resize((size_t)n);
+    size_t total = 0;
+    for (auto &r : rows) total += r.size();
+    if ((long long)total != 2 * m) {
+      LOG(ERROR) << total << " != " << 2 * m;
+      assert(0);
+    }
+    initGraph((VertexId)n, (EdgeId)total);
+    EdgeId e = 0;
+    for (size_t v = 0; v < (size_t)n; ++v) {
+      vertices[v] = e;
+      for (VertexId d : rows[v]) edges[e++] = d;
+    }
+    vertices[n] = e;
+  }
+
+  /** Degree histogram (and the adjacency when verbose). */
+  void print(bool verbose = false) const {
+    if (verbose) {
+      for (VertexId v = 0; v < vertexCount; v++) {
+        printf("[%u] ", v);
+        for (EdgeId e = vertices[v]; e < vertices[v + 1]; e++) printf(" ->%u", edges[e]);
+        printf("\n");
+      }
+    }
+    size_t degLog[34] = {0};
+    int slotMax = 0;
+    for (VertexId v = 0; v < vertexCount; v++) {
+      EdgeId deg = vertices[v + 1] - vertices[v];
+      int slot = 0;
+      while (deg > 0) {
+        deg /= 2;
+        slot++;
+      }
+      slotMax = std::max(slotMax, slot);
+      degLog[slot]++;
+    }
+    printf("deg[0]: 0\t%zu\t%.2f%%\n", degLog[0], (float)degLog[0] / vertexCount * 100);
+    for (int i = 1; i <= slotMax; i++)
+      printf("deg[%d]: (%d, %d]\t%zu\t%.2f%%\n", i, i >= 2 ? 1 << (i - 2) : 0, 1 << (i - 1), degLog[i],
+             (float)degLog[i] / vertexCount * 100);
+  }
+};
+
+#endif  // OLIVE_CSR_GRAPH_H

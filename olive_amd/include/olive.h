@@ -1,0 +1,244 @@
+// The Olive C++ API on MI355X: Olive<VertexValue, EdgeValue, AccumValue>
+// with edgeFilter / vertexFilter / edgeMap / vertexMap / vertexReduce /
+// readGraph, the interface of the reference's single-GPU engine
+// (oliver.h:46-249, documented as `Olive` in README.md:44-97).
+//
+// Differences from the reference, all deliberate:
+//  * edgeFilter / edgeMap schedule a frontier by out-degree bucket (one lane,
+//    one wave64 or one workgroup per vertex) instead of a fixed GroupSize;
+//    GroupSize is still accepted (and defaults to 0) so callers that pass it
+//    (BFS.cu:127-135) or omit it (SSSP.cu:148) both compile.  UseScan
+//    defaults to true for the same reason (oliver.h:92 has no default).
+//  * vertexFilter hands cond() the vertex's accumulator, as update() gets
+//    it; the reference passes the vertex id there (oliverKernel.h:112,135),
+//    under which SSSP.cu's `v.distance > accum` compares against the id.
+//  * readGraph accepts graphs with zero edges (grd.h:77 asserts), and copies
+//    the loaded int edge values into EdgeValue when EdgeValue is a 4-byte
+//    non-empty type (oliver.h:190-202 leaves them uninitialised).
+//  * Every call is ordered on libolive_hip's stream; host reads (size(),
+//    vertexReduce, printVertices) synchronise that stream instead of a
+//    device-wide synchronise after every call (oliver.h:82,112,140,169).
+#ifndef OLIVE_H
+#define OLIVE_H
+
+#include <algorithm>
+#include <type_traits>
+
+#include "commandLine.h"
+#include "common.h"
+#include "csrGraph.h"
+#include "grd.h"
+#include "logging.h"
+#include "oliveKernel.h"
+#include "timer.h"
+#include "utils.h"
+#include "vertexSubset.h"
+
+template <typename VertexValue, typename EdgeValue, typename AccumValue>
+class Olive {
+ public:
+  /**
+   * Push every out-edge of the queued vertices of `src` (a dense subset):
+   * acc[v] = reduce(acc[v], gather(val[u], outdeg(u), edge)), and set v in
+   * the sparse subset `dst`.  Accumulators start at the default value.
+   */
+  template <typename F, int GroupSize = 0>
+  void edgeFilter(VertexSubset &dst, const VertexSubset &src, F f) {
+    assert(!dst.isDense);
+    assert(src.isDense);
+    dst.clear();
+    accumulators.allTo(defaultAccumValue);
+    expand<F, true>(src.queue, src.qSizeDevice, dst.flags.words, f);
+  }
+
+  /**
+   * For every v of the sparse subset `src` with cond(val[v], acc[v]):
+   * update(val[v], acc[v]) and append v to the dense subset `dst`.
+   */
+  template <typename F, bool UseScan = true>
+  void vertexFilter(VertexSubset &dst, const VertexSubset &src, F f) {
+    assert(!src.isDense);
+    assert(dst.isDense);
+    dst.clear();
+    const uint32_t nw = src.flags.nwords;
+    if (!nw) return;
+    hipLaunchKernelGGL((olive_detail::filter_kernel<VertexValue, AccumValue, F>), dim3(grid(nw)),
+                       dim3(olive_detail::kBlock), 0, stream(), src.flags.words, nw, vertexValues.elemsDevice,
+                       accumulators.elemsDevice, dst.queue, dst.qSizeDevice, f);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  /**
+   * Push every out-edge of the vertices of the sparse subset `src` (no
+   * output subset).  The edge value reaches gather() by reference, so a
+   * gather may write it (SSSP.cu:96-99).
+   */
+  template <typename F>
+  void edgeMap(const VertexSubset &src, F f) {
+    assert(!src.isDense);
+    accumulators.allTo(defaultAccumValue);
+    OLV_CHECK(olv_bitmap_to_queue(src.flags.words, vertexCount, mapQueue, mapCount));
+    expand<F, false>(mapQueue, mapCount, NULL, f);
+  }
+
+  /** f(val[v], acc[v]) for every v of `src` (either representation). */
+  template <typename F>
+  void vertexMap(const VertexSubset &src, F f) {
+    if (src.isDense) {
+      hipLaunchKernelGGL((olive_detail::map_queue_kernel<VertexValue, AccumValue, F>), dim3(grid(vertexCount)),
+                         dim3(olive_detail::kBlock), 0, stream(), src.queue, src.qSizeDevice,
+                         vertexValues.elemsDevice, accumulators.elemsDevice, f);
+    } else {
+      const uint32_t nw = src.flags.nwords;
+      if (!nw) return;
+      hipLaunchKernelGGL((olive_detail::map_words_kernel<VertexValue, AccumValue, F>), dim3(grid(nw)),
+                         dim3(olive_detail::kBlock), 0, stream(), src.flags.words, nw, vertexValues.elemsDevice,
+                         accumulators.elemsDevice, f);
+    }
+    HIP_CHECK(hipGetLastError());
+  }
+
+  /** Host reduction of VertexValue::reduce over all vertices. */
+  AccumValue vertexReduce() {
+    vertexValues.persist();
+    AccumValue r = (AccumValue)0;
+    for (VertexId i = 0; i < vertexCount; i++) vertexValues.elemsHost[i].reduce(r);
+    return r;
+  }
+
+  Olive(AccumValue accum) : defaultAccumValue(accum) { init(); }
+  Olive() : defaultAccumValue(0) { init(); }
+
+  /** Upload the CSR of `graph` and allocate the vertex and edge state. */
+  void readGraph(const CsrGraph<int, int> &graph) {
+    release();
+    vertexCount = graph.vertexCount;
+    edgeCount = graph.edgeCount;
+    OLV_CHECK(olv_graph_create(&g, vertexCount, edgeCount, graph.vertices, graph.edges, NULL, OLV_W_NONE,
+                               OLV_PTR_HOST));
+    const void *w = NULL;
+    OLV_CHECK(olv_graph_device_arrays(g, &rowPtr, &colIdx, &w, NULL, NULL, NULL));
+    vertexValues.reserve(vertexCount);
+    accumulators.reserve(vertexCount);
+    vertexValues.clear();
+    accumulators.clear();
+    const bool perEdge = !std::is_empty<EdgeValue>::value;
+    edgeValues.reserve(perEdge ? edgeCount : 1);
+    edgeValues.clear();
+    if (perEdge && sizeof(EdgeValue) == sizeof(int) && graph.hasEdgeValues && edgeCount) {
+      memcpy(edgeValues.elemsHost, graph.edgeValues, sizeof(int) * (size_t)edgeCount);
+      edgeValues.cache();
+    }
+    void *d = NULL;
+    for (int b = 0; b < 3; b++) {
+      OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * (size_t)(vertexCount ? vertexCount : 1)));
+      buckets[b] = reinterpret_cast<uint32_t *>(d);
+    }
+    OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * 4));
+    bucketCounts = reinterpret_cast<uint32_t *>(d);
+    OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * (size_t)(vertexCount ? vertexCount : 1)));
+    mapQueue = reinterpret_cast<uint32_t *>(d);
+    OLV_CHECK(olv_malloc(&d, sizeof(uint32_t)));
+    mapCount = reinterpret_cast<uint32_t *>(d);
+  }
+
+  inline void printVertices() {
+    vertexValues.persist();
+    vertexValues.print();
+  }
+
+  inline void printEdges() {
+    edgeValues.persist();
+    edgeValues.print();
+  }
+
+  /** Copy every vertex value to `out` (vertexCount elements, host). */
+  inline void copyVertices(VertexValue *out) {
+    vertexValues.persist();
+    memcpy(out, vertexValues.elemsHost, sizeof(VertexValue) * (size_t)vertexCount);
+  }
+
+  inline VertexId getVertexCount() const { return vertexCount; }
+  inline EdgeId getEdgeCount() const { return edgeCount; }
+
+  /** The resident graph, for the C ABI algorithms (olv_bfs, olv_sssp ...). */
+  inline olv_graph *graphHandle() const { return g; }
+
+  ~Olive() { release(); }
+
+ private:
+  Olive(const Olive &) = delete;
+  Olive &operator=(const Olive &) = delete;
+
+  static hipStream_t stream() { return reinterpret_cast<hipStream_t>(olv_get_stream()); }
+
+  unsigned grid(size_t items) const {
+    const size_t want = (items + olive_detail::kBlock - 1) / olive_detail::kBlock;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(want, (size_t)cuCount * 8));
+  }
+
+  void init() {
+    vertexCount = 0;
+    edgeCount = 0;
+    g = NULL;
+    rowPtr = colIdx = NULL;
+    buckets[0] = buckets[1] = buckets[2] = NULL;
+    bucketCounts = mapQueue = mapCount = NULL;
+    OLV_CHECK(olv_device_info(&cuCount, NULL, NULL));
+  }
+
+  void release() {
+    if (g) OLV_CHECK(olv_graph_destroy(g));
+    g = NULL;
+    for (int b = 0; b < 3; b++)
+      if (buckets[b]) olv_free(buckets[b]);
+    if (bucketCounts) olv_free(bucketCounts);
+    if (mapQueue) olv_free(mapQueue);
+    if (mapCount) olv_free(mapCount);
+    buckets[0] = buckets[1] = buckets[2] = NULL;
+    bucketCounts = mapQueue = mapCount = NULL;
+    vertexValues.del();
+    accumulators.del();
+    edgeValues.del();
+  }
+
+  // Bin queue[0..*count) by out-degree and push each bucket with its lane
+  // width.  Grids are sized from the capacity (the count stays on the
+  // device) and every kernel strides over the device-side count.
+  template <typename F, bool MARK>
+  void expand(const uint32_t *queue, const uint32_t *count, uint32_t *dstWords, F f) {
+    if (!vertexCount) return;
+    uint32_t *const qs[3] = {buckets[0], buckets[1], buckets[2]};
+    OLV_CHECK(olv_frontier_bin(g, queue, count, qs, bucketCounts));
+    const unsigned full = (unsigned)cuCount * 8;
+    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 0, MARK>),
+                       dim3(grid(vertexCount)), dim3(olive_detail::kBlock), 0, stream(), buckets[0], bucketCounts + 0,
+                       rowPtr, colIdx, vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice,
+                       dstWords, f);
+    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 1, MARK>), dim3(full),
+                       dim3(olive_detail::kBlock), 0, stream(), buckets[1], bucketCounts + 1, rowPtr, colIdx,
+                       vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice, dstWords, f);
+    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 2, MARK>), dim3(full),
+                       dim3(olive_detail::kBlock), 0, stream(), buckets[2], bucketCounts + 2, rowPtr, colIdx,
+                       vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice, dstWords, f);
+    HIP_CHECK(hipGetLastError());
+  }
+
+  VertexId vertexCount;
+  EdgeId edgeCount;
+  int cuCount;
+  olv_graph *g;
+  const uint32_t *rowPtr;
+  const uint32_t *colIdx;
+  uint32_t *buckets[3];
+  uint32_t *bucketCounts;
+  uint32_t *mapQueue;
+  uint32_t *mapCount;
+
+  GRD<VertexValue> vertexValues;
+  GRD<AccumValue> accumulators;
+  GRD<EdgeValue> edgeValues;
+  AccumValue defaultAccumValue;
+};
+
+#endif  // OLIVE_H

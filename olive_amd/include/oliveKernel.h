@@ -1,0 +1,167 @@
+// Functor-generic kernels of the Olive C++ API, compiled into the
+// application (they call the user's gather/reduce/cond/update/operator()
+// functors).  Everything type-independent -- degree binning of a frontier,
+// bitmap -> queue compaction, fills, device memory -- is in libolive_hip.so.
+//
+// Replaces the reference's K1-K5 (oliverKernel.h:45-221):
+//  * K1/K3 (push expansion): a frontier is first split by out-degree into
+//    three buckets (olv_frontier_bin); each bucket runs with one lane, one
+//    wave64 or one 256-lane workgroup per vertex, grid-stride over a
+//    device-side count (the reference gives every vertex a fixed GroupSize
+//    of lanes, and K3 one thread per vertex however large its list).
+//  * K2 (vertexFilter): one lane per 32-vertex bitmap word applies cond /
+//    update and the kept ids are compacted into the queue with a wave64
+//    inclusive scan, a workgroup prefix over the waves and one global atomic
+//    per workgroup (the reference scans one int flag per vertex and takes
+//    an atomic per vertex or per block-local queue of 1200).
+//  * K4/K5 (vertexMap): over bitmap words or the queue, grid-stride.
+#ifndef OLIVE_KERNEL_H
+#define OLIVE_KERNEL_H
+
+#include <type_traits>
+
+#include "bitmap.h"
+#include "common.h"
+
+namespace olive_detail {
+
+constexpr int kBlock = 256;
+constexpr int kWave = 64;
+
+template <class T>
+__global__ void fill_kernel(T *p, T v, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+__device__ inline uint32_t lane() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+// Edge value handed to gather(): the edge's own value, or one shared dummy
+// for value-less edge types (Dump_Edge).
+template <class E>
+__device__ inline E &edge_ref(E *ev, EdgeId e) {
+  if constexpr (std::is_empty<E>::value)
+    return ev[0];
+  else
+    return ev[e];
+}
+
+// Push over one degree bucket.  MODE 0: one lane per vertex, 1: one wave,
+// 2: one workgroup.  MARK: set the destination's bit in dst_words.
+template <class V, class A, class E, class F, int MODE, bool MARK>
+__global__ void __launch_bounds__(kBlock)
+    expand_kernel(const uint32_t *bucket, const uint32_t *bucket_count, const EdgeId *row_ptr, const VertexId *col,
+                  V *vals, A *accs, E *evals, uint32_t *dst_words, F f) {
+  const uint32_t cnt = *bucket_count;
+  if (MODE == 0) {
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
+      const VertexId u = bucket[i];
+      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
+      const V sv = vals[u];
+      for (EdgeId e = beg; e < end; ++e) {
+        const VertexId v = col[e];
+        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
+        if (MARK) Bitmap::set(dst_words, v);
+      }
+    }
+  } else if (MODE == 1) {
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t i = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
+      const VertexId u = bucket[i];
+      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
+      const V sv = vals[u];
+      for (EdgeId e = beg + lane(); e < end; e += kWave) {
+        const VertexId v = col[e];
+        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
+        if (MARK) Bitmap::set(dst_words, v);
+      }
+    }
+  } else {
+    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
+      const VertexId u = bucket[i];
+      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
+      const V sv = vals[u];
+      for (EdgeId e = beg + threadIdx.x; e < end; e += kBlock) {
+        const VertexId v = col[e];
+        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
+        if (MARK) Bitmap::set(dst_words, v);
+      }
+    }
+  }
+}
+
+// Exclusive prefix of `x` over the workgroup; *total gets the sum.
+__device__ inline uint32_t block_scan(uint32_t x, uint32_t *total) {
+  __shared__ uint32_t s_wave[kBlock / kWave + 1];
+  const uint32_t l = lane(), wid = threadIdx.x / kWave;
+  uint32_t inc = x;
+#pragma unroll
+  for (int o = 1; o < kWave; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, o, kWave);
+    if (l >= (uint32_t)o) inc += y;
+  }
+  if (l == kWave - 1) s_wave[wid] = inc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) {
+      const uint32_t t = s_wave[w];
+      s_wave[w] = run;
+      run += t;
+    }
+    s_wave[kBlock / kWave] = run;
+  }
+  __syncthreads();
+  const uint32_t r = s_wave[wid] + inc - x;
+  *total = s_wave[kBlock / kWave];
+  __syncthreads();
+  return r;
+}
+
+// vertexFilter: for every flagged v with cond(val[v], acc[v]): update and
+// enqueue.  One lane per 32-vertex word.
+template <class V, class A, class F>
+__global__ void __launch_bounds__(kBlock)
+    filter_kernel(const uint32_t *words, uint32_t nwords, V *vals, A *accs, uint32_t *queue, uint32_t *qcount, F f) {
+  __shared__ uint32_t s_base;
+  for (uint32_t w0 = blockIdx.x * kBlock; w0 < nwords; w0 += gridDim.x * kBlock) {
+    const uint32_t w = w0 + threadIdx.x;
+    uint32_t keep = 0;
+    for (uint32_t x = w < nwords ? words[w] : 0u; x; x &= x - 1) {
+      const uint32_t b = __ffs(x) - 1;
+      const VertexId v = w * 32u + b;
+      if (f.cond(vals[v], accs[v])) {
+        f.update(vals[v], accs[v]);
+        keep |= 1u << b;
+      }
+    }
+    uint32_t total;
+    const uint32_t off = block_scan((uint32_t)__popc(keep), &total);
+    if (threadIdx.x == 0) s_base = total ? atomicAdd(qcount, total) : 0u;
+    __syncthreads();
+    uint32_t pos = s_base + off;
+    for (uint32_t x = keep; x; x &= x - 1) queue[pos++] = w * 32u + (__ffs(x) - 1);
+    __syncthreads();
+  }
+}
+
+template <class V, class A, class F>
+__global__ void map_queue_kernel(const uint32_t *queue, const uint32_t *qcount, V *vals, A *accs, F f) {
+  const uint32_t cnt = *qcount;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const VertexId v = queue[i];
+    f(vals[v], accs[v]);
+  }
+}
+
+template <class V, class A, class F>
+__global__ void map_words_kernel(const uint32_t *words, uint32_t nwords, V *vals, A *accs, F f) {
+  for (uint32_t w = blockIdx.x * blockDim.x + threadIdx.x; w < nwords; w += gridDim.x * blockDim.x)
+    for (uint32_t x = words[w]; x; x &= x - 1) {
+      const VertexId v = w * 32u + (__ffs(x) - 1);
+      f(vals[v], accs[v]);
+    }
+}
+
+}  // namespace olive_detail
+
+#endif  // OLIVE_KERNEL_H
