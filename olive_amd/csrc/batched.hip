@@ -29,7 +29,8 @@ namespace olv {
 int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool seeds_host, int32_t *levels,
                     unsigned flags, olv_stats *st);
 int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w);
-int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels);
+int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels,
+                     bool can_bu);
 int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count);
 int bfs_result(BfsWork *w, uint32_t *levels_run, uint32_t *bu_levels);
 bool bfs_done(BfsWork *w);
@@ -86,9 +87,10 @@ struct MbfsTdVisitor {
   __device__ MbfsTdVisitor(const MbfsParams &pp, const uint32_t *f, uint32_t *t) : p(pp), Ff(f), T(t) {}
   __device__ __forceinline__ uint32_t row(uint32_t q) const { return q; }
   __device__ __forceinline__ uint32_t load(uint32_t q) const { return q; }
-  __device__ __forceinline__ void visit(uint32_t e, uint32_t u) {
-    const uint32_t v = p.col[e];
-    if (bm_test(p.D, v)) return;
+  __device__ __forceinline__ uint32_t tgt(uint32_t e, uint32_t) const { return p.col[e]; }
+  __device__ __forceinline__ uint32_t state(uint32_t v) const { return p.D[v >> 5]; }
+  __device__ __forceinline__ void apply(uint32_t v, uint32_t dword, uint32_t u) {
+    if ((dword >> (v & 31)) & 1u) return;
     bool any = false;
 #pragma unroll
     for (int j = 0; j < kMW; ++j) {
@@ -895,7 +897,7 @@ int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k, int32_t *
   const uint32_t depth = bfs_depth_hint(p->bw);
   if (s == OLV_OK)
     s = capture(p, [&]() -> int {
-      OLV_TRY(bfs_enqueue_init(g, p->bw, p->seeds.as<uint32_t>(), k, levels));
+      OLV_TRY(bfs_enqueue_init(g, p->bw, p->seeds.as<uint32_t>(), k, levels, p->can_bu));
       return bfs_enqueue_levels(g, p->bw, levels, p->can_bu, depth);
     });
   if (s != OLV_OK) {

@@ -98,14 +98,17 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
 }
 
 // ------------------------------------------------------------- kernels
-// levels = INF, visited/bitmaps/control/barrier = 0.
+// levels = INF, bitmaps/control/barrier = 0, visited = the vertices without
+// in-edges when bottom-up steps may run (no parent can claim them, so
+// bottom-up steps skip them; bfs_seed_kernel clears the seeds' bits), else 0.
 __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint32_t *__restrict__ visited,
-                                 uint32_t *__restrict__ bms, uint32_t nwp, BfsCtl *__restrict__ ctl,
-                                 GridBar *__restrict__ bar) {
+                                 const uint32_t *__restrict__ no_in, uint32_t *__restrict__ bms, uint32_t nwp,
+                                 BfsCtl *__restrict__ ctl, GridBar *__restrict__ bar) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
+  const size_t nw = ((size_t)n + 31) / 32;
   for (size_t i = tid; i < n; i += stride) levels[i] = kInf;
-  for (size_t i = tid; i < nwp; i += stride) visited[i] = 0u;
+  for (size_t i = tid; i < nwp; i += stride) visited[i] = (no_in && i < nw) ? no_in[i] : 0u;
   for (size_t i = tid; i < (size_t)nwp * 3; i += stride) bms[i] = 0u;
   for (size_t i = tid; i < sizeof(BfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
   for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
@@ -114,6 +117,11 @@ __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint3
 // One workgroup: distinct seeds become level 0 and the level-0 frontier.
 __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, BfsArgs a, unsigned long long m,
                                 BfsCtl *__restrict__ ctl) {
+  for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+    const uint32_t s = seeds[i];
+    atomicAnd(&a.visited[s >> 5], ~(1u << (s & 31)));
+  }
+  __syncthreads();
   auto claim = [&](uint32_t i) -> uint32_t {
     const uint32_t s = seeds[i];
     const uint32_t old = atomicOr(&a.visited[s >> 5], 1u << (s & 31));
@@ -318,9 +326,11 @@ int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w) {
 }
 
 // Stream-ordered, capturable: reset and seeding.
-int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels) {
+int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels,
+                     bool can_bu) {
+  const uint32_t *no_in = can_bu ? g->no_in_bm : nullptr;
   hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 3), 256, 8)), dim3(256),
-                     0, stream(), levels, g->n, w->visited, w->bms, w->nwp, w->ctl, w->bar);
+                     0, stream(), levels, g->n, w->visited, no_in, w->bms, w->nwp, w->ctl, w->bar);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, make_args(g, w, levels),
                      (unsigned long long)g->m, w->ctl);
   OLV_HIP(hipGetLastError());
@@ -375,12 +385,16 @@ uint32_t *bfs_ctl_words(BfsWork *w) { return reinterpret_cast<uint32_t *>(w->ctl
 GridBar *bfs_bar(BfsWork *w) { return w->bar; }
 
 // Enqueue levels in growing chunks until the device reports done.
-// chunk0 == 1 gives one host round trip per level (OLV_HOST_LOOP).
-static int run_pipeline(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t chunk0) {
+// chunk0 == 1 gives one host round trip per level (OLV_HOST_LOOP).  The
+// stop event of `t` follows the last enqueued level, ahead of the control
+// block read-back, so host latency after the last kernel is not timed (the
+// gaps between chunks are).
+static int run_pipeline(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t chunk0, Timer &t) {
   uint32_t chunk = std::max(1u, chunk0);
   uint64_t enq = 0;
   for (;;) {
     OLV_TRY(bfs_enqueue_levels(g, w, levels, can_bu, chunk));
+    t.stop();
     enq += chunk;
     OLV_TRY(bfs_result(w, nullptr, nullptr));
     if (w->h_ctl->done) break;
@@ -428,14 +442,14 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
   }
   Timer t;
   t.start();
-  OLV_TRY(bfs_enqueue_init(g, w, d_seeds.as<uint32_t>(), k, out));
+  OLV_TRY(bfs_enqueue_init(g, w, d_seeds.as<uint32_t>(), k, out, can_bu));
   if (flags & OLV_PERSISTENT) {
     OLV_TRY(enqueue_persistent(g, w, out, can_bu));
+    t.stop();
   } else {
     const uint32_t chunk0 = (flags & OLV_HOST_LOOP) ? 1u : std::max(4u, w->last_levels + 1);
-    OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0));
+    OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0, t));
   }
-  t.stop();
   const double ms = t.ms();
   uint32_t levels_run = 0, bu_levels = 0;
   OLV_TRY(bfs_result(w, &levels_run, &bu_levels));

@@ -43,6 +43,36 @@ struct BfsArgs {
 // Hub items needed for the worst case of m edges.
 inline size_t hub_capacity(uint64_t m) { return 2 * (m / kHubChunk) + 64; }
 
+// Visit edges e, e + stride, ... < end with U edges in flight per lane: the
+// U target loads are issued together, then the U state loads, then the
+// updates -- a lane's chain of dependent memory latencies is cut U-fold.
+// Visitor: `Tgt tgt(e, payload)` (reads col[e] and other per-edge data),
+// `S state(const Tgt &)` (reads the target's state), `void apply(const Tgt &,
+// S, payload)` (claims / updates the target).
+template <int U, class Visitor, class P>
+__device__ __forceinline__ void visit_range(Visitor &vis, uint32_t e, uint32_t end, uint32_t stride, const P &pl) {
+  using T = decltype(vis.tgt(0u, pl));
+  using S = decltype(vis.state(vis.tgt(0u, pl)));
+  for (uint64_t b = e; b < end; b += (uint64_t)U * stride) {
+    T t[U];
+    S s[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (b + (uint64_t)k * stride < end) t[k] = vis.tgt((uint32_t)(b + (uint64_t)k * stride), pl);
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (b + (uint64_t)k * stride < end) s[k] = vis.state(t[k]);
+#pragma unroll
+    for (int k = 0; k < U; ++k)
+      if (b + (uint64_t)k * stride < end) vis.apply(t[k], s[k], pl);
+  }
+}
+
+#ifndef OLV_VISIT_UNROLL
+#define OLV_VISIT_UNROLL 4
+#endif
+constexpr int kVisitUnroll = OLV_VISIT_UNROLL;
+
 // Workload-balanced push expansion of a frontier (queue[0, qsize) plus hub
 // items hubs[0, nhubs)).  Each queued out-list is read by the whole
 // workgroup (degree >= block_thr), by one wave (>= wave_thr), or -- for the
@@ -52,7 +82,7 @@ inline size_t hub_capacity(uint64_t m) { return 2 * (m / kHubChunk) + 64; }
 //
 // Visitor: `uint32_t row(uint32_t q)` maps a frontier entry to its CSR row,
 // `P load(uint32_t q)` gives the per-source payload read once per frontier
-// entry, `void visit(uint32_t e, P payload)` handles edge e.
+// entry; edges are handled through visit_range (tgt / state / apply).
 template <int BLOCK, class Visitor>
 __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *hubs, uint32_t nhubs,
                             const uint32_t *row_ptr, uint32_t wave_thr, uint32_t block_thr, uint32_t vb, uint32_t nvb,
@@ -61,8 +91,9 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
   __shared__ uint32_t s_owner, s_beg, s_end;
   __shared__ P s_own_payload;
   __shared__ uint32_t s_scan[BLOCK / kWave + 1];
-  __shared__ uint32_t s_edges[BLOCK];
-  __shared__ P s_payload[BLOCK];
+  constexpr int U = kVisitUnroll;
+  __shared__ uint32_t s_edges[BLOCK * U];
+  __shared__ P s_payload[BLOCK * U];
   const uint32_t lane = lane_id();
   // hub pieces: one workgroup per item
   for (uint32_t h = vb; h < nhubs; h += nvb) {
@@ -73,7 +104,7 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
     const uint32_t end = e1 < beg + kHubChunk ? e1 : beg + kHubChunk;
     const P pl = vis.load(it.x);
     if (threadIdx.x == 0) insp += end - beg;
-    for (uint32_t e = beg + threadIdx.x; e < end; e += BLOCK) vis.visit(e, pl);
+    visit_range<U>(vis, beg + threadIdx.x, end, BLOCK, pl);
   }
   for (uint32_t base = vb * BLOCK; base < qsize; base += nvb * BLOCK) {
     const uint32_t i = base + threadIdx.x;
@@ -104,7 +135,7 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
       __syncthreads();
       const uint32_t eb = s_beg, ee = s_end;
       const P opl = s_own_payload;
-      for (uint32_t e = eb + threadIdx.x; e < ee; e += BLOCK) vis.visit(e, opl);
+      visit_range<U>(vis, eb + threadIdx.x, ee, BLOCK, opl);
       __syncthreads();
     }
     // wave per vertex
@@ -115,21 +146,34 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
       const uint32_t eb = __shfl(beg, leader, kWave), ee = __shfl(end, leader, kWave);
       const P lpl = __shfl(pl, leader, kWave);
       if (lane == leader) beg = end;
-      for (uint32_t e = eb + lane; e < ee; e += kWave) vis.visit(e, lpl);
+      visit_range<U>(vis, eb + lane, ee, kWave, lpl);
     }
     // short lists: scan-based edge dealing
     const uint32_t rem = end - beg;
     uint32_t total;
     const uint32_t off = block_exclusive_scan<BLOCK>(rem, s_scan, &total);
-    for (uint32_t done = 0; done < total; done += BLOCK) {
+    for (uint32_t done = 0; done < total; done += BLOCK * U) {
       const uint32_t lo = off > done ? off : done;
-      const uint32_t hi = (off + rem) < (done + BLOCK) ? (off + rem) : (done + BLOCK);
+      const uint32_t hi = (off + rem) < (done + BLOCK * U) ? (off + rem) : (done + BLOCK * U);
       for (uint32_t k = lo; k < hi; ++k) {
         s_edges[k - done] = beg + (k - off);
         s_payload[k - done] = pl;
       }
       __syncthreads();
-      if (done + threadIdx.x < total) vis.visit(s_edges[threadIdx.x], s_payload[threadIdx.x]);
+      const uint32_t cnt = total - done < BLOCK * U ? total - done : BLOCK * U;
+      using T = decltype(vis.tgt(0u, pl));
+      using S = decltype(vis.state(vis.tgt(0u, pl)));
+      T t[U];
+      S sv[U];
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (threadIdx.x + k * BLOCK < cnt) t[k] = vis.tgt(s_edges[threadIdx.x + k * BLOCK], s_payload[threadIdx.x + k * BLOCK]);
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (threadIdx.x + k * BLOCK < cnt) sv[k] = vis.state(t[k]);
+#pragma unroll
+      for (int k = 0; k < U; ++k)
+        if (threadIdx.x + k * BLOCK < cnt) vis.apply(t[k], sv[k], s_payload[threadIdx.x + k * BLOCK]);
       __syncthreads();
     }
   }
@@ -146,10 +190,11 @@ struct BfsVisitor {
   __device__ BfsVisitor(const BfsArgs &aa, int32_t l, uint32_t *nb) : a(aa), nlevel(l), next_bm(nb) {}
   __device__ __forceinline__ uint32_t row(uint32_t q) const { return q; }
   __device__ __forceinline__ uint32_t load(uint32_t) const { return 0u; }
-  __device__ __forceinline__ void visit(uint32_t e, uint32_t) {
-    const uint32_t v = a.col[e];
+  __device__ __forceinline__ uint32_t tgt(uint32_t e, uint32_t) const { return a.col[e]; }
+  __device__ __forceinline__ uint32_t state(uint32_t v) const { return a.visited[v >> 5]; }
+  __device__ __forceinline__ void apply(uint32_t v, uint32_t word, uint32_t) {
     const uint32_t w = v >> 5, bit = 1u << (v & 31);
-    if (a.visited[w] & bit) return;
+    if (word & bit) return;
     const uint32_t old = atomicOr(&a.visited[w], bit);
     if (old & bit) return;
     a.levels[v] = nlevel;
@@ -188,18 +233,24 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
     if (!ballot(todo)) continue;
     bool found = false;
     if (todo) {
-      uint32_t e = a.in_row_ptr[v];
+      // kVisitUnroll in-edges per step: their loads and frontier tests are
+      // issued together; the scan stops after the step with the first hit.
+      constexpr int U = kVisitUnroll;
+      const uint32_t eb = a.in_row_ptr[v];
       const uint32_t ee = a.in_row_ptr[v + 1];
-      uint32_t k = 0;
-      for (; e < ee; ++e) {
-        ++k;
-        const uint32_t u = a.in_col[e];
-        if (bm_test(front_bm, u)) {
-          found = true;
-          break;
+      for (uint32_t e = eb; e < ee && !found; e += U) {
+        uint32_t u[U];
+#pragma unroll
+        for (int k = 0; k < U; ++k) u[k] = e + k < ee ? a.in_col[e + k] : 0xffffffffu;
+        bool hit = false;
+#pragma unroll
+        for (int k = 0; k < U; ++k) {
+          const bool ok = u[k] != 0xffffffffu;
+          hit |= ok & bm_test(front_bm, ok ? u[k] : 0u);  // word 0 stands in for a missing edge
         }
+        found = hit;
+        insp += ee - e < (uint32_t)U ? ee - e : (uint32_t)U;
       }
-      insp += k;
     }
     const uint64_t fm = ballot(found);
     if (found) {

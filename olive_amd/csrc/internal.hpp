@@ -56,6 +56,7 @@ struct olv_graph {
   void *weights = nullptr;
   int wdtype = OLV_W_NONE;
   uint32_t *in_row_ptr = nullptr, *in_col = nullptr, *in_eid = nullptr;
+  uint32_t *no_in_bm = nullptr;    // bitmap: in-degree 0 (built with the transpose)
   uint32_t max_outdeg = 0, max_indeg = 0;
   float *inv_outdeg = nullptr;     // 1/outdeg (0 for sinks), fp32
   olv::Tuning tune;

