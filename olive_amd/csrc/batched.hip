@@ -50,7 +50,7 @@ struct MbfsCtl {
   uint32_t qfill[2], hfill[2];  // frontier queue / hub items of level L, slot L & 1
   uint32_t pad[2];
   MbfsState st[2];
-  BfsCtr cnt[3];
+  CtrSet cnt[3];
 };
 
 struct MbfsParams {
@@ -150,13 +150,13 @@ __global__ void __launch_bounds__(kMB) mbfs_expand_kernel(MbfsParams p) {
     Ac[i] = 0u;
     Tx[i] = 0u;
   }
-  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+  if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
       ctl->level_e = L;
       ctl->qfill[(L + 1) & 1] = 0;
       ctl->hfill[(L + 1) & 1] = 0;
     }
-    reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
+    ctr_clear(&ctl->cnt[(L + 1) % 3]);
   }
   uint32_t nf = 0;
   unsigned long long mf = 0, insp = 0;
@@ -394,7 +394,7 @@ __global__ void __launch_bounds__(kMB) mbfs_compact_kernel(MbfsParams p) {
   MbfsCtl *ctl = p.ctl;
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t L = ctl->level_e;
-  const BfsCtr c = ctl->cnt[L % 3];
+  const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
   const MbfsState st = ctl->st[L & 1];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const uint32_t nbu = st.nbu + st.mode;

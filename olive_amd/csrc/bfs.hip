@@ -49,7 +49,7 @@ struct BfsCtl {
   uint32_t pad[2];
   unsigned long long inspected;  // result: edges read
   BfsState st[2];
-  BfsCtr cnt[3];       // level counters, slot L % 3
+  CtrSet cnt[3];       // level counters, slot L % 3
 };
 
 struct BfsWork {
@@ -156,13 +156,13 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *c
   const uint32_t *front = bms + (size_t)((L + 2) % 3) * nwp;
   uint32_t *clearb = bms + (size_t)((L + 1) % 3) * nwp;
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nwp; i += gridDim.x * kBlock) clearb[i] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+  if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
       ctl->level_e = L;
       ctl->qfill[(L + 1) & 1] = 0;
       ctl->hfill[(L + 1) & 1] = 0;
     }
-    reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
+    ctr_clear(&ctl->cnt[(L + 1) % 3]);
   }
   uint32_t nf = 0;
   unsigned long long mf = 0, insp = 0;
@@ -180,7 +180,7 @@ __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *
                                                              uint32_t nwp, unsigned long long m, BfsDir dir) {
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t L = ctl->level_e;
-  const BfsCtr c = ctl->cnt[L % 3];
+  const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
   const BfsState st = ctl->st[L & 1];
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   const uint32_t nbu = st.nbu + st.mode;
@@ -227,12 +227,12 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
     const uint32_t qsize = __hip_atomic_load(&ctl->qfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t nhubs = __hip_atomic_load(&ctl->hfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t i = vb * kBlock + threadIdx.x; i < nwp; i += nvb * kBlock) clearb[i] = 0u;
-    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+    if (vb == 0) {
       if (threadIdx.x == 0) {
         ctl->qfill[(L + 1) & 1] = 0;
         ctl->hfill[(L + 1) & 1] = 0;
       }
-      reinterpret_cast<uint32_t *>(&ctl->cnt[(L + 1) % 3])[threadIdx.x] = 0u;
+      ctr_clear(&ctl->cnt[(L + 1) % 3]);
     }
     uint32_t nf = 0;
     unsigned long long mf = 0, insp = 0;
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
       bu_step<kBlock>(a, front, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
     flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
     if (!grid_barrier(bar, gb, kSpinLimit)) break;
-    const BfsCtr c = ctl->cnt[L % 3];
+    const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
     insp_all += c.inspected;
     const uint32_t nbu = st.nbu + st.mode;
     if (c.nf == 0 || L + 1 > a.n) {

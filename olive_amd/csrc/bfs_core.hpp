@@ -18,12 +18,57 @@ namespace olv {
 
 constexpr uint32_t kHubChunk = 4096;
 
-struct BfsCtr {
+// Level counters.  Every workgroup adds its totals with device-scope
+// atomics, which on MI355X are performed at the memory side and serialise
+// per address (≈12 ns each, measured: 4K same-address atomics cost a
+// near-empty level ~48 us).  So a counter set is kCtrSlots copies, one per
+// 128-byte line, and workgroup b adds to copy b % kCtrSlots.
+constexpr int kCtrSlots = 32;
+
+struct alignas(128) BfsCtr {
   unsigned long long mf;         // sum of out-degrees of vertices claimed
   unsigned long long inspected;  // edges read
   uint32_t nf;                   // vertices claimed
-  uint32_t pad[3];
+  uint32_t pad[27];
 };
+
+struct CtrSet {
+  BfsCtr s[kCtrSlots];
+};
+
+struct CtrSum {
+  unsigned long long mf, inspected;
+  uint32_t nf;
+};
+
+// Zero a counter set (all threads of one workgroup).
+__device__ __forceinline__ void ctr_clear(CtrSet *c) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(c);
+  for (uint32_t i = threadIdx.x; i < sizeof(CtrSet) / 4; i += blockDim.x) w[i] = 0u;
+}
+
+// Totals of a counter set, for every thread of the calling workgroup (the
+// first wave reads one slot per lane and reduces; all threads must call).
+__device__ __forceinline__ CtrSum ctr_total(const CtrSet *c) {
+  __shared__ CtrSum s_tot;
+  if (threadIdx.x < kWave) {
+    unsigned long long mf = 0, in = 0, nf = 0;
+    if (threadIdx.x < kCtrSlots) {
+      const BfsCtr &x = c->s[threadIdx.x];
+      mf = x.mf;
+      in = x.inspected;
+      nf = x.nf;
+    }
+    mf = wave_sum(mf);
+    in = wave_sum(in);
+    nf = wave_sum(nf);
+    if (threadIdx.x == 0) s_tot = CtrSum{mf, in, (uint32_t)nf};
+  }
+  __syncthreads();
+  const CtrSum r = s_tot;
+  __syncthreads();
+  return r;
+}
 
 struct BfsArgs {
   const uint32_t *row_ptr;
@@ -106,11 +151,15 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
     if (threadIdx.x == 0) insp += end - beg;
     visit_range<U>(vis, beg + threadIdx.x, end, BLOCK, pl);
   }
-  for (uint32_t base = vb * BLOCK; base < qsize; base += nvb * BLOCK) {
+  // Queue entries per workgroup and pass: a full workgroup's worth when the
+  // queue fills the grid, else an even share, so that a short queue of
+  // high-degree vertices is not left to the first workgroup.
+  const uint32_t per = qsize >= nvb * BLOCK ? BLOCK : (qsize + nvb - 1) / nvb;
+  for (uint32_t base = vb * per; base < qsize; base += nvb * per) {
     const uint32_t i = base + threadIdx.x;
     uint32_t beg = 0, end = 0;
     P pl{};
-    if (i < qsize) {
+    if (threadIdx.x < per && i < qsize) {
       const uint32_t q = queue[i];
       const uint32_t u = vis.row(q);
       beg = row_ptr[u];
@@ -215,53 +264,121 @@ __device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_
 
 // Bottom-up step: every unvisited vertex scans its in-list for a parent in
 // the current frontier bitmap and stops at the first one.  A wave owns 64
-// consecutive vertices (two bitmap words), so visited/next words are written
-// without atomics.
+// consecutive vertices (two bitmap words) per slot, so visited/next words are
+// written without atomics.  Each lane carries kBuBatch vertices (grid-stride
+// slots) at once: their state, row and in-list loads are issued together, so
+// the chain of dependent memory latencies is paid once per kBuBatch slots.
+// A lane scans at most kBuLaneSteps batches of kVisitUnroll in-edges per
+// vertex on its own; in-lists left unfinished are then scanned by the whole
+// wave, 64 x kVisitUnroll edges per step, one vertex at a time.
+#ifndef OLV_BU_LANE_STEPS
+#define OLV_BU_LANE_STEPS 2
+#endif
+#ifndef OLV_BU_BATCH
+#define OLV_BU_BATCH 4
+#endif
+constexpr int kBuLaneSteps = OLV_BU_LANE_STEPS;
+constexpr int kBuBatch = OLV_BU_BATCH;
+
 template <int BLOCK>
 __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlevel, uint32_t *next_bm,
                         uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf,
                         unsigned long long &insp) {
+  constexpr int U = kVisitUnroll;
+  constexpr int J = kBuBatch;
   const uint32_t lane = lane_id();
-  const uint32_t waves = nvb * (BLOCK / kWave);
+  const uint64_t stride = (uint64_t)nvb * (BLOCK / kWave) * kWave;
   const uint32_t gw = vb * (BLOCK / kWave) + threadIdx.x / kWave;
-  for (uint64_t base = (uint64_t)gw * kWave; base < a.n; base += (uint64_t)waves * kWave) {
-    const uint32_t v = (uint32_t)base + lane;
-    const bool inb = v < a.n;
-    const uint32_t w = v >> 5;
-    const uint32_t vis = inb ? a.visited[w] : 0xffffffffu;
-    const bool todo = inb && !((vis >> (v & 31)) & 1u);
-    if (!ballot(todo)) continue;
-    bool found = false;
-    if (todo) {
-      // kVisitUnroll in-edges per step: their loads and frontier tests are
-      // issued together; the scan stops after the step with the first hit.
-      constexpr int U = kVisitUnroll;
-      const uint32_t eb = a.in_row_ptr[v];
-      const uint32_t ee = a.in_row_ptr[v + 1];
-      for (uint32_t e = eb; e < ee && !found; e += U) {
-        uint32_t u[U];
+  for (uint64_t base = (uint64_t)gw * kWave; base < a.n; base += J * stride) {
+    uint32_t vis[J], e[J], ee[J];
+    bool todo[J], found[J];
+    bool any_todo = false;
 #pragma unroll
-        for (int k = 0; k < U; ++k) u[k] = e + k < ee ? a.in_col[e + k] : 0xffffffffu;
+    for (int j = 0; j < J; ++j) {
+      const uint64_t v = base + j * stride + lane;
+      const bool inb = v < a.n;
+      vis[j] = inb ? a.visited[v >> 5] : 0xffffffffu;
+      todo[j] = inb && !((vis[j] >> (v & 31)) & 1u);
+      any_todo |= todo[j];
+      found[j] = false;
+      e[j] = ee[j] = 0;
+    }
+    if (!ballot(any_todo)) continue;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (todo[j]) {
+        const uint32_t v = (uint32_t)(base + j * stride) + lane;
+        e[j] = a.in_row_ptr[v];
+        ee[j] = a.in_row_ptr[v + 1];
+      }
+    // own scans: U in-edges of each of the J vertices per step, all loads
+    // and frontier tests issued together
+    for (int st = 0; st < kBuLaneSteps; ++st) {
+      uint32_t u[J][U];
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const bool act = todo[j] && !found[j] && e[j] < ee[j];
+#pragma unroll
+        for (int k = 0; k < U; ++k) u[j][k] = act && e[j] + k < ee[j] ? a.in_col[e[j] + k] : 0xffffffffu;
+      }
+#pragma unroll
+      for (int j = 0; j < J; ++j) {
+        const bool act = todo[j] && !found[j] && e[j] < ee[j];
         bool hit = false;
 #pragma unroll
         for (int k = 0; k < U; ++k) {
-          const bool ok = u[k] != 0xffffffffu;
-          hit |= ok & bm_test(front_bm, ok ? u[k] : 0u);  // word 0 stands in for a missing edge
+          const bool ok = u[j][k] != 0xffffffffu;
+          hit |= ok & bm_test(front_bm, ok ? u[j][k] : 0u);  // word 0 stands in for a missing edge
         }
-        found = hit;
-        insp += ee - e < (uint32_t)U ? ee - e : (uint32_t)U;
+        if (act) {
+          found[j] = hit;
+          insp += ee[j] - e[j] < (uint32_t)U ? ee[j] - e[j] : (uint32_t)U;
+          e[j] = ee[j] - e[j] < (uint32_t)U ? ee[j] : e[j] + U;
+        }
       }
     }
-    const uint64_t fm = ballot(found);
-    if (found) {
-      a.levels[v] = nlevel;
-      nf += 1;
-      mf += a.row_ptr[v + 1] - a.row_ptr[v];
+    // unfinished in-lists: the whole wave scans them, one vertex at a time
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      uint64_t pend = ballot(todo[j] && !found[j] && e[j] < ee[j]);
+      while (pend) {
+        const uint32_t leader = __ffsll((unsigned long long)pend) - 1;
+        pend &= pend - 1;
+        const uint32_t lb = __shfl(e[j], leader, kWave), le = __shfl(ee[j], leader, kWave);
+        bool any = false;
+        for (uint32_t x = lb; x < le && !any; x += kWave * U) {
+          bool hit = false;
+          uint32_t u[U];
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            const uint32_t xe = x + k * kWave + lane;
+            u[k] = xe < le ? a.in_col[xe] : 0xffffffffu;
+          }
+#pragma unroll
+          for (int k = 0; k < U; ++k) {
+            const bool ok = u[k] != 0xffffffffu;
+            hit |= ok & bm_test(front_bm, ok ? u[k] : 0u);
+          }
+          any = ballot(hit) != 0;
+          if (lane == 0) insp += le - x < (uint32_t)(kWave * U) ? le - x : (uint32_t)(kWave * U);
+        }
+        if (lane == leader) found[j] = any;
+      }
     }
-    const uint32_t mine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
-    if ((lane == 0 || lane == 32) && mine) {
-      a.visited[w] = vis | mine;
-      next_bm[w] = mine;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint32_t v = (uint32_t)(base + j * stride) + lane;
+      const uint64_t fm = ballot(found[j]);
+      if (found[j]) {
+        a.levels[v] = nlevel;
+        nf += 1;
+        mf += a.row_ptr[v + 1] - a.row_ptr[v];
+      }
+      const uint32_t mine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
+      if ((lane == 0 || lane == 32) && mine) {
+        a.visited[v >> 5] = vis[j] | mine;
+        next_bm[v >> 5] = mine;
+      }
     }
   }
 }
@@ -269,16 +386,17 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
 // Bitmap -> frontier.  One lane per word; bits whose row has out-degree >=
 // hub_thr become hub items (vertex, chunk) appended with one atomic per hub,
 // the rest are placed in the queue by a block scan of per-lane counts with
-// one global atomic per block.  row(v) maps a bit to its CSR row.
-template <int BLOCK, class RowFn>
-__device__ void compact_step(const uint32_t *bm, uint32_t nwords, const uint32_t *row_ptr, uint32_t hub_thr,
-                             uint32_t *queue, uint32_t *qcount, uint2 *hubs, uint32_t *hcount, uint32_t vb,
-                             uint32_t nvb, RowFn row) {
+// one global atomic per block.  row(v) maps a bit to its CSR row;
+// word_of(w) yields bitmap word w (called once per word, by its lane).
+template <int BLOCK, class RowFn, class WordFn>
+__device__ void compact_words(uint32_t nwords, WordFn word_of, const uint32_t *row_ptr, uint32_t hub_thr,
+                              uint32_t *queue, uint32_t *qcount, uint2 *hubs, uint32_t *hcount, uint32_t vb,
+                              uint32_t nvb, RowFn row) {
   __shared__ uint32_t s_scan[BLOCK / kWave + 1];
   __shared__ uint32_t s_base;
   for (uint32_t w0 = vb * BLOCK; w0 < nwords; w0 += nvb * BLOCK) {
     const uint32_t w = w0 + threadIdx.x;
-    const uint32_t word = w < nwords ? bm[w] : 0u;
+    const uint32_t word = w < nwords ? word_of(w) : 0u;
     uint32_t small = 0;
     for (uint32_t x = word; x;) {
       const uint32_t b = __ffs(x) - 1;
@@ -310,18 +428,27 @@ __device__ void compact_step(const uint32_t *bm, uint32_t nwords, const uint32_t
   }
 }
 
+template <int BLOCK, class RowFn>
+__device__ __forceinline__ void compact_step(const uint32_t *bm, uint32_t nwords, const uint32_t *row_ptr,
+                                             uint32_t hub_thr, uint32_t *queue, uint32_t *qcount, uint2 *hubs,
+                                             uint32_t *hcount, uint32_t vb, uint32_t nvb, RowFn row) {
+  compact_words<BLOCK>(nwords, [bm](uint32_t w) { return bm[w]; }, row_ptr, hub_thr, queue, qcount, hubs, hcount,
+                       vb, nvb, row);
+}
+
 struct IdRow {
   __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v; }
 };
 
 template <int BLOCK>
-__device__ __forceinline__ void flush_counters(BfsCtr *c, uint32_t nf, unsigned long long mf,
+__device__ __forceinline__ void flush_counters(CtrSet *cs, uint32_t nf, unsigned long long mf,
                                                unsigned long long insp) {
   __shared__ unsigned long long s_red[BLOCK / kWave];
   const unsigned long long tnf = block_sum<BLOCK, unsigned long long>((unsigned long long)nf, s_red);
   const unsigned long long tmf = block_sum<BLOCK, unsigned long long>(mf, s_red);
   const unsigned long long tin = block_sum<BLOCK, unsigned long long>(insp, s_red);
   if (threadIdx.x == 0) {
+    BfsCtr *c = &cs->s[blockIdx.x % kCtrSlots];
     if (tnf) atomicAdd(&c->nf, (uint32_t)tnf);
     if (tmf) atomicAdd(&c->mf, tmf);
     if (tin) atomicAdd(&c->inspected, tin);

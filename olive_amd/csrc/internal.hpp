@@ -57,6 +57,9 @@ struct olv_graph {
   int wdtype = OLV_W_NONE;
   uint32_t *in_row_ptr = nullptr, *in_col = nullptr, *in_eid = nullptr;
   uint32_t *no_in_bm = nullptr;    // bitmap: in-degree 0 (built with the transpose)
+  bool wstat_ready = false;        // int32 weight statistics (SSSP near-far step)
+  int32_t wmin = 0;
+  double wmean = 0.0;
   uint32_t max_outdeg = 0, max_indeg = 0;
   float *inv_outdeg = nullptr;     // 1/outdeg (0 for sinks), fp32
   olv::Tuning tune;

@@ -10,14 +10,25 @@
 // per-round O(V) reset -- reaches the same fixed point: a vertex whose
 // distance dropped this round is marked in a bitmap and expanded next round.
 //
+// Near-far ordering (Davidson et al., IPDPS'14): with a split distance T, an
+// improved vertex joins the next round's frontier only if its new distance
+// is below T; otherwise its bit goes to a far bitmap.  When the near
+// frontier runs dry, T grows by delta and the far vertices now below T (read
+// from dist) form the next frontier.  Vertices are then mostly expanded once
+// with their final distance instead of once per improvement, and the fixed
+// point -- hence every distance -- is the same.  Negative weights turn the
+// ordering off (plain rounds, so the reference's n-round negative-cycle rule
+// applies unchanged).
+//
 // k sources run as k instances over one shared CSR: instance s owns virtual
 // vertices s*n .. s*n+n-1 (dist row s), so one launch per round relaxes all
 // k frontiers together.  Rounds are a relax kernel and a compact kernel
 // whose control lives on the device (as for BFS); OLV_PERSISTENT runs all
-// rounds in one cooperative kernel instead.
+// rounds (plain, without the near-far split) in one cooperative kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cstring>
 #include <type_traits>
 #include <vector>
@@ -37,10 +48,13 @@ struct SsspVisitor {
   const int32_t *w;
   int32_t *dist;
   uint32_t *next_bm;
+  uint32_t *far_bm;  // nullptr: plain rounds
+  int32_t split;     // near-far split distance T
   uint32_t n;
   uint32_t nf = 0;
-  __device__ SsspVisitor(const uint32_t *c, const int32_t *ww, int32_t *d, uint32_t *nb, uint32_t nn)
-      : col(c), w(ww), dist(d), next_bm(nb), n(nn) {}
+  __device__ SsspVisitor(const uint32_t *c, const int32_t *ww, int32_t *d, uint32_t *nb, uint32_t *fb, int32_t t,
+                         uint32_t nn)
+      : col(c), w(ww), dist(d), next_bm(nb), far_bm(fb), split(t), n(nn) {}
   __device__ __forceinline__ uint32_t row(uint32_t q) const { return Multi ? q % n : q; }
   __device__ __forceinline__ P load(uint32_t q) const {
     if constexpr (Multi)
@@ -69,6 +83,10 @@ struct SsspVisitor {
     const int32_t old = atomicMin(&dist[v], cand);
     if (cand >= old) return;
     const uint32_t bit = 1u << (v & 31);
+    if (far_bm && cand >= split) {
+      if (!(far_bm[v >> 5] & bit)) atomicOr(&far_bm[v >> 5], bit);
+      return;
+    }
     if (!(atomicOr(&next_bm[v >> 5], bit) & bit)) nf += 1;
   }
 };
@@ -83,7 +101,12 @@ struct SsspCtl {
   uint32_t qfill[2];  // frontier queue length of round R, slot R & 1
   uint32_t hfill[2];  // hub items of round R, slot R & 1
   unsigned long long relax;  // result: edges read
-  BfsCtr cnt[3];
+  int32_t split[2];   // near-far split T of round R, slot R & 1
+  uint32_t promo[2];  // round R's frontier came from the far bitmap
+  uint32_t kept[2];   // far bits left by the promotion that built round R
+  uint32_t phases;    // result: near-far phases
+  uint32_t pad2;
+  CtrSet cnt[3];
 };
 
 struct SsspParams {
@@ -96,6 +119,8 @@ struct SsspParams {
   uint32_t *bms;     // 2 x nwp
   SsspCtl *ctl;
   GridBar *bar;
+  uint32_t *far_bm;  // nwp words, or nullptr for plain rounds
+  int32_t delta;     // near-far step (0: plain rounds)
   uint32_t n, ninst, nw, nwp;
   uint32_t wave_thr, block_thr, hub_thr;
   uint32_t max_rounds;
@@ -109,8 +134,9 @@ struct ModRow {
 
 template <bool Multi>
 __device__ __forceinline__ void sssp_round(const SsspParams &p, uint32_t qsize, uint32_t nhubs, uint32_t *nextb,
-                                           uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &insp) {
-  SsspVisitor<Multi> vis(p.col, p.w, p.dist, nextb, p.n);
+                                           int32_t split, uint32_t vb, uint32_t nvb, uint32_t &nf,
+                                           unsigned long long &insp) {
+  SsspVisitor<Multi> vis(p.col, p.w, p.dist, nextb, p.far_bm, split, p.n);
   push_expand<kSBlock>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, vb, nvb, vis, insp);
   nf += vis.nf;
 }
@@ -124,17 +150,18 @@ __global__ void __launch_bounds__(kSBlock) sssp_relax_kernel(SsspParams p) {
   uint32_t *nextb = p.bms + (size_t)(R & 1) * p.nwp;
   uint32_t *clearb = p.bms + (size_t)((R + 1) & 1) * p.nwp;
   for (uint32_t i = blockIdx.x * kSBlock + threadIdx.x; i < p.nwp; i += gridDim.x * kSBlock) clearb[i] = 0u;
-  if (blockIdx.x == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+  if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
       ctl->round_e = R;
       ctl->qfill[(R + 1) & 1] = 0;
       ctl->hfill[(R + 1) & 1] = 0;
+      ctl->kept[(R + 1) & 1] = 0;
     }
-    reinterpret_cast<uint32_t *>(&ctl->cnt[(R + 1) % 3])[threadIdx.x] = 0u;
+    ctr_clear(&ctl->cnt[(R + 1) % 3]);
   }
   uint32_t nf = 0;
   unsigned long long insp = 0;
-  sssp_round<Multi>(p, qsize, nhubs, nextb, blockIdx.x, gridDim.x, nf, insp);
+  sssp_round<Multi>(p, qsize, nhubs, nextb, ctl->split[R & 1], blockIdx.x, gridDim.x, nf, insp);
   flush_counters<kSBlock>(&ctl->cnt[R % 3], nf, 0ull, insp);
 }
 
@@ -142,20 +169,62 @@ __global__ void __launch_bounds__(kSBlock) sssp_compact_kernel(SsspParams p) {
   SsspCtl *ctl = p.ctl;
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t R = ctl->round_e;
-  const BfsCtr c = ctl->cnt[R % 3];
+  const CtrSum c = ctr_total(&ctl->cnt[R % 3]);
   const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
   if (lead) ctl->relax += c.inspected;
-  if (c.nf == 0 || R + 1 >= p.max_rounds) {
+  const int32_t split = ctl->split[R & 1];
+  // near frontier dry: finished in plain mode, or when the far bitmap was
+  // found empty by the promotion that built this round's (empty) frontier
+  const bool dry = c.nf == 0;
+  const bool finished =
+      dry && (p.delta == 0 || (ctl->promo[R & 1] && ctl->qfill[R & 1] == 0 && ctl->hfill[R & 1] == 0 &&
+                               ctl->kept[R & 1] == 0));
+  if (finished || R + 1 >= p.max_rounds) {
     if (lead) {
       ctl->rounds = R + 1;
-      ctl->capped = c.nf != 0;
+      ctl->capped = !finished;
       ctl->done = 1u;
     }
     return;
   }
-  if (lead) ctl->round = R + 1;
-  compact_step<kSBlock>(p.bms + (size_t)(R & 1) * p.nwp, p.nw, p.row_ptr, p.hub_thr, p.queue, &ctl->qfill[(R + 1) & 1],
-                        p.hubs, &ctl->hfill[(R + 1) & 1], blockIdx.x, gridDim.x, ModRow{p.n});
+  if (!dry) {
+    if (lead) {
+      ctl->split[(R + 1) & 1] = split;
+      ctl->promo[(R + 1) & 1] = 0;
+      ctl->round = R + 1;
+    }
+    compact_step<kSBlock>(p.bms + (size_t)(R & 1) * p.nwp, p.nw, p.row_ptr, p.hub_thr, p.queue,
+                          &ctl->qfill[(R + 1) & 1], p.hubs, &ctl->hfill[(R + 1) & 1], blockIdx.x, gridDim.x,
+                          ModRow{p.n});
+    return;
+  }
+  // promotion: T += delta; far vertices now below T become the frontier
+  const int32_t nsplit = split > kInf - p.delta ? kInf : split + p.delta;
+  if (lead) {
+    ctl->split[(R + 1) & 1] = nsplit;
+    ctl->promo[(R + 1) & 1] = 1;
+    ctl->phases += 1;
+    ctl->round = R + 1;
+  }
+  uint32_t kept = 0;
+  const int32_t *dist = p.dist;
+  uint32_t *far = p.far_bm;
+  auto take = [&](uint32_t w) -> uint32_t {
+    const uint32_t word = far[w];
+    uint32_t sel = 0;
+    for (uint32_t x = word; x; x &= x - 1) {
+      const uint32_t b = __ffs(x) - 1;
+      if (dist[w * 32u + b] < nsplit) sel |= 1u << b;
+    }
+    if (sel) far[w] = word & ~sel;
+    kept += __popc(word & ~sel);
+    return sel;
+  };
+  compact_words<kSBlock>(p.nw, take, p.row_ptr, p.hub_thr, p.queue, &ctl->qfill[(R + 1) & 1], p.hubs,
+                         &ctl->hfill[(R + 1) & 1], blockIdx.x, gridDim.x, ModRow{p.n});
+  __shared__ unsigned long long s_red[kSBlock / kWave];
+  const unsigned long long tk = block_sum<kSBlock, unsigned long long>(kept, s_red);
+  if (threadIdx.x == 0 && tk) atomicAdd(&ctl->kept[(R + 1) & 1], (uint32_t)tk);
 }
 
 template <bool Multi>
@@ -172,19 +241,19 @@ __global__ void __launch_bounds__(kSBlock) sssp_persistent_kernel(SsspParams p) 
     const uint32_t qsize = __hip_atomic_load(&ctl->qfill[R & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t nhubs = __hip_atomic_load(&ctl->hfill[R & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     for (uint32_t i = vb * kSBlock + threadIdx.x; i < p.nwp; i += nvb * kSBlock) clearb[i] = 0u;
-    if (vb == 0 && threadIdx.x < sizeof(BfsCtr) / 4) {
+    if (vb == 0) {
       if (threadIdx.x == 0) {
         ctl->qfill[(R + 1) & 1] = 0;
         ctl->hfill[(R + 1) & 1] = 0;
       }
-      reinterpret_cast<uint32_t *>(&ctl->cnt[(R + 1) % 3])[threadIdx.x] = 0u;
+      ctr_clear(&ctl->cnt[(R + 1) % 3]);
     }
     uint32_t nf = 0;
     unsigned long long insp = 0;
-    sssp_round<Multi>(p, qsize, nhubs, nextb, vb, nvb, nf, insp);
+    sssp_round<Multi>(p, qsize, nhubs, nextb, kInf, vb, nvb, nf, insp);
     flush_counters<kSBlock>(&ctl->cnt[R % 3], nf, 0ull, insp);
     if (!grid_barrier(p.bar, gb, kSSpin)) break;
-    const BfsCtr c = ctl->cnt[R % 3];
+    const CtrSum c = ctr_total(&ctl->cnt[R % 3]);
     insp_all += c.inspected;
     if (c.nf == 0 || R + 1 >= p.max_rounds) {
       if (lead) {
@@ -201,15 +270,16 @@ __global__ void __launch_bounds__(kSBlock) sssp_persistent_kernel(SsspParams p) 
   }
 }
 
-// dist = INF, bitmaps/control/barrier = 0.
+// dist = INF, bitmaps (near x2, far)/control/barrier = 0, split = delta.
 __global__ void sssp_reset_kernel(int32_t *__restrict__ dist, size_t total, uint32_t *__restrict__ bms,
-                                  uint32_t nwp, SsspCtl *__restrict__ ctl, GridBar *__restrict__ bar) {
+                                  uint32_t nwp, SsspCtl *__restrict__ ctl, GridBar *__restrict__ bar, int32_t delta) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (size_t i = tid; i < total; i += stride) dist[i] = kInf;
-  for (size_t i = tid; i < (size_t)nwp * 2; i += stride) bms[i] = 0u;
+  for (size_t i = tid; i < (size_t)nwp * 3; i += stride) bms[i] = 0u;
   for (size_t i = tid; i < sizeof(SsspCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
   for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
+  if (tid == 0) ctl->split[0] = delta > 0 ? delta : kInf;
 }
 
 // One workgroup: instance s starts from virtual vertex s*n + srcs[s].
@@ -241,7 +311,7 @@ struct SsspWork {
     nwp = (nw + 63) / 64 * 64;
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&queue), (size_t)nv * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&hubs), hub_capacity(m) * k * sizeof(uint2)));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bms), (size_t)nwp * 8));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bms), (size_t)nwp * 12));  // near x2, far
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&ctl), sizeof(SsspCtl)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bar), sizeof(GridBar)));
     OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&h_ctl), sizeof(SsspCtl), hipHostMallocDefault));
@@ -252,6 +322,74 @@ struct SsspWork {
 static unsigned sgrid(size_t items, size_t per) {
   size_t b = (items + per - 1) / per;
   return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * 8));
+}
+
+// Minimum and mean of the int32 weights (one reduction, read back).
+__global__ void wstat_kernel(const int32_t *__restrict__ w, uint32_t m, long long *__restrict__ out) {
+  __shared__ long long s_red[256 / kWave];
+  long long mn = LLONG_MAX, sum = 0;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < m; e += gridDim.x * blockDim.x) {
+    const long long x = w[e];
+    mn = x < mn ? x : mn;
+    sum += x;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    const long long y = __shfl_xor(mn, o, kWave);
+    mn = y < mn ? y : mn;
+  }
+  if (lane_id() == 0) s_red[threadIdx.x / kWave] = mn;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int i = 1; i < 256 / kWave; ++i) mn = s_red[i] < mn ? s_red[i] : mn;
+    atomicMin(&out[0], mn);
+  }
+  __syncthreads();
+  sum = block_sum<256, long long>(sum, s_red);
+  if (threadIdx.x == 0) atomicAdd(reinterpret_cast<unsigned long long *>(&out[1]), (unsigned long long)sum);
+}
+
+static int weight_stats(const int32_t *w, uint32_t m, int32_t *wmin, double *wmean) {
+  long long h[2] = {LLONG_MAX, 0};
+  DevBuf b;
+  OLV_TRY(b.alloc(sizeof(h)));
+  OLV_HIP(hipMemcpyAsync(b.p, h, sizeof(h), hipMemcpyHostToDevice, stream()));
+  hipLaunchKernelGGL(wstat_kernel, dim3(sgrid(m, 256)), dim3(256), 0, stream(), w, m, b.as<long long>());
+  OLV_HIP(hipMemcpyAsync(h, b.p, sizeof(h), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  *wmin = m ? (int32_t)h[0] : 0;
+  *wmean = m ? (double)h[1] / m : 0.0;
+  return OLV_OK;
+}
+
+#ifndef OLV_SSSP_DELTA_NUM
+#define OLV_SSSP_DELTA_NUM 32
+#endif
+
+// Near-far step: about OLV_SSSP_DELTA_NUM edges' worth of distance per
+// vertex (mean weight x NUM / mean out-degree); 0 (plain rounds) when a
+// weight is negative.
+static int sssp_delta(const olv_graph *g, const int32_t *w, int32_t *delta) {
+  *delta = 0;
+  if (!g->m || !g->n) return OLV_OK;
+  int32_t wmin = 0;
+  double wmean = 0.0;
+  olv_graph *mg = const_cast<olv_graph *>(g);
+  if (w == g->weights && mg->wstat_ready) {
+    wmin = mg->wmin;
+    wmean = mg->wmean;
+  } else {
+    OLV_TRY(weight_stats(w, g->m, &wmin, &wmean));
+    if (w == g->weights) {
+      mg->wmin = wmin;
+      mg->wmean = wmean;
+      mg->wstat_ready = true;
+    }
+  }
+  if (wmin < 0) return OLV_OK;
+  const double deg = (double)g->m / g->n;
+  const double d = wmean * OLV_SSSP_DELTA_NUM / (deg > 1.0 ? deg : 1.0);
+  *delta = d < 1.0 ? 1 : (d > 1e9 ? 1000000000 : (int32_t)(d + 0.5));
+  return OLV_OK;
 }
 
 static int sssp_read(SsspWork &w) {
@@ -285,12 +423,18 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
   p.wave_thr = g->tune.td_wave_thr;
   p.block_thr = g->tune.td_block_thr;
   p.hub_thr = g->tune.hub_thr;
-  p.max_rounds = g->n;
+  int32_t delta = 0;
+  if (!(flags & OLV_PERSISTENT)) OLV_TRY(sssp_delta(g, weights, &delta));
+  p.delta = delta;
+  p.far_bm = delta ? w.bms + (size_t)2 * w.nwp : nullptr;
+  // plain rounds keep the reference's negative-cycle rule (n rounds); with
+  // non-negative weights the cap only guards against a runaway loop
+  p.max_rounds = delta ? (uint32_t)std::min<uint64_t>(4ull * g->n + 1024, 0xffffffffull) : g->n;
   hipStream_t s = stream();
   Timer t;
   t.start();
-  hipLaunchKernelGGL(sssp_reset_kernel, dim3(sgrid(std::max<size_t>(nv, (size_t)w.nwp * 2), 256)), dim3(256), 0, s,
-                     dist, (size_t)nv, w.bms, w.nwp, w.ctl, w.bar);
+  hipLaunchKernelGGL(sssp_reset_kernel, dim3(sgrid(std::max<size_t>(nv, (size_t)w.nwp * 3), 256)), dim3(256), 0, s,
+                     dist, (size_t)nv, w.bms, w.nwp, w.ctl, w.bar, delta);
   hipLaunchKernelGGL(sssp_seed_kernel, dim3(1), dim3(256), 0, s, d_srcs, k, p);
   OLV_HIP(hipGetLastError());
   if (flags & OLV_PERSISTENT) {

@@ -158,6 +158,34 @@ def test_sssp_rmat_heavy_weights(olv):
     assert np.array_equal(g.sssp(s)[0].numpy(), oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, s))
 
 
+def test_sssp_negative_weights_without_cycle(olv):
+    """Negative weights (an acyclic graph) switch off the near-far split;
+    distances still equal the oracle's frontier relaxation."""
+    csr0 = olv.gen_rmat(12, 8, seed=9)
+    src = np.repeat(np.arange(csr0.n), csr0.outdeg())
+    keep = src < csr0.col  # forward edges only: a DAG
+    rp = np.zeros(csr0.n + 1, np.uint32)
+    np.cumsum(np.bincount(src[keep], minlength=csr0.n), out=rp[1:])
+    col = csr0.col[keep].astype(np.uint32)
+    w = (np.random.default_rng(3).integers(-20, 60, col.size)).astype(np.int32)
+    csr = olv.CsrGraph(rp, col, w)
+    g = olv.Graph(csr)
+    for s in (0, int(np.argmax(csr.outdeg()))):
+        ref, rounds = oracle.sssp(rp, col, w, s)
+        assert rounds != -1
+        assert np.array_equal(g.sssp(s)[0].numpy(), ref)
+
+
+def test_sssp_zero_and_large_weights(olv):
+    """Near-far split with zero weights (step 1) and with weights up to 2^20."""
+    base = olv.gen_uniform(1 << 14, 1 << 17, seed=4)
+    for lo, hi in ((0, 1), (0, 3), (1, 1 << 20)):
+        csr = olv.with_weights_i32(base, lo, hi, seed=lo + hi)
+        g = olv.Graph(csr, transpose=False)
+        ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 0)
+        assert np.array_equal(g.sssp(0)[0].numpy(), ref), (lo, hi)
+
+
 def test_sssp_negative_cycle_reports_enoconv(olv):
     csr = olv.CsrGraph(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.array([-1, -1], np.int32))
     g = olv.Graph(csr)

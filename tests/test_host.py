@@ -171,3 +171,28 @@ def test_loader_rejects_bad_files(tmp_path):
     p.write_text("3 1\n0 7\n")
     with pytest.raises(olive_amd.OliveError):
         olive_amd.load_edge_list(str(p))
+
+
+def test_autograd_wrapper_validates_inputs():
+    """The torch wrapper rejects host tensors, other dtypes and wrong shapes
+    before any library call (no GPU needed)."""
+    torch = pytest.importorskip("torch")
+    from olive_amd import autograd
+
+    with pytest.raises(TypeError):
+        autograd._check_vec(torch.zeros(4), 4, "w")          # host tensor
+    with pytest.raises(TypeError):
+        autograd._check_vec(np.zeros(4, np.float32), 4, "w")  # not a tensor
+    assert autograd.PageRankFunction._fn is None or callable(autograd.PageRankFunction._fn.apply)
+
+
+def test_gpus_flag_other_than_one_exits(tmp_path):
+    """bench.py runs one GPU; any other --gpus value exits with a message."""
+    import json
+    import subprocess
+    import sys
+
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2
+    assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
