@@ -55,7 +55,8 @@ struct BfsCtl {
 struct BfsWork {
   uint32_t n = 0, nw = 0, nwp = 0;  // vertices, bitmap words, padded words
   uint32_t *visited = nullptr;
-  uint32_t *bms = nullptr;          // 3 bitmaps of nwp words
+  uint32_t *snap = nullptr;         // visited as of the last frontier extraction
+  uint32_t *bms = nullptr;          // 2 frontier bitmaps of nwp words
   uint32_t *queue = nullptr;
   uint2 *hubs = nullptr;
   BfsCtl *ctl = nullptr;
@@ -68,7 +69,7 @@ struct BfsWork {
 void free_bfs_work(olv_graph *g) {
   BfsWork *w = g->bfs;
   if (!w) return;
-  void *ps[] = {w->visited, w->bms, w->queue, w->hubs, w->ctl, w->bar, w->stat};
+  void *ps[] = {w->visited, w->snap, w->bms, w->queue, w->hubs, w->ctl, w->bar, w->stat};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
@@ -85,7 +86,8 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
     w->nw = (g->n + 31) / 32;
     w->nwp = (w->nw + 63) / 64 * 64;
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 3));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->snap), (size_t)w->nwp * 4));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 2));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->queue), (size_t)g->n * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->hubs), hub_capacity(g->m) * sizeof(uint2)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(BfsCtl)));
@@ -98,18 +100,20 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
 }
 
 // ------------------------------------------------------------- kernels
-// levels = INF, bitmaps/control/barrier = 0, visited = the vertices without
-// in-edges when bottom-up steps may run (no parent can claim them, so
-// bottom-up steps skip them; bfs_seed_kernel clears the seeds' bits), else 0.
+// levels = INF, bitmaps/control/barrier = 0, visited = snap = the vertices
+// without in-edges when bottom-up steps may run (no parent can claim them,
+// so bottom-up steps skip them; bfs_seed_kernel clears the seeds' bits),
+// else 0.
 __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint32_t *__restrict__ visited,
-                                 const uint32_t *__restrict__ no_in, uint32_t *__restrict__ bms, uint32_t nwp,
-                                 BfsCtl *__restrict__ ctl, GridBar *__restrict__ bar) {
+                                 uint32_t *__restrict__ snap, const uint32_t *__restrict__ no_in,
+                                 uint32_t *__restrict__ bms, uint32_t nwp, BfsCtl *__restrict__ ctl,
+                                 GridBar *__restrict__ bar) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t nw = ((size_t)n + 31) / 32;
   for (size_t i = tid; i < n; i += stride) levels[i] = kInf;
-  for (size_t i = tid; i < nwp; i += stride) visited[i] = (no_in && i < nw) ? no_in[i] : 0u;
-  for (size_t i = tid; i < (size_t)nwp * 3; i += stride) bms[i] = 0u;
+  for (size_t i = tid; i < nwp; i += stride) visited[i] = snap[i] = (no_in && i < nw) ? no_in[i] : 0u;
+  for (size_t i = tid; i < (size_t)nwp * 2; i += stride) bms[i] = 0u;
   for (size_t i = tid; i < sizeof(BfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
   for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
 }
@@ -126,6 +130,7 @@ __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, 
     const uint32_t s = seeds[i];
     const uint32_t old = atomicOr(&a.visited[s >> 5], 1u << (s & 31));
     if (old & (1u << (s & 31))) return 0xffffffffu;
+    atomicOr(&a.snap[s >> 5], 1u << (s & 31));  // level 0 is not a next frontier
     a.levels[s] = 0;
     return s;
   };
@@ -144,18 +149,15 @@ struct BfsDir {
   float alpha, beta;
 };
 
-// Level L = ctl->level: expand into bitmap L%3 (top-down from the level's
-// queue and hub items, or bottom-up against bitmap (L+2)%3), clear bitmap
-// (L+1)%3 and the next level's fill counters.
+// Level L = ctl->level: claim the vertices of level L + 1, top-down from
+// the level's queue and hub items or bottom-up against frontier bitmap
+// L & 1; reset the next level's fill counters.
 __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nwp) {
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t L = ctl->level;
   const BfsState st = ctl->st[L & 1];
   const uint32_t qsize = ctl->qfill[L & 1], nhubs = ctl->hfill[L & 1];
-  uint32_t *nextb = bms + (size_t)(L % 3) * nwp;
-  const uint32_t *front = bms + (size_t)((L + 2) % 3) * nwp;
-  uint32_t *clearb = bms + (size_t)((L + 1) % 3) * nwp;
-  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < nwp; i += gridDim.x * kBlock) clearb[i] = 0u;
+  const uint32_t *front = bms + (size_t)(L & 1) * nwp;
   if (blockIdx.x == 0) {
     if (threadIdx.x == 0) {
       ctl->level_e = L;
@@ -167,15 +169,15 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *c
   uint32_t nf = 0;
   unsigned long long mf = 0, insp = 0;
   if (st.mode == 0)
-    td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), nextb, blockIdx.x, gridDim.x, nf, mf, insp);
+    td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
   else
-    bu_step<kBlock>(a, front, (int32_t)(L + 1), nextb, blockIdx.x, gridDim.x, nf, mf, insp);
+    bu_step<kBlock>(a, front, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
   flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
 }
 
-// Level L = ctl->level_e: decide the next direction; if top-down, turn
-// bitmap L%3 into the next level's queue and hub items.  Workgroup 0
-// publishes the next level's state.
+// Level L = ctl->level_e: decide the next direction and extract level
+// L + 1's frontier (bitmap (L + 1) & 1; also queue and hub items if the
+// next step is top-down).  Workgroup 0 publishes the next level's state.
 __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nw,
                                                              uint32_t nwp, unsigned long long m, BfsDir dir) {
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -204,9 +206,8 @@ __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *
     ctl->st[(L + 1) & 1] = ns;
     ctl->level = L + 1;
   }
-  if (!next_bu)
-    compact_step<kBlock>(bms + (size_t)(L % 3) * nwp, nw, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[(L + 1) & 1],
-                         a.hubs, &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x, IdRow());
+  frontier_words<kBlock>(a, bms + (size_t)((L + 1) & 1) * nwp, nw, !next_bu, &ctl->qfill[(L + 1) & 1],
+                         &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x);
 }
 
 // All levels in one cooperative launch with the XCD-aware grid barrier
@@ -221,12 +222,9 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
   unsigned long long insp_all = 0;
   uint32_t L = 0;
   for (;;) {
-    uint32_t *nextb = bms + (size_t)(L % 3) * nwp;
-    const uint32_t *front = bms + (size_t)((L + 2) % 3) * nwp;
-    uint32_t *clearb = bms + (size_t)((L + 1) % 3) * nwp;
+    const uint32_t *front = bms + (size_t)(L & 1) * nwp;
     const uint32_t qsize = __hip_atomic_load(&ctl->qfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t nhubs = __hip_atomic_load(&ctl->hfill[L & 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    for (uint32_t i = vb * kBlock + threadIdx.x; i < nwp; i += nvb * kBlock) clearb[i] = 0u;
     if (vb == 0) {
       if (threadIdx.x == 0) {
         ctl->qfill[(L + 1) & 1] = 0;
@@ -237,9 +235,9 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
     uint32_t nf = 0;
     unsigned long long mf = 0, insp = 0;
     if (st.mode == 0)
-      td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
+      td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
     else
-      bu_step<kBlock>(a, front, (int32_t)(L + 1), nextb, vb, nvb, nf, mf, insp);
+      bu_step<kBlock>(a, front, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
     flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
     if (!grid_barrier(bar, gb, kSpinLimit)) break;
     const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
@@ -260,11 +258,9 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
     st.mu = mu;
     st.mode = next_bu ? 1u : 0u;
     st.nbu = nbu;
-    if (!next_bu) {
-      compact_step<kBlock>(nextb, nw, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[(L + 1) & 1], a.hubs,
-                           &ctl->hfill[(L + 1) & 1], vb, nvb, IdRow());
-      if (!grid_barrier(bar, gb, kSpinLimit)) break;
-    }
+    frontier_words<kBlock>(a, bms + (size_t)((L + 1) & 1) * nwp, nw, !next_bu, &ctl->qfill[(L + 1) & 1],
+                           &ctl->hfill[(L + 1) & 1], vb, nvb);
+    if (!grid_barrier(bar, gb, kSpinLimit)) break;
     ++L;
   }
 }
@@ -296,6 +292,7 @@ static BfsArgs make_args(const olv_graph *g, BfsWork *w, int32_t *levels) {
   a.in_col = g->in_col;
   a.levels = levels;
   a.visited = w->visited;
+  a.snap = w->snap;
   a.queue = w->queue;
   a.hubs = w->hubs;
   a.n = g->n;
@@ -329,8 +326,8 @@ int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w) {
 int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels,
                      bool can_bu) {
   const uint32_t *no_in = can_bu ? g->no_in_bm : nullptr;
-  hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 3), 256, 8)), dim3(256),
-                     0, stream(), levels, g->n, w->visited, no_in, w->bms, w->nwp, w->ctl, w->bar);
+  hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 2), 256, 8)), dim3(256),
+                     0, stream(), levels, g->n, w->visited, w->snap, no_in, w->bms, w->nwp, w->ctl, w->bar);
   hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, make_args(g, w, levels),
                      (unsigned long long)g->m, w->ctl);
   OLV_HIP(hipGetLastError());

@@ -77,6 +77,7 @@ struct BfsArgs {
   const uint32_t *in_col;
   int32_t *levels;
   uint32_t *visited;
+  uint32_t *snap;      // visited as of the last frontier extraction
   uint32_t *queue;
   uint2 *hubs;
   uint32_t n;
@@ -228,35 +229,36 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
   }
 }
 
-// BFS claim: first visitor of an unvisited vertex sets its level and its bit
-// in the next-frontier bitmap.
+// BFS claim: a visitor that finds the vertex unvisited sets its visited bit
+// (a non-returning atomic: nothing waits for it) and its level.  Two
+// visitors may both see the bit clear; both store the same level, and the
+// level counters then count the vertex twice -- they only steer the
+// direction choice.  The next frontier is not recorded here: it is the set
+// of visited bits that are new since the last extraction (frontier_words).
 struct BfsVisitor {
   const BfsArgs &a;
   int32_t nlevel;
-  uint32_t *next_bm;
   uint32_t nf = 0;
   unsigned long long mf = 0;
-  __device__ BfsVisitor(const BfsArgs &aa, int32_t l, uint32_t *nb) : a(aa), nlevel(l), next_bm(nb) {}
+  __device__ BfsVisitor(const BfsArgs &aa, int32_t l) : a(aa), nlevel(l) {}
   __device__ __forceinline__ uint32_t row(uint32_t q) const { return q; }
   __device__ __forceinline__ uint32_t load(uint32_t) const { return 0u; }
   __device__ __forceinline__ uint32_t tgt(uint32_t e, uint32_t) const { return a.col[e]; }
   __device__ __forceinline__ uint32_t state(uint32_t v) const { return a.visited[v >> 5]; }
   __device__ __forceinline__ void apply(uint32_t v, uint32_t word, uint32_t) {
-    const uint32_t w = v >> 5, bit = 1u << (v & 31);
+    const uint32_t bit = 1u << (v & 31);
     if (word & bit) return;
-    const uint32_t old = atomicOr(&a.visited[w], bit);
-    if (old & bit) return;
+    __hip_atomic_fetch_or(&a.visited[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     a.levels[v] = nlevel;
-    atomicOr(&next_bm[w], bit);
     nf += 1;
     mf += a.row_ptr[v + 1] - a.row_ptr[v];
   }
 };
 
 template <int BLOCK>
-__device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_t nlevel, uint32_t *next_bm,
-                        uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
-  BfsVisitor vis(a, nlevel, next_bm);
+__device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_t nlevel, uint32_t vb, uint32_t nvb,
+                        uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
+  BfsVisitor vis(a, nlevel);
   push_expand<BLOCK>(a.queue, qsize, a.hubs, nhubs, a.row_ptr, a.wave_thr, a.block_thr, vb, nvb, vis, insp);
   nf += vis.nf;
   mf += vis.mf;
@@ -264,7 +266,7 @@ __device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_
 
 // Bottom-up step: every unvisited vertex scans its in-list for a parent in
 // the current frontier bitmap and stops at the first one.  A wave owns 64
-// consecutive vertices (two bitmap words) per slot, so visited/next words are
+// consecutive vertices (two bitmap words) per slot, so visited words are
 // written without atomics.  Each lane carries kBuBatch vertices (grid-stride
 // slots) at once: their state, row and in-list loads are issued together, so
 // the chain of dependent memory latencies is paid once per kBuBatch slots.
@@ -281,9 +283,8 @@ constexpr int kBuLaneSteps = OLV_BU_LANE_STEPS;
 constexpr int kBuBatch = OLV_BU_BATCH;
 
 template <int BLOCK>
-__device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlevel, uint32_t *next_bm,
-                        uint32_t vb, uint32_t nvb, uint32_t &nf, unsigned long long &mf,
-                        unsigned long long &insp) {
+__device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlevel, uint32_t vb, uint32_t nvb,
+                        uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
   constexpr int U = kVisitUnroll;
   constexpr int J = kBuBatch;
   const uint32_t lane = lane_id();
@@ -375,10 +376,7 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
         mf += a.row_ptr[v + 1] - a.row_ptr[v];
       }
       const uint32_t mine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
-      if ((lane == 0 || lane == 32) && mine) {
-        a.visited[v >> 5] = vis[j] | mine;
-        next_bm[v >> 5] = mine;
-      }
+      if ((lane == 0 || lane == 32) && mine) a.visited[v >> 5] = vis[j] | mine;
     }
   }
 }
@@ -439,6 +437,25 @@ __device__ __forceinline__ void compact_step(const uint32_t *bm, uint32_t nwords
 struct IdRow {
   __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v; }
 };
+
+// Next frontier = visited bits set since the last extraction: written to
+// `front` (for a bottom-up step) and, if `to_queue`, compacted into the
+// queue and hub items (for a top-down step); `snap` catches up.
+template <int BLOCK>
+__device__ void frontier_words(const BfsArgs &a, uint32_t *front, uint32_t nw, bool to_queue, uint32_t *qcount,
+                               uint32_t *hcount, uint32_t vb, uint32_t nvb) {
+  auto word_of = [&](uint32_t w) -> uint32_t {
+    const uint32_t v = a.visited[w], s = a.snap[w];
+    const uint32_t nb = v & ~s;
+    if (nb) a.snap[w] = v;
+    front[w] = nb;
+    return nb;
+  };
+  if (to_queue)
+    compact_words<BLOCK>(nw, word_of, a.row_ptr, a.hub_thr, a.queue, qcount, a.hubs, hcount, vb, nvb, IdRow());
+  else
+    for (uint32_t w = vb * BLOCK + threadIdx.x; w < nw; w += nvb * BLOCK) (void)word_of(w);
+}
 
 template <int BLOCK>
 __device__ __forceinline__ void flush_counters(CtrSet *cs, uint32_t nf, unsigned long long mf,
