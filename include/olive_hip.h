@@ -41,7 +41,8 @@ enum {
 /* ---- flags ------------------------------------------------------------- */
 #define OLV_PTR_HOST        0x0001u  /* inputs/outputs are host pointers            */
 #define OLV_NO_DIRECTION_OPT 0x0002u /* BFS: top-down (push) only                    */
-#define OLV_FORCE_BOTTOM_UP  0x0004u /* BFS: bottom-up (pull) only, needs transpose  */
+#define OLV_FORCE_BOTTOM_UP  0x0004u /* BFS: every level after the first pulls
+                                        (bottom-up); builds the transpose         */
 #define OLV_HOST_LOOP        0x0008u /* wait for every level on the host instead
                                         of enqueueing levels in chunks whose
                                         control lives on the device             */

@@ -64,6 +64,7 @@ struct BfsWork {
   GridBar *bar = nullptr;           // persistent mode
   unsigned long long *stat = nullptr;
   uint32_t last_levels = 0;         // depth of the previous run (chunk sizing)
+  bool force_bu = false;            // OLV_FORCE_BOTTOM_UP of the current run
 };
 
 void free_bfs_work(olv_graph *g) {
@@ -146,6 +147,7 @@ __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, 
 
 struct BfsDir {
   uint32_t can_bu;
+  uint32_t force_bu;  // every level after the first bottom-up (OLV_FORCE_BOTTOM_UP)
   float alpha, beta;
 };
 
@@ -197,7 +199,8 @@ __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *
     return;
   }
   const unsigned long long mu = st.mu > c.mf ? st.mu - c.mf : 0ull;
-  const bool next_bu = choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
+  const bool next_bu =
+      dir.force_bu != 0 || choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
   if (lead) {
     BfsState ns;
     ns.mu = mu;
@@ -254,7 +257,8 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
       break;
     }
     const unsigned long long mu = st.mu > c.mf ? st.mu - c.mf : 0ull;
-    const bool next_bu = choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
+    const bool next_bu =
+        dir.force_bu != 0 || choose_bottom_up(st.mode != 0, dir.can_bu != 0, c.nf, c.mf, mu, a.n, dir.alpha, dir.beta);
     st.mu = mu;
     st.mode = next_bu ? 1u : 0u;
     st.nbu = nbu;
@@ -308,9 +312,10 @@ static unsigned grid_for(size_t items, size_t per_block, size_t per_cu) {
   return (unsigned)std::max<size_t>(1, std::min(b, cap));
 }
 
-static BfsDir make_dir(const olv_graph *g, bool can_bu) {
+static BfsDir make_dir(const olv_graph *g, const BfsWork *w, bool can_bu) {
   BfsDir d;
   d.can_bu = (can_bu && g->in_row_ptr) ? 1u : 0u;
+  d.force_bu = (d.can_bu && w->force_bu) ? 1u : 0u;
   d.alpha = g->tune.alpha;
   d.beta = g->tune.beta;
   return d;
@@ -337,7 +342,7 @@ int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, ui
 // Stream-ordered, capturable: `count` levels (expand + compact each).
 int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count) {
   const BfsArgs a = make_args(g, w, levels);
-  const BfsDir dir = make_dir(g, can_bu);
+  const BfsDir dir = make_dir(g, w, can_bu);
   const unsigned ge = (unsigned)cu_count() * 8;
   const unsigned gc = grid_for(w->nw, kBlock, 8);
   for (uint32_t i = 0; i < count; ++i) {
@@ -358,7 +363,7 @@ static int enqueue_persistent(const olv_graph *g, BfsWork *w, int32_t *levels, b
   BfsCtl *ctl = w->ctl;
   uint32_t *bms = w->bms;
   uint32_t nw = w->nw, nwp = w->nwp;
-  BfsDir dir = make_dir(g, can_bu);
+  BfsDir dir = make_dir(g, w, can_bu);
   GridBar *bar = w->bar;
   void *args[] = {&a, &ctl, &bms, &nw, &nwp, &dir, &bar};
   OLV_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(bfs_persistent_kernel), dim3(grid), dim3(kBlock),
@@ -427,8 +432,11 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
   for (uint32_t i = 0; i < k; ++i)
     if (h_seeds[i] >= g->n) return fail(OLV_EINVAL, "olv_bfs: source %u out of range (n=%u)", h_seeds[i], g->n);
   const bool can_bu = !(flags & OLV_NO_DIRECTION_OPT);
+  if (!can_bu && (flags & OLV_FORCE_BOTTOM_UP))
+    return fail(OLV_EINVAL, "olv_bfs: OLV_FORCE_BOTTOM_UP and OLV_NO_DIRECTION_OPT together");
   BfsWork *w = nullptr;
   OLV_TRY(bfs_prepare(g, can_bu, &w));
+  w->force_bu = (flags & OLV_FORCE_BOTTOM_UP) != 0;
   DevBuf d_seeds, d_levels;
   OLV_TRY(d_seeds.alloc((size_t)k * 4));
   OLV_HIP(hipMemcpyAsync(d_seeds.p, h_seeds.data(), (size_t)k * 4, hipMemcpyHostToDevice, stream()));

@@ -27,8 +27,6 @@ struct PrWork {
   uint32_t n = 0, m = 0;
   float *buf[4] = {nullptr, nullptr, nullptr, nullptr};  // n floats each
   float *w_perm = nullptr;                                // m floats
-  float *err_part = nullptr;
-  uint32_t err_cap = 0;
   IterCtl *ctl = nullptr;
   IterCtl *h_ctl = nullptr;
 };
@@ -39,14 +37,13 @@ void free_pr_work(olv_graph *g) {
   for (float *p : w->buf)
     if (p) (void)hipFree(p);
   if (w->w_perm) (void)hipFree(w->w_perm);
-  if (w->err_part) (void)hipFree(w->err_part);
   if (w->ctl) (void)hipFree(w->ctl);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
   delete w;
   g->pr = nullptr;
 }
 
-static int get_pr_work(olv_graph *g, bool need_wperm, uint32_t err_blocks, PrWork **out) {
+static int get_pr_work(olv_graph *g, bool need_wperm, PrWork **out) {
   if (!g->pr) {
     PrWork *w = new PrWork();
     g->pr = w;
@@ -58,12 +55,6 @@ static int get_pr_work(olv_graph *g, bool need_wperm, uint32_t err_blocks, PrWor
   }
   PrWork *w = g->pr;
   if (need_wperm && !w->w_perm) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->w_perm), std::max<size_t>(g->m, 1) * 4));
-  if (w->err_cap < err_blocks) {
-    if (w->err_part) OLV_HIP(hipFree(w->err_part));
-    w->err_part = nullptr;
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->err_part), (size_t)err_blocks * 4));
-    w->err_cap = err_blocks;
-  }
   *out = w;
   return OLV_OK;
 }
@@ -153,6 +144,14 @@ struct GradOp {
 };
 
 // --------------------------------------------------------------- kernels
+// Zero the control block and set eps (all threads of the calling workgroup).
+__device__ __forceinline__ void ctl_reset(IterCtl *ctl, double eps) {
+  uint32_t *w = reinterpret_cast<uint32_t *>(ctl);
+  for (uint32_t i = threadIdx.x; i < sizeof(IterCtl) / 4; i += blockDim.x) w[i] = 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) ctl->eps = eps;
+}
+
 __global__ void pr_init_kernel(uint32_t n, float r0, const float *__restrict__ inv_outdeg, float *__restrict__ r,
                                float *__restrict__ contrib, IterCtl *__restrict__ ctl, double eps) {
   const uint32_t tid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -160,22 +159,10 @@ __global__ void pr_init_kernel(uint32_t n, float r0, const float *__restrict__ i
     r[v] = r0;
     contrib[v] = r0 * inv_outdeg[v];
   }
-  if (tid == 0) {
-    ctl->done = 0;
-    ctl->iters = 0;
-    ctl->eps = eps;
-    ctl->residual = 0.0;
-    ctl->arrived = 0;
-  }
+  if (blockIdx.x == 0) ctl_reset(ctl, eps);
 }
 
-__global__ void ctl_init_kernel(IterCtl *ctl, double eps) {
-  ctl->done = 0;
-  ctl->iters = 0;
-  ctl->eps = eps;
-  ctl->residual = 0.0;
-  ctl->arrived = 0;
-}
+__global__ void ctl_init_kernel(IterCtl *ctl, double eps) { ctl_reset(ctl, eps); }
 
 // eps = tol * sum |gbar| (single workgroup)
 __global__ void __launch_bounds__(1024) ctl_init_l1_kernel(IterCtl *ctl, const float *__restrict__ g, uint32_t n,
@@ -184,13 +171,8 @@ __global__ void __launch_bounds__(1024) ctl_init_l1_kernel(IterCtl *ctl, const f
   double s = 0.0;
   for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += fabs((double)g[i]);
   s = block_sum<1024, double>(s, s_red);
-  if (threadIdx.x == 0) {
-    ctl->done = 0;
-    ctl->iters = 0;
-    ctl->eps = tol * s;
-    ctl->residual = 0.0;
-    ctl->arrived = 0;
-  }
+  ctl_reset(ctl, 0.0);
+  if (threadIdx.x == 0) ctl->eps = tol * s;
 }
 
 __global__ void permute_kernel(const float *__restrict__ w, const uint32_t *__restrict__ eid, uint32_t m,
@@ -237,16 +219,12 @@ struct KernelProfile {
 };
 
 template <class Op>
-static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, PrWork *w,
+static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, PrWork *w, uint32_t k,
                              KernelProfile *prof = nullptr) {
-  const uint32_t blocks = seg_grid(d);
   hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
   if (a) OLV_HIP(hipEventRecord(a, stream()));
-  hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(blocks), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl,
-                     w->err_part);
+  hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl, k);
   if (b) OLV_HIP(hipEventRecord(b, stream()));
-  hipLaunchKernelGGL(seg_finalize_kernel<Op>, dim3(1), dim3(1024), 0, stream(), d, op, (const float *)w->err_part,
-                     blocks, w->ctl);
   OLV_HIP(hipGetLastError());
   return OLV_OK;
 }
@@ -255,6 +233,31 @@ static int read_ctl(PrWork *w) {
   OLV_HIP(hipMemcpyAsync(w->h_ctl, w->ctl, sizeof(IterCtl), hipMemcpyDeviceToHost, stream()));
   OLV_HIP(hipStreamSynchronize(stream()));
   return OLV_OK;
+}
+
+// Outcome after `enq` enqueued iterations (control block read back):
+// updates performed, the last residual, and whether it met eps.
+struct IterResult {
+  uint32_t iters;
+  double residual;
+  bool converged;
+};
+
+static IterResult iter_result(const IterCtl &h, uint32_t enq) {
+  IterResult r;
+  if (h.done) {
+    r.iters = h.iters;
+    r.residual = h.residual;
+    r.converged = true;
+    return r;
+  }
+  double s = 0.0;
+  if (enq)
+    for (int i = 0; i < kErrSlots; ++i) s += h.errs[(enq - 1) % 3][i].v;
+  r.iters = enq;
+  r.residual = s;
+  r.converged = enq > 0 && s < h.eps;
+  return r;
 }
 
 // Host-pointer staging for OLV_PTR_HOST.
@@ -286,7 +289,7 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   OLV_TRY(ensure_plan(g, true));
   const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, seg_grid(d), &w));
+  OLV_TRY(get_pr_work(g, false, &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage so;
@@ -301,11 +304,12 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   const float base = (float)((1.0 - (double)damping) / (double)g->n);
   for (uint32_t it = 0; it < max_iters; ++it) {
     PrOp op{g->in_col, w->buf[it & 1], w->buf[(it + 1) & 1], r, g->inv_outdeg, damping, base};
-    OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, &prof));
+    OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it, &prof));
   }
   t.stop();
   const double ms = t.ms();
   OLV_TRY(read_ctl(w));
+  const IterResult res = iter_result(*w->h_ctl, max_iters);
   double kms = 0;
   uint32_t kl = 0;
   if (prof.on) OLV_TRY(prof.total(&kms, &kl));
@@ -313,9 +317,9 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
-    st->iterations = w->h_ctl->iters;
-    st->residual = w->h_ctl->residual;
-    st->edges_traversed = (uint64_t)g->m * w->h_ctl->iters;
+    st->iterations = res.iters;
+    st->residual = res.residual;
+    st->edges_traversed = (uint64_t)g->m * res.iters;
     st->edges_inspected = st->edges_traversed;
     st->vertices_reached = g->n;
     st->kernel_ms = kms;
@@ -331,7 +335,7 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   OLV_TRY(ensure_plan(g, true));
   const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, true, seg_grid(d), &w));
+  OLV_TRY(get_pr_work(g, true, &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, st_t, so;
@@ -345,22 +349,23 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   t.start();
   if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
   OLV_HIP(hipMemcpyAsync(w->buf[0], dt, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
-  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(1), 0, stream(), w->ctl, (double)tol);
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
   const uint32_t group = 8;
   uint32_t it = 0;
+  IterResult res{0, 0.0, false};
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
       WprOp op{g->in_col, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], dt, damping, 1.0f - damping};
-      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w));
+      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it));
     }
+    t.stop();
     OLV_TRY(read_ctl(w));
-    if (w->h_ctl->done) break;
+    res = iter_result(*w->h_ctl, it);
+    if (res.converged) break;
   }
-  t.stop();
   const double ms = t.ms();
-  OLV_TRY(read_ctl(w));
-  const uint32_t iters = w->h_ctl->iters;
+  const uint32_t iters = res.iters;
   OLV_HIP(hipMemcpyAsync(r, w->buf[iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
   if (host) OLV_HIP(hipMemcpy(ranks, r, (size_t)g->n * 4, hipMemcpyDeviceToHost));
   else OLV_HIP(hipStreamSynchronize(stream()));
@@ -368,13 +373,14 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
     st->iterations = iters;
-    st->residual = w->h_ctl->residual;
+    st->residual = res.residual;
     st->edges_traversed = (uint64_t)g->m * iters;
     st->edges_inspected = st->edges_traversed;
     st->vertices_reached = g->n;
   }
-  if (!w->h_ctl->done) return fail(OLV_ENOCONV, "olv_pagerank_weighted_f32: residual %.3e after %u iterations (tol %.3e)",
-                                   w->h_ctl->residual, iters, (double)tol);
+  if (!res.converged)
+    return fail(OLV_ENOCONV, "olv_pagerank_weighted_f32: residual %.3e after %u iterations (tol %.3e)", res.residual,
+                iters, (double)tol);
   return OLV_OK;
 }
 
@@ -386,7 +392,7 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(ensure_plan(g, false));
   const SegDev d = make_seg_dev(g->plan_out);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, seg_grid(d), &w));
+  OLV_TRY(get_pr_work(g, false, &w));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, sr, sg, sgw, sgt, sl;
@@ -410,21 +416,23 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   hipLaunchKernelGGL(ctl_init_l1_kernel, dim3(1), dim3(1024), 0, stream(), w->ctl, dg, g->n, (double)tol);
   const uint32_t group = 8;
   uint32_t it = 0;
+  IterResult res{0, 0.0, false};
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
       AdjOp op{g->col, dw, lam[it & 1], lam[(it + 1) & 1], dg, damping};
-      OLV_TRY(enqueue_iteration(d, g->row_ptr, op, w));
+      OLV_TRY(enqueue_iteration(d, g->row_ptr, op, w, it));
     }
     OLV_TRY(read_ctl(w));
-    if (w->h_ctl->done) break;
+    res = iter_result(*w->h_ctl, it);
+    if (res.converged) break;
   }
-  const uint32_t iters = w->h_ctl->iters;
+  const uint32_t iters = res.iters;
   const float *lf = lam[iters & 1];
   if (dgw && g->m) {
     GradOp gop{g->col, dr, lf, dgw, damping};
     hipLaunchKernelGGL(seg_iter_kernel<GradOp>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, g->row_ptr,
-                       gop, (IterCtl *)nullptr, w->err_part);
+                       gop, (IterCtl *)nullptr, 0u);
     // chunked hub rows wrote their edges inside the chunk blocks already
   }
   if (dgt) hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf, 1.0f - damping, g->n, dgt);
@@ -441,13 +449,13 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
     st->iterations = iters;
-    st->residual = w->h_ctl->residual;
+    st->residual = res.residual;
     st->edges_traversed = (uint64_t)g->m * (iters + 1);
     st->edges_inspected = st->edges_traversed;
     st->vertices_reached = g->n;
   }
-  if (!w->h_ctl->done) return fail(OLV_ENOCONV, "olv_pagerank_vjp_f32: residual %.3e after %u iterations",
-                                   w->h_ctl->residual, iters);
+  if (!res.converged)
+    return fail(OLV_ENOCONV, "olv_pagerank_vjp_f32: residual %.3e after %u iterations", res.residual, iters);
   return OLV_OK;
 }
 

@@ -27,6 +27,7 @@ struct SegDev {
   const uint32_t *hub_first;
   const uint32_t *chunk_hub;
   float *chunk_partial;
+  uint32_t *hub_done;  // chunks of each chunked row finished in this launch
   uint32_t off[SegPlan::kBuckets + 1];
   uint32_t nhub, nchunks, chunk;
   // first virtual block of each role: [0]=hub chunks, [1]=bucket3,
@@ -35,6 +36,8 @@ struct SegDev {
 };
 
 constexpr int kSegBlock = 256;
+// s_waitcnt operand (gfx9 encoding): vmcnt = 0, expcnt and lgkmcnt unbounded.
+constexpr int kWaitVmcnt0 = (0x7 << 4) | (0xF << 8);
 
 inline SegDev make_seg_dev(const SegPlan &p) {
   SegDev d;
@@ -43,6 +46,7 @@ inline SegDev make_seg_dev(const SegPlan &p) {
   d.hub_first = p.hub_first;
   d.chunk_hub = p.chunk_hub;
   d.chunk_partial = p.chunk_partial;
+  d.hub_done = p.hub_done;
   for (uint32_t i = 0; i <= SegPlan::kBuckets; ++i) d.off[i] = p.offsets[i];
   d.nhub = p.nhub;
   d.nchunks = p.nchunks;
@@ -63,7 +67,7 @@ inline SegDev make_seg_dev(const SegPlan &p) {
 // Workgroups per CU of a segment launch (0: one workgroup per virtual
 // block); never more workgroups than there are virtual blocks.
 #ifndef OLV_SEG_GRID_PER_CU
-#define OLV_SEG_GRID_PER_CU 16
+#define OLV_SEG_GRID_PER_CU 32
 #endif
 // Independent gathers in flight per lane.
 #ifndef OLV_SEG_UNROLL
@@ -106,68 +110,108 @@ __device__ __forceinline__ float lane_sum(const Op &op, uint32_t v, uint32_t beg
   return acc[0];
 }
 
-// Iteration control block (device).
-struct IterCtl {
-  uint32_t done;      // set once the residual drops below eps
-  uint32_t iters;     // updates performed
-  double eps;         // stop threshold on the L1 residual
-  double residual;    // last residual
-  uint32_t arrived;   // workgroups finished in the current launch
-  uint32_t pad;
+// Iteration control block (device).  The L1 residual of iteration k is
+// accumulated in errs[k % 3] -- kErrSlots copies on separate 128-byte lines,
+// workgroup b adding to copy b % kErrSlots (device-scope atomics to one
+// address serialise at the memory side).  Iteration k + 1 reads it at its
+// start, so no separate reduction launch is needed between iterations.
+constexpr int kErrSlots = 32;
+
+struct alignas(128) ErrSlot {
+  double v;
+  double pad[15];
 };
 
-// Finish the chunked hub rows (partials in chunk order), total the
-// residual, set `done`, re-arm the arrival counter.  Called by the last
-// workgroup of a launch (fused) or by a one-workgroup kernel.
-template <int BLOCK, class Op>
-__device__ void seg_finalize(const SegDev &d, const Op &op, const float *err_part, uint32_t nparts, IterCtl *ctl) {
-  __shared__ double s_red[BLOCK / kWave];
-  float err = 0.f;
-  for (uint32_t h = threadIdx.x; h < d.nhub; h += BLOCK) {
-    float s = 0.f;
-    for (uint32_t c = d.hub_first[h]; c < d.hub_first[h + 1]; ++c)
-      s += __hip_atomic_load(&d.chunk_partial[c], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    op.finish(d.hub_vertex[h], s, err);
+struct IterCtl {
+  uint32_t done;      // set by the first iteration that finds the residual below eps
+  uint32_t iters;     // updates performed, when done
+  double eps;         // stop threshold on the L1 residual
+  double residual;    // residual that stopped the iteration, when done
+  uint32_t pad[26];
+  ErrSlot errs[3][kErrSlots];
+};
+
+// Total of errs[slot] for every thread of the calling workgroup (all
+// threads must call).
+__device__ __forceinline__ double err_total(const IterCtl *ctl, uint32_t slot) {
+  __shared__ double s_tot;
+  if (threadIdx.x < kWave) {
+    double x = threadIdx.x < kErrSlots ? ctl->errs[slot][threadIdx.x].v : 0.0;
+    x = wave_sum(x);
+    if (threadIdx.x == 0) s_tot = x;
   }
-  double tot = err;
-  for (uint32_t i = threadIdx.x; i < nparts; i += BLOCK)
-    tot += __hip_atomic_load(&err_part[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  tot = block_sum<BLOCK, double>(tot, s_red);
-  if (threadIdx.x == 0) {
-    ctl->residual = tot;
-    ctl->iters += 1;
-    if (tot < ctl->eps) ctl->done = 1u;
-    ctl->arrived = 0u;
-  }
+  __syncthreads();
+  const double r = s_tot;
+  __syncthreads();
+  return r;
 }
 
 // Op: uint32_t idx(uint32_t k)                 -- index read of slot k
 //     float val(uint32_t v, uint32_t k, uint32_t i) -- summand of slot k of row v
 //     void finish(uint32_t v, float sum, float &err)
-// ctl == nullptr: no residual (pure edge pass, e.g. gradients).  The
-// residual partials are totalled by seg_finalize_kernel in a second launch:
-// a last-workgroup finalize inside this kernel would need an agent-scope
-// release fence per workgroup, which on gfx950 writes back the XCD's whole
-// L2 each time.
+// Iteration k of a fixed point (ctl != nullptr): returns at once when done;
+// stops (sets done) if iteration k - 1 left a residual below eps; else
+// updates every vertex and adds its residual to errs[k % 3].  ctl ==
+// nullptr: a pure edge pass (e.g. gradients).  Rows longer than d.chunk are
+// cut into chunks; the workgroup that completes a row's last chunk sums the
+// chunk partials in chunk order (fixed order: reproducible) and finishes it.
 template <class Op>
 __global__ void __launch_bounds__(kSegBlock) seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op,
-                                                             IterCtl *ctl, float *__restrict__ err_part) {
-  if (ctl && __hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+                                                             IterCtl *ctl, uint32_t k) {
+  if (ctl) {
+    if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (k > 0) {
+      const double res = err_total(ctl, (k - 1) % 3);
+      if (res < ctl->eps) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+          ctl->iters = k;
+          ctl->residual = res;
+          __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+      }
+    }
+    if (blockIdx.x == 0) {
+      double *z = &ctl->errs[(k + 1) % 3][0].v;
+      for (uint32_t i = threadIdx.x; i < kErrSlots * (sizeof(ErrSlot) / sizeof(double)); i += kSegBlock) z[i] = 0.0;
+    }
+  }
   __shared__ float s_red[kSegBlock / kWave];
   float err = 0.f;
   const uint32_t nvb = d.role_start[5];
   for (uint32_t b = blockIdx.x; b < nvb; b += gridDim.x) {
     if (b < d.role_start[1]) {
-      // hub chunk: partial sum only
+      // hub chunk: partial sum; the last chunk of the row to finish
+      // completes the row
       const uint32_t c = b;
       const uint32_t h = d.chunk_hub[c];
       const uint32_t v = d.hub_vertex[h];
-      const uint32_t k0 = c - d.hub_first[h];
-      const uint32_t beg = rp[v] + k0 * d.chunk;
+      const uint32_t c0 = d.hub_first[h], c1 = d.hub_first[h + 1];
+      const uint32_t beg = rp[v] + (c - c0) * d.chunk;
       const uint32_t end = min(rp[v + 1], beg + d.chunk);
       float s = lane_sum<kSegBlock>(op, v, beg, end, threadIdx.x);
       s = block_sum<kSegBlock, float>(s, s_red);
-      if (threadIdx.x == 0) d.chunk_partial[c] = s;
+      if (threadIdx.x == 0) {
+        // The partials are device-coherent atomic stores and loads, so the
+        // hand-off needs ordering, not cache maintenance: wait for the
+        // partial's store to complete before counting in (an acq_rel
+        // atomic would also write back and invalidate this XCD's whole L2,
+        // once per chunk and iteration).
+        __hip_atomic_store(&d.chunk_partial[c], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        __builtin_amdgcn_s_waitcnt(kWaitVmcnt0);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        const uint32_t before =
+            __hip_atomic_fetch_add(&d.hub_done[h], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __atomic_signal_fence(__ATOMIC_SEQ_CST);
+        if (before == c1 - c0 - 1) {
+          float t = 0.f;
+          for (uint32_t q = c0; q < c1; ++q)
+            t += __hip_atomic_load(&d.chunk_partial[q], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          op.finish(v, t, err);
+          __hip_atomic_store(&d.hub_done[h], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+      }
     } else if (b < d.role_start[2]) {
       const uint32_t v = d.list[d.off[3] + (b - d.role_start[1])];
       float s = lane_sum<kSegBlock>(op, v, rp[v], rp[v + 1], threadIdx.x);
@@ -205,14 +249,8 @@ __global__ void __launch_bounds__(kSegBlock) seg_iter_kernel(SegDev d, const uin
   }
   if (!ctl) return;
   err = block_sum<kSegBlock, float>(err, s_red);
-  if (threadIdx.x == 0) err_part[blockIdx.x] = err;
-}
-
-template <class Op>
-__global__ void __launch_bounds__(1024) seg_finalize_kernel(SegDev d, Op op, const float *__restrict__ err_part,
-                                                            uint32_t nparts, IterCtl *__restrict__ ctl) {
-  if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-  seg_finalize<1024>(d, op, err_part, nparts, ctl);
+  if (threadIdx.x == 0 && err != 0.f)
+    atomicAdd(&ctl->errs[k % 3][blockIdx.x % kErrSlots].v, (double)err);
 }
 
 }  // namespace olv

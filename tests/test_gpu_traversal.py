@@ -21,6 +21,8 @@ MODES = {
     "push_only_host_loop": 0x2 | 0x8,
     "persistent": 0x20,                 # one cooperative kernel, grid barrier
     "persistent_push_only": 0x20 | 0x2,
+    "bottom_up": 0x4,                   # every level after the first pulls
+    "persistent_bottom_up": 0x20 | 0x4,
 }
 
 
