@@ -155,9 +155,16 @@ __device__ __forceinline__ double err_total(const IterCtl *ctl, uint32_t slot) {
 // nullptr: a pure edge pass (e.g. gradients).  Rows longer than d.chunk are
 // cut into chunks; the workgroup that completes a row's last chunk sums the
 // chunk partials in chunk order (fixed order: reproducible) and finishes it.
+// Waves per SIMD the iteration kernel is compiled for (register budget):
+// the kernel is bound by gather latency, so residency matters more than
+// the few scalar registers spilled to fit it.
+#ifndef OLV_SEG_WAVES_PER_EU
+#define OLV_SEG_WAVES_PER_EU 8
+#endif
+
 template <class Op>
-__global__ void __launch_bounds__(kSegBlock) seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op,
-                                                             IterCtl *ctl, uint32_t k) {
+__global__ void __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(OLV_SEG_WAVES_PER_EU)))
+seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, uint32_t k) {
   if (ctl) {
     if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     if (k > 0) {
