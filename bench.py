@@ -182,7 +182,8 @@ def bench_bfs(args, olv):
     res = base_line(args, cfg, r["gteps_harmonic"], "GTEPS", r["ms_mean"], "int32")
     res["roofline"] = {"bound": "hbm", "achieved": r["hbm_frac_mean"] * HBM_PEAK_GBS, "peak": HBM_PEAK_GBS,
                        "unit": "GB/s", "frac": r["hbm_frac_mean"], "traffic": load_traffic("bfs_rmat22"),
-                       "kernel": "bfs_persistent_kernel", "bytes_model": "8*E_r + 16*V_r + 4*V"}
+                       "kernel": "bfs_expand_kernel + bfs_compact_kernel (whole traversal)",
+                       "bytes_model": "8*E_r + 16*V_r + 4*V"}
     if "cpu_gteps" in r:
         res["cpu_baseline"] = {"value": r["cpu_gteps"], "unit": "GTEPS", "cores": 1, "kind": "port",
                                "sample": "one source of the same graph, serial restatement of testBFS.cu:56-79"}
@@ -213,7 +214,8 @@ def bench_sssp(args, olv):
     b = 12 * er + 16 * vr + 4 * csr.n
     res["roofline"] = {"bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": b / t / 1e9 / HBM_PEAK_GBS, "traffic": load_traffic("sssp_uniform"),
-                       "kernel": "sssp_persistent_kernel", "bytes_model": "12*E_r + 16*V_r + 4*V"}
+                       "kernel": "sssp_relax_kernel + sssp_compact_kernel (whole run)",
+                       "bytes_model": "12*E_r + 16*V_r + 4*V"}
     if not args.no_cpu_baseline:
         tc, runs = timed_oracle(lambda: oracle.sssp(csr.row_ptr, csr.col, csr.weights, 0), 30.0, 2)
         res["cpu_baseline"] = {"value": er / tc / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
