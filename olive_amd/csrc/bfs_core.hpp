@@ -16,7 +16,10 @@
 
 namespace olv {
 
-constexpr uint32_t kHubChunk = 4096;
+#ifndef OLV_HUB_CHUNK
+#define OLV_HUB_CHUNK 1024
+#endif
+constexpr uint32_t kHubChunk = OLV_HUB_CHUNK;
 
 // Level counters.  Every workgroup adds its totals with device-scope
 // atomics, which on MI355X are performed at the memory side and serialise

@@ -8,6 +8,10 @@
 #include "error.hpp"
 #include "olive_hip.h"
 
+#ifndef OLV_HUB_CHUNK
+#define OLV_HUB_CHUNK 1024
+#endif
+
 namespace olv {
 
 hipStream_t stream();
@@ -37,7 +41,7 @@ struct SegPlan {
 struct Tuning {
   uint32_t td_block_thr = 256;  // degree >= this: whole workgroup per vertex
   uint32_t td_wave_thr = 64;    // degree >= this: one wave per vertex
-  uint32_t hub_thr = 4096;      // degree >= this: split into 4096-edge items
+  uint32_t hub_thr = OLV_HUB_CHUNK;  // degree >= this: split into OLV_HUB_CHUNK-edge items
   uint32_t block_threads = 256;
   float alpha = 14.0f;          // top-down -> bottom-up when m_f > m_u / alpha
   float beta = 24.0f;           // bottom-up -> top-down when n_f < n / beta

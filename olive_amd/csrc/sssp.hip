@@ -76,18 +76,20 @@ struct SsspVisitor {
     return make_uint2(base + col[e], (uint32_t)du + (uint32_t)w[e]);
   }
   __device__ __forceinline__ int32_t state(uint2 t) const { return dist[t.x]; }
+  // Relax: lower dist[v] and mark v.  Both atomics are non-returning (no
+  // lane waits on them): a relaxation that loses a race to a lower distance
+  // still marks v, which only re-expands v with its final distance, and nf
+  // may count a vertex more than once -- it only has to be non-zero exactly
+  // when something improved.
   __device__ __forceinline__ void apply(uint2 t, int32_t cur, P) {
     const uint32_t v = t.x;
     const int32_t cand = (int32_t)t.y;
     if (cand >= cur) return;
-    const int32_t old = atomicMin(&dist[v], cand);
-    if (cand >= old) return;
+    __hip_atomic_fetch_min(&dist[v], cand, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     const uint32_t bit = 1u << (v & 31);
-    if (far_bm && cand >= split) {
-      if (!(far_bm[v >> 5] & bit)) atomicOr(&far_bm[v >> 5], bit);
-      return;
-    }
-    if (!(atomicOr(&next_bm[v >> 5], bit) & bit)) nf += 1;
+    uint32_t *bm = (far_bm && cand >= split) ? far_bm : next_bm;
+    __hip_atomic_fetch_or(&bm[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (bm == next_bm) nf += 1;
   }
 };
 
