@@ -35,6 +35,19 @@ __device__ __forceinline__ T ld_stream(const T *p) {
 #endif
 }
 
+// Random gather from a vector (PageRank contributions, ranks, adjoints).
+#ifndef OLV_GATHER_NT
+#define OLV_GATHER_NT 0
+#endif
+template <class T>
+__device__ __forceinline__ T ld_gather(const T *p) {
+#if OLV_GATHER_NT
+  return __builtin_nontemporal_load(p);
+#else
+  return *p;
+#endif
+}
+
 template <class T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll

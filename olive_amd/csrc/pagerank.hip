@@ -84,7 +84,7 @@ struct PrOp {
   const float *inv_outdeg;
   float d, base;
   __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
-  __device__ __forceinline__ float val(uint32_t, uint32_t, uint32_t u) const { return contrib_cur[u]; }
+  __device__ __forceinline__ float val(uint32_t, uint32_t, uint32_t u) const { return ld_gather(&contrib_cur[u]); }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + base;
     err += fabsf(rn - r[v]);
@@ -103,7 +103,7 @@ struct WprOp {
   const float *t;
   float d, omd;
   __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
-  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t u) const { return ld_stream(&w[k]) * r_cur[u]; }
+  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t u) const { return ld_stream(&w[k]) * ld_gather(&r_cur[u]); }
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + omd * t[v];
     err += fabsf(rn - r_cur[v]);
@@ -120,7 +120,7 @@ struct AdjOp {
   const float *gbar;
   float d;
   __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
-  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t v) const { return ld_stream(&w[k]) * l_cur[v]; }
+  __device__ __forceinline__ float val(uint32_t, uint32_t k, uint32_t v) const { return ld_stream(&w[k]) * ld_gather(&l_cur[v]); }
   __device__ __forceinline__ void finish(uint32_t u, float s, float &err) const {
     const float ln = gbar[u] + d * s;
     err += fabsf(ln - l_cur[u]);
