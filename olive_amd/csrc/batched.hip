@@ -821,6 +821,7 @@ static int graph_union(olv_graph *const *gs, uint32_t k, olv_graph **out, std::v
       if ((st = concat_rows(irps.data(), ns.data(), voff.data(), eoff.data(), k, u->n, u->m, &u->in_row_ptr))) break;
       if ((st = concat_cols(ics.data(), ms.data(), eoff.data(), voff.data(), k, u->m, &u->in_col))) break;
       if ((st = concat_cols(ies.data(), ms.data(), eoff.data(), eoff.data(), k, u->m, &u->in_eid))) break;
+      if ((st = ensure_in_head(u))) break;
     }
     if (hipStreamSynchronize(stream()) != hipSuccess) st = fail(OLV_EHIP, "olv_graph_union: build failed");
   } while (0);

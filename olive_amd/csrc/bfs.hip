@@ -294,6 +294,7 @@ static BfsArgs make_args(const olv_graph *g, BfsWork *w, int32_t *levels) {
   a.col = g->col;
   a.in_row_ptr = g->in_row_ptr;
   a.in_col = g->in_col;
+  a.in_head = g->in_head;
   a.levels = levels;
   a.visited = w->visited;
   a.snap = w->snap;
@@ -324,6 +325,7 @@ static BfsDir make_dir(const olv_graph *g, const BfsWork *w, bool can_bu) {
 // Transpose (for bottom-up steps) and work buffers of g.
 int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w) {
   if (can_bu && !g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(const_cast<olv_graph *>(g)));
+  if (can_bu) OLV_TRY(ensure_in_head(const_cast<olv_graph *>(g)));
   return get_work(g, w);
 }
 

@@ -78,6 +78,7 @@ struct BfsArgs {
   const uint32_t *col;
   const uint32_t *in_row_ptr;
   const uint32_t *in_col;
+  const uint2 *in_head;  // first two in-neighbours (0xffffffff where absent)
   int32_t *levels;
   uint32_t *visited;
   uint32_t *snap;      // visited as of the last frontier extraction
@@ -308,11 +309,28 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
       e[j] = ee[j] = 0;
     }
     if (!ballot(any_todo)) continue;
+    // the first two in-neighbours, one coalesced load per vertex (without
+    // a head array every in-list is scanned from its first entry)
+    uint2 hd[J];
+    const bool heads = a.in_head != nullptr;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const uint32_t v = (uint32_t)(base + j * stride) + lane;
+      hd[j] = (todo[j] && heads) ? a.in_head[v] : make_uint2(0xffffffffu, 0xffffffffu);
+    }
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const bool okx = hd[j].x != 0xffffffffu, oky = hd[j].y != 0xffffffffu;
+      const bool hit = (okx & bm_test(front_bm, okx ? hd[j].x : 0u)) | (oky & bm_test(front_bm, oky ? hd[j].y : 0u));
+      found[j] = todo[j] && hit;
+      if (todo[j]) insp += (uint32_t)okx + (uint32_t)oky;
+    }
+    // the rest of the in-list, from its third entry, for the vertices not done
 #pragma unroll
     for (int j = 0; j < J; ++j)
-      if (todo[j]) {
+      if (todo[j] && !found[j] && (!heads || hd[j].y != 0xffffffffu)) {
         const uint32_t v = (uint32_t)(base + j * stride) + lane;
-        e[j] = a.in_row_ptr[v];
+        e[j] = a.in_row_ptr[v] + (heads ? 2u : 0u);
         ee[j] = a.in_row_ptr[v + 1];
       }
     // own scans: U in-edges of each of the J vertices per step, all loads

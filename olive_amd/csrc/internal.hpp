@@ -61,6 +61,7 @@ struct olv_graph {
   int wdtype = OLV_W_NONE;
   uint32_t *in_row_ptr = nullptr, *in_col = nullptr, *in_eid = nullptr;
   uint32_t *no_in_bm = nullptr;    // bitmap: in-degree 0 (built with the transpose)
+  uint2 *in_head = nullptr;        // first two in-neighbours per vertex (built with the transpose)
   bool wstat_ready = false;        // int32 weight statistics (SSSP near-far step)
   int32_t wmin = 0;
   double wmean = 0.0;
@@ -80,6 +81,7 @@ namespace olv {
 // Device helpers implemented in runtime.hip
 int fill_u32(uint32_t *p, uint32_t v, size_t count);
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
+int ensure_in_head(olv_graph *g);
 void free_bfs_work(olv_graph *g);
 void free_pr_work(olv_graph *g);
 

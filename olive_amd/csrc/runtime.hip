@@ -211,6 +211,29 @@ static int upload(uint32_t **dst, const uint32_t *src, size_t count) {
   return OLV_OK;
 }
 
+// First two in-neighbours of every vertex (0xffffffff where absent).
+__global__ void in_head_kernel(const uint32_t *__restrict__ irp, const uint32_t *__restrict__ icol, uint32_t n,
+                               uint2 *__restrict__ head) {
+  for (uint32_t v = blockIdx.x * blockDim.x + threadIdx.x; v < n; v += gridDim.x * blockDim.x) {
+    const uint32_t b = irp[v], d = irp[v + 1] - b;
+    head[v] = make_uint2(d > 0 ? icol[b] : 0xffffffffu, d > 1 ? icol[b + 1] : 0xffffffffu);
+  }
+}
+
+// Build g->in_head from the in-CSR on the device if it is missing (graphs
+// whose in-CSR was assembled on the device, e.g. graph unions).
+int ensure_in_head(olv_graph *g) {
+  if (g->in_head || !g->in_row_ptr) return OLV_OK;
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&g->in_head), std::max<size_t>(g->n, 1) * sizeof(uint2)));
+  if (g->n) {
+    hipLaunchKernelGGL(in_head_kernel, dim3(fill_grid(g->n)), dim3(256), 0, g_stream, g->in_row_ptr, g->in_col, g->n,
+                       g->in_head);
+    OLV_HIP(hipGetLastError());
+    OLV_HIP(hipStreamSynchronize(g_stream));
+  }
+  return OLV_OK;
+}
+
 static int attach_transpose(olv_graph *g, const uint32_t *h_rp, const uint32_t *h_col) {
   std::vector<uint32_t> irp, icol, ieid;
   OLV_TRY(host_transpose(g->n, g->m, h_rp, h_col, irp, icol, ieid));
@@ -220,6 +243,16 @@ static int attach_transpose(olv_graph *g, const uint32_t *h_rp, const uint32_t *
   OLV_TRY(upload(&g->in_row_ptr, irp.data(), (size_t)g->n + 1));
   OLV_TRY(upload(&g->in_col, icol.data(), g->m));
   OLV_TRY(upload(&g->in_eid, ieid.data(), g->m));
+  // First two in-neighbours of every vertex (0xffffffff where absent): a
+  // bottom-up step finds most parents there with one coalesced load.
+  std::vector<uint2> head(std::max<uint32_t>(g->n, 1), make_uint2(0xffffffffu, 0xffffffffu));
+  for (uint32_t v = 0; v < g->n; ++v) {
+    const uint32_t d = irp[v + 1] - irp[v];
+    if (d > 0) head[v].x = icol[irp[v]];
+    if (d > 1) head[v].y = icol[irp[v] + 1];
+  }
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&g->in_head), head.size() * sizeof(uint2)));
+  OLV_HIP(hipMemcpy(g->in_head, head.data(), head.size() * sizeof(uint2), hipMemcpyHostToDevice));
   // Vertices without in-edges: only a seed can reach them.
   const size_t nw = ((size_t)g->n + 31) / 32;
   std::vector<uint32_t> no_in(std::max<size_t>(nw, 1), 0u);
@@ -301,8 +334,8 @@ static void graph_free(olv_graph *g) {
   free_pr_work(g);
   g->plan_in.release();
   g->plan_out.release();
-  void *ps[] = {g->row_ptr, g->col, g->weights, g->in_row_ptr, g->in_col, g->in_eid, g->no_in_bm, g->inv_outdeg,
-                g->union_map};
+  void *ps[] = {g->row_ptr, g->col, g->weights, g->in_row_ptr, g->in_col, g->in_eid, g->no_in_bm, g->in_head,
+                g->inv_outdeg, g->union_map};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete g;

@@ -83,9 +83,14 @@ def test_multigraph_union(olv):
     arr_g = (ctypes.c_void_p * len(gs))(*[g.h.value for g in gs])
     arr_o = (ctypes.c_void_p * len(outs))(*[o.ptr for o in outs])
     st = _lib.Stats()
-    _lib.check(_lib.lib().olv_bfs_multigraph(arr_g, len(gs), srcs.ctypes.data, arr_o, 0, ctypes.byref(st)))
-    for c, s, o in zip(csrs, srcs, outs):
-        assert np.array_equal(o.numpy(), oracle.bfs(c.row_ptr, c.col, int(s)))
+    # direction-optimising, and bottom-up on every level after the first: the
+    # union's in-CSR (and its in-list heads) is assembled on the device
+    for flags in (0, _lib.OLV_FORCE_BOTTOM_UP):
+        for o in outs:
+            o.fill(0x5A5A5A5A)
+        _lib.check(_lib.lib().olv_bfs_multigraph(arr_g, len(gs), srcs.ctypes.data, arr_o, flags, ctypes.byref(st)))
+        for c, s, o in zip(csrs, srcs, outs):
+            assert np.array_equal(o.numpy(), oracle.bfs(c.row_ptr, c.col, int(s))), flags
 
 
 def test_plans_replay(olv):
