@@ -54,8 +54,8 @@ struct BfsCtl {
 
 struct BfsWork {
   uint32_t n = 0, nw = 0, nwp = 0;  // vertices, bitmap words, padded words
-  uint32_t *visited = nullptr;
-  uint32_t *snap = nullptr;         // visited as of the last frontier extraction
+  uint8_t *visited = nullptr;       // one byte per vertex, nwp * 32 bytes
+  uint32_t *snap = nullptr;         // claims as of the last frontier extraction (bitmap)
   uint32_t *bms = nullptr;          // 2 frontier bitmaps of nwp words
   uint32_t *queue = nullptr;
   uint2 *hubs = nullptr;
@@ -86,7 +86,7 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
     w->n = g->n;
     w->nw = (g->n + 31) / 32;
     w->nwp = (w->nw + 63) / 64 * 64;
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 4));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 32));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->snap), (size_t)w->nwp * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 2));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->queue), (size_t)g->n * 4));
@@ -101,11 +101,11 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
 }
 
 // ------------------------------------------------------------- kernels
-// levels = INF, bitmaps/control/barrier = 0, visited = snap = the vertices
-// without in-edges when bottom-up steps may run (no parent can claim them,
-// so bottom-up steps skip them; bfs_seed_kernel clears the seeds' bits),
-// else 0.
-__global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint32_t *__restrict__ visited,
+// levels = INF, bitmaps/control/barrier = 0; the vertices without in-edges
+// are pre-claimed (visited byte and snapshot bit) when bottom-up steps may
+// run -- no parent can claim them, so bottom-up steps skip them;
+// bfs_seed_kernel releases the seeds among them -- else nothing is.
+__global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint8_t *__restrict__ visited,
                                  uint32_t *__restrict__ snap, const uint32_t *__restrict__ no_in,
                                  uint32_t *__restrict__ bms, uint32_t nwp, BfsCtl *__restrict__ ctl,
                                  GridBar *__restrict__ bar) {
@@ -113,7 +113,19 @@ __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint3
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t nw = ((size_t)n + 31) / 32;
   for (size_t i = tid; i < n; i += stride) levels[i] = kInf;
-  for (size_t i = tid; i < nwp; i += stride) visited[i] = snap[i] = (no_in && i < nw) ? no_in[i] : 0u;
+  for (size_t i = tid; i < nwp; i += stride) {
+    const uint32_t d = (no_in && i < nw) ? no_in[i] : 0u;
+    snap[i] = d;
+    uint32_t b[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const uint32_t nib = (d >> (4 * k)) & 0xfu;
+      b[k] = (nib & 1u) | ((nib & 2u) << 7) | ((nib & 4u) << 14) | ((nib & 8u) << 21);
+    }
+    uint4 *o = reinterpret_cast<uint4 *>(visited + i * 32);
+    o[0] = make_uint4(b[0], b[1], b[2], b[3]);
+    o[1] = make_uint4(b[4], b[5], b[6], b[7]);
+  }
   for (size_t i = tid; i < (size_t)nwp * 2; i += stride) bms[i] = 0u;
   for (size_t i = tid; i < sizeof(BfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
   for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
@@ -124,14 +136,16 @@ __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, 
                                 BfsCtl *__restrict__ ctl) {
   for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
     const uint32_t s = seeds[i];
-    atomicAnd(&a.visited[s >> 5], ~(1u << (s & 31)));
+    atomicAnd(&a.snap[s >> 5], ~(1u << (s & 31)));
   }
   __syncthreads();
+  // the snapshot bit decides between duplicate seeds (level 0 is not a next
+  // frontier, so seeds stay in the snapshot)
   auto claim = [&](uint32_t i) -> uint32_t {
     const uint32_t s = seeds[i];
-    const uint32_t old = atomicOr(&a.visited[s >> 5], 1u << (s & 31));
+    const uint32_t old = atomicOr(&a.snap[s >> 5], 1u << (s & 31));
     if (old & (1u << (s & 31))) return 0xffffffffu;
-    atomicOr(&a.snap[s >> 5], 1u << (s & 31));  // level 0 is not a next frontier
+    a.visited[s] = 1;
     a.levels[s] = 0;
     return s;
   };

@@ -80,8 +80,8 @@ struct BfsArgs {
   const uint32_t *in_col;
   const uint2 *in_head;  // first two in-neighbours (0xffffffff where absent)
   int32_t *levels;
-  uint32_t *visited;
-  uint32_t *snap;      // visited as of the last frontier extraction
+  uint8_t *visited;    // one byte per vertex: nonzero = claimed (padded to 32-vertex words)
+  uint32_t *snap;      // bitmap of the claims as of the last frontier extraction
   uint32_t *queue;
   uint2 *hubs;
   uint32_t n;
@@ -233,12 +233,14 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
   }
 }
 
-// BFS claim: a visitor that finds the vertex unvisited sets its visited bit
-// (a non-returning atomic: nothing waits for it) and its level.  Two
-// visitors may both see the bit clear; both store the same level, and the
-// level counters then count the vertex twice -- they only steer the
-// direction choice.  The next frontier is not recorded here: it is the set
-// of visited bits that are new since the last extraction (frontier_words).
+// BFS claim: a visitor that finds the vertex unvisited stores its visited
+// byte and its level -- plain byte stores, no atomics (32 neighbouring
+// vertices sharing a bitmap word made the claims serialise on the hot words
+// of an RMAT graph).  Two visitors may both find the byte clear; both store
+// the same values, and the level counters then count the vertex twice --
+// they only steer the direction choice.  The next frontier is not recorded
+// here: it is the set of claims new since the last extraction
+// (frontier_words).
 struct BfsVisitor {
   const BfsArgs &a;
   int32_t nlevel;
@@ -248,11 +250,10 @@ struct BfsVisitor {
   __device__ __forceinline__ uint32_t row(uint32_t q) const { return q; }
   __device__ __forceinline__ uint32_t load(uint32_t) const { return 0u; }
   __device__ __forceinline__ uint32_t tgt(uint32_t e, uint32_t) const { return a.col[e]; }
-  __device__ __forceinline__ uint32_t state(uint32_t v) const { return a.visited[v >> 5]; }
-  __device__ __forceinline__ void apply(uint32_t v, uint32_t word, uint32_t) {
-    const uint32_t bit = 1u << (v & 31);
-    if (word & bit) return;
-    __hip_atomic_fetch_or(&a.visited[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __device__ __forceinline__ uint32_t state(uint32_t v) const { return a.visited[v]; }
+  __device__ __forceinline__ void apply(uint32_t v, uint32_t seen, uint32_t) {
+    if (seen) return;
+    a.visited[v] = 1;
     a.levels[v] = nlevel;
     nf += 1;
     mf += a.row_ptr[v + 1] - a.row_ptr[v];
@@ -269,9 +270,7 @@ __device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_
 }
 
 // Bottom-up step: every unvisited vertex scans its in-list for a parent in
-// the current frontier bitmap and stops at the first one.  A wave owns 64
-// consecutive vertices (two bitmap words) per slot, so visited words are
-// written without atomics.  Each lane carries kBuBatch vertices (grid-stride
+// the current frontier bitmap and stops at the first one.  Each lane carries kBuBatch vertices (grid-stride
 // slots) at once: their state, row and in-list loads are issued together, so
 // the chain of dependent memory latencies is paid once per kBuBatch slots.
 // A lane scans at most kBuLaneSteps batches of kVisitUnroll in-edges per
@@ -295,15 +294,14 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
   const uint64_t stride = (uint64_t)nvb * (BLOCK / kWave) * kWave;
   const uint32_t gw = vb * (BLOCK / kWave) + threadIdx.x / kWave;
   for (uint64_t base = (uint64_t)gw * kWave; base < a.n; base += J * stride) {
-    uint32_t vis[J], e[J], ee[J];
+    uint32_t e[J], ee[J];
     bool todo[J], found[J];
     bool any_todo = false;
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const uint64_t v = base + j * stride + lane;
       const bool inb = v < a.n;
-      vis[j] = inb ? a.visited[v >> 5] : 0xffffffffu;
-      todo[j] = inb && !((vis[j] >> (v & 31)) & 1u);
+      todo[j] = inb && a.visited[v] == 0;
       any_todo |= todo[j];
       found[j] = false;
       e[j] = ee[j] = 0;
@@ -390,14 +388,12 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
 #pragma unroll
     for (int j = 0; j < J; ++j) {
       const uint32_t v = (uint32_t)(base + j * stride) + lane;
-      const uint64_t fm = ballot(found[j]);
       if (found[j]) {
+        a.visited[v] = 1;
         a.levels[v] = nlevel;
         nf += 1;
         mf += a.row_ptr[v + 1] - a.row_ptr[v];
       }
-      const uint32_t mine = lane < 32 ? (uint32_t)fm : (uint32_t)(fm >> 32);
-      if ((lane == 0 || lane == 32) && mine) a.visited[v >> 5] = vis[j] | mine;
     }
   }
 }
@@ -466,9 +462,18 @@ template <int BLOCK>
 __device__ void frontier_words(const BfsArgs &a, uint32_t *front, uint32_t nw, bool to_queue, uint32_t *qcount,
                                uint32_t *hcount, uint32_t vb, uint32_t nvb) {
   auto word_of = [&](uint32_t w) -> uint32_t {
-    const uint32_t v = a.visited[w], s = a.snap[w];
+    // the 32 visited bytes of word w (32-byte aligned: two 16-byte loads)
+    const uint4 *p = reinterpret_cast<const uint4 *>(a.visited + (size_t)w * 32u);
+    const uint4 q0 = p[0], q1 = p[1];
+    const uint32_t parts[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int b = 0; b < 4; ++b) v |= ((parts[i] >> (8 * b)) & 0xffu ? 1u : 0u) << (4 * i + b);
+    const uint32_t s = a.snap[w];
     const uint32_t nb = v & ~s;
-    if (nb) a.snap[w] = v;
+    if (nb) a.snap[w] = s | nb;
     front[w] = nb;
     return nb;
   };
