@@ -277,10 +277,10 @@ __device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_
 // vertex on its own; in-lists left unfinished are then scanned by the whole
 // wave, 64 x kVisitUnroll edges per step, one vertex at a time.
 #ifndef OLV_BU_LANE_STEPS
-#define OLV_BU_LANE_STEPS 2
+#define OLV_BU_LANE_STEPS 4
 #endif
 #ifndef OLV_BU_BATCH
-#define OLV_BU_BATCH 4
+#define OLV_BU_BATCH 1
 #endif
 constexpr int kBuLaneSteps = OLV_BU_LANE_STEPS;
 constexpr int kBuBatch = OLV_BU_BATCH;
