@@ -30,6 +30,7 @@ struct SegPlan {
   uint32_t nhub = 0;
   uint32_t nchunks = 0;
   uint32_t chunk = 0;
+  uint32_t nempty = 0;              // rows without entries, at the end of bucket 0
   uint32_t *hub_vertex = nullptr;   // nhub
   uint32_t *hub_first = nullptr;    // nhub + 1 (chunk slot ranges)
   uint32_t *chunk_hub = nullptr;    // nchunks: owning hub index

@@ -294,14 +294,18 @@ int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan) {
   plan.bounds[3] = plan.chunk;
   std::vector<uint32_t> lists[SegPlan::kBuckets];
   std::vector<uint32_t> hubs;
+  std::vector<uint32_t> empty_rows;  // listed last in bucket 0: iterations may skip them
   for (uint32_t v = 0; v < n; ++v) {
     uint32_t d = rp[v + 1] - rp[v];
-    if (d <= plan.bounds[0]) lists[0].push_back(v);
+    if (d == 0) empty_rows.push_back(v);
+    else if (d <= plan.bounds[0]) lists[0].push_back(v);
     else if (d <= plan.bounds[1]) lists[1].push_back(v);
     else if (d <= plan.bounds[2]) lists[2].push_back(v);
     else if (d <= plan.bounds[3]) lists[3].push_back(v);
     else hubs.push_back(v);
   }
+  plan.nempty = (uint32_t)empty_rows.size();
+  lists[0].insert(lists[0].end(), empty_rows.begin(), empty_rows.end());
   std::vector<uint32_t> all;
   all.reserve(n);
   plan.offsets[0] = 0;

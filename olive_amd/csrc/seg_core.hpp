@@ -33,6 +33,9 @@ struct SegDev {
   // first virtual block of each role: [0]=hub chunks, [1]=bucket3,
   // [2]=bucket2, [3]=bucket1, [4]=bucket0, [5]=end
   uint32_t role_start[6];
+  // end of the virtual blocks that hold a row with entries: the trailing
+  // bucket-0 blocks hold only empty rows
+  uint32_t live_end;
 };
 
 constexpr int kSegBlock = 256;
@@ -61,6 +64,8 @@ inline SegDev make_seg_dev(const SegPlan &p) {
   d.role_start[3] = d.role_start[2] + (c2 + kSegBlock / 64 - 1) / (kSegBlock / 64);
   d.role_start[4] = d.role_start[3] + (c1 + kSegBlock / 8 - 1) / (kSegBlock / 8);
   d.role_start[5] = d.role_start[4] + (c0 + kSegBlock - 1) / kSegBlock;
+  const uint32_t p0 = c0 - p.nempty;  // first empty row within bucket 0
+  d.live_end = d.role_start[4] + (p0 + kSegBlock - 1) / kSegBlock;
   return d;
 }
 
@@ -185,7 +190,10 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
   }
   __shared__ float s_red[kSegBlock / kWave];
   float err = 0.f;
-  const uint32_t nvb = d.role_start[5];
+  // An empty row's update is a constant: iterations 0 and 1 write it into
+  // both ping-pong buffers, later iterations skip those rows (their
+  // residual is 0).  An edge pass (ctl == nullptr) has nothing to do there.
+  const uint32_t nvb = (!ctl || k >= 2) ? d.live_end : d.role_start[5];
   for (uint32_t b = blockIdx.x; b < nvb; b += gridDim.x) {
     if (b < d.role_start[1]) {
       // hub chunk: partial sum; the last chunk of the row to finish
