@@ -30,6 +30,14 @@ namespace olv {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kSpinLimit = 1u << 24;
+// Workgroups per CU of the per-level kernels (8: full occupancy; 4 and 16
+// measured slower for expansion, compaction insensitive).
+#ifndef OLV_BFS_EXPAND_PER_CU
+#define OLV_BFS_EXPAND_PER_CU 8
+#endif
+#ifndef OLV_BFS_COMPACT_PER_CU
+#define OLV_BFS_COMPACT_PER_CU 8
+#endif
 
 struct BfsState {              // per level parity
   unsigned long long mu;       // unexplored out-edges before this level
@@ -359,8 +367,8 @@ int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, ui
 int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count) {
   const BfsArgs a = make_args(g, w, levels);
   const BfsDir dir = make_dir(g, w, can_bu);
-  const unsigned ge = (unsigned)cu_count() * 8;
-  const unsigned gc = grid_for(w->nw, kBlock, 8);
+  const unsigned ge = (unsigned)cu_count() * OLV_BFS_EXPAND_PER_CU;
+  const unsigned gc = grid_for(w->nw, kBlock, OLV_BFS_COMPACT_PER_CU);
   for (uint32_t i = 0; i < count; ++i) {
     hipLaunchKernelGGL(bfs_expand_kernel, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);
     hipLaunchKernelGGL(bfs_compact_kernel, dim3(gc), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nw, w->nwp,
