@@ -204,6 +204,8 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *c
 // next step is top-down).  Workgroup 0 publishes the next level's state.
 __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nw,
                                                              uint32_t nwp, unsigned long long m, BfsDir dir) {
+  const uint32_t w0 = blockIdx.x * kBlock + threadIdx.x;
+  const WordLoad pre = w0 < nw ? load_word(a, w0) : WordLoad{};
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t L = ctl->level_e;
   const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
@@ -232,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *
     ctl->level = L + 1;
   }
   frontier_words<kBlock>(a, bms + (size_t)((L + 1) & 1) * nwp, nw, !next_bu, &ctl->qfill[(L + 1) & 1],
-                         &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x);
+                         &ctl->hfill[(L + 1) & 1], blockIdx.x, gridDim.x, pre);
 }
 
 // All levels in one cooperative launch with the XCD-aware grid barrier
@@ -284,8 +286,9 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
     st.mu = mu;
     st.mode = next_bu ? 1u : 0u;
     st.nbu = nbu;
+    const uint32_t wp = vb * kBlock + threadIdx.x;
     frontier_words<kBlock>(a, bms + (size_t)((L + 1) & 1) * nwp, nw, !next_bu, &ctl->qfill[(L + 1) & 1],
-                           &ctl->hfill[(L + 1) & 1], vb, nvb);
+                           &ctl->hfill[(L + 1) & 1], vb, nvb, wp < nw ? load_word(a, wp) : WordLoad{});
     if (!grid_barrier(bar, gb, kSpinLimit)) break;
     ++L;
   }

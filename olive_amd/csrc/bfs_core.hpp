@@ -455,25 +455,36 @@ struct IdRow {
   __device__ __forceinline__ uint32_t operator()(uint32_t v) const { return v; }
 };
 
-// Next frontier = visited bits set since the last extraction: written to
+// The visited bytes and snapshot word of one 32-vertex word, loaded ahead
+// of the control-state reads they do not depend on.
+struct WordLoad {
+  uint4 q0, q1;
+  uint32_t snap;
+};
+
+__device__ __forceinline__ WordLoad load_word(const BfsArgs &a, uint32_t w) {
+  const uint4 *p = reinterpret_cast<const uint4 *>(a.visited + (size_t)w * 32u);  // 32-byte aligned
+  return WordLoad{p[0], p[1], a.snap[w]};
+}
+
+// Next frontier = visited bytes set since the last extraction: written to
 // `front` (for a bottom-up step) and, if `to_queue`, compacted into the
-// queue and hub items (for a top-down step); `snap` catches up.
+// queue and hub items (for a top-down step); `snap` catches up.  `pre` is
+// this lane's first word (index vb * BLOCK + threadIdx.x), already loaded.
 template <int BLOCK>
 __device__ void frontier_words(const BfsArgs &a, uint32_t *front, uint32_t nw, bool to_queue, uint32_t *qcount,
-                               uint32_t *hcount, uint32_t vb, uint32_t nvb) {
+                               uint32_t *hcount, uint32_t vb, uint32_t nvb, const WordLoad &pre) {
+  const uint32_t wpre = vb * BLOCK + threadIdx.x;
   auto word_of = [&](uint32_t w) -> uint32_t {
-    // the 32 visited bytes of word w (32-byte aligned: two 16-byte loads)
-    const uint4 *p = reinterpret_cast<const uint4 *>(a.visited + (size_t)w * 32u);
-    const uint4 q0 = p[0], q1 = p[1];
-    const uint32_t parts[8] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y, q1.z, q1.w};
+    const WordLoad x = w == wpre ? pre : load_word(a, w);
+    const uint32_t parts[8] = {x.q0.x, x.q0.y, x.q0.z, x.q0.w, x.q1.x, x.q1.y, x.q1.z, x.q1.w};
     uint32_t v = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i)
 #pragma unroll
       for (int b = 0; b < 4; ++b) v |= ((parts[i] >> (8 * b)) & 0xffu ? 1u : 0u) << (4 * i + b);
-    const uint32_t s = a.snap[w];
-    const uint32_t nb = v & ~s;
-    if (nb) a.snap[w] = s | nb;
+    const uint32_t nb = v & ~x.snap;
+    if (nb) a.snap[w] = x.snap | nb;
     front[w] = nb;
     return nb;
   };
