@@ -183,13 +183,23 @@ __device__ __forceinline__ void or_mask(uint32_t *acc, const uint32_t *Ff, uint3
   acc[7] |= b.w;
 }
 
-// OR of F[u] over active in-neighbours k = beg + j, beg + j + G, ...
+// OR of F[u] over active in-neighbours k = beg + j, beg + j + G, ...;
+// kVisitUnroll in-neighbours per step: their index loads, then their
+// activity probes, then their mask loads are issued together.
 template <int G>
 __device__ __forceinline__ void lane_or(const MbfsParams &p, const uint32_t *Af, const uint32_t *Ff, uint32_t beg,
                                         uint32_t end, uint32_t j, uint32_t *acc) {
-  for (uint32_t k = beg + j; k < end; k += G) {
-    const uint32_t u = p.in_col[k];
-    if (bm_test(Af, u)) or_mask(acc, Ff, u);
+  constexpr int U = kVisitUnroll;
+  for (uint64_t k = (uint64_t)beg + j; k < end; k += (uint64_t)U * G) {
+    uint32_t u[U];
+    bool act[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) u[i] = k + (uint64_t)i * G < end ? p.in_col[k + (uint64_t)i * G] : 0xffffffffu;
+#pragma unroll
+    for (int i = 0; i < U; ++i) act[i] = u[i] != 0xffffffffu && bm_test(Af, u[i]);
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (act[i]) or_mask(acc, Ff, u[i]);
   }
 }
 
@@ -319,12 +329,22 @@ __global__ void __launch_bounds__(kMB) mbfs_pull_kernel(MbfsParams p, SegDev d) 
       uint32_t vv[kMW];
 #pragma unroll
       for (int j = 0; j < kMW; ++j) vv[j] = p.V[(size_t)v * kMW + j];
+      // kVisitUnroll in-neighbours per step, stopping after the step that
+      // completes the mask
+      constexpr int U = kVisitUnroll;
       uint32_t k = 0;
-      for (uint32_t e = rp[v]; e < rp[v + 1]; ++e) {
-        ++k;
-        const uint32_t u = p.in_col[e];
-        if (!bm_test(Af, u)) continue;
-        or_mask(acc, Ff, u);
+      const uint32_t eb = rp[v], ee = rp[v + 1];
+      for (uint32_t e = eb; e < ee; e += U) {
+        uint32_t u[U];
+        bool act[U];
+#pragma unroll
+        for (int i = 0; i < U; ++i) u[i] = e + i < ee ? p.in_col[e + i] : 0xffffffffu;
+#pragma unroll
+        for (int i = 0; i < U; ++i) act[i] = u[i] != 0xffffffffu && bm_test(Af, u[i]);
+#pragma unroll
+        for (int i = 0; i < U; ++i)
+          if (act[i]) or_mask(acc, Ff, u[i]);
+        k += ee - e < (uint32_t)U ? ee - e : (uint32_t)U;
         bool full = true;
 #pragma unroll
         for (int j = 0; j < kMW; ++j) full = full && ((acc[j] | vv[j]) == p.full[j]);
