@@ -66,7 +66,9 @@ struct MbfsParams {
   uint32_t *D;     // nwp      vertices that have every source
   uint32_t *queue; // n
   uint2 *hubs;     // hub items
+  uint8_t *S;      // n * sstride  level byte per (vertex, source)
   int32_t *levels; // k * n
+  uint32_t sstride;
   MbfsCtl *ctl;
   unsigned long long m;
   uint32_t wave_thr, block_thr, hub_thr;
@@ -120,6 +122,46 @@ __device__ __forceinline__ bool mbfs_apply(const MbfsParams &p, uint32_t v, cons
     full = full && ((vv[j] | nwv[j]) == p.full[j]);
   }
   return full;
+}
+
+// Level bytes: S holds one byte per (vertex, source of the group), the
+// level at which the source reached the vertex; it is valid where V has
+// the source's bit (the bytes are never cleared).  mbfs_levels_kernel turns
+// it into the source-major int32 output with coalesced stores.  Levels of
+// 255 and deeper are stored as 0xFF and written to the output directly.
+__device__ __forceinline__ uint32_t byte_mask4(uint32_t nib) {
+  return ((nib * 0x00204081u) & 0x01010101u) * 0xFFu;  // bit i of nib -> byte i = 0xFF
+}
+
+// Record level `nlevel` for the sources in nwv at v: read-modify-write of
+// the 32-byte groups of v's row that hold a new source.
+__device__ __forceinline__ void mbfs_record(const MbfsParams &p, uint32_t v, const uint32_t *nwv, int32_t nlevel) {
+  const uint32_t l4 = (nlevel < 255 ? (uint32_t)nlevel : 0xFFu) * 0x01010101u;
+  uint4 *row = reinterpret_cast<uint4 *>(p.S + (size_t)v * p.sstride);
+  auto mix = [l4](uint32_t o, uint32_t nib) {
+    const uint32_t m = byte_mask4(nib & 15u);
+    return (o & ~m) | (l4 & m);
+  };
+#pragma unroll
+  for (int j = 0; j < kMW; ++j) {
+    const uint32_t x = nwv[j];
+    if (!x) continue;
+    uint4 a = row[2 * j], b = row[2 * j + 1];
+    a.x = mix(a.x, x);
+    a.y = mix(a.y, x >> 4);
+    a.z = mix(a.z, x >> 8);
+    a.w = mix(a.w, x >> 12);
+    b.x = mix(b.x, x >> 16);
+    b.y = mix(b.y, x >> 20);
+    b.z = mix(b.z, x >> 24);
+    b.w = mix(b.w, x >> 28);
+    row[2 * j] = a;
+    row[2 * j + 1] = b;
+  }
+  if (nlevel >= 255)
+#pragma unroll
+    for (int j = 0; j < kMW; ++j)
+      for (uint32_t x = nwv[j]; x; x &= x - 1) p.levels[(size_t)(j * 32 + __ffs(x) - 1) * p.n + v] = nlevel;
 }
 
 __device__ __forceinline__ void mbfs_buffers(const MbfsParams &p, uint32_t L, const uint32_t *&Ff, uint32_t *&Fn,
@@ -244,9 +286,7 @@ __device__ __forceinline__ void mbfs_pull_finish(const MbfsParams &p, uint32_t v
   atomicOr(&An[v >> 5], bit);
   nf += 1;
   mf += p.row_ptr[v + 1] - p.row_ptr[v];
-#pragma unroll
-  for (int j = 0; j < kMW; ++j)
-    for (uint32_t x = nwv[j]; x; x &= x - 1) p.levels[(size_t)(j * 32 + __ffs(x) - 1) * p.n + v] = nlevel;
+  mbfs_record(p, v, nwv, nlevel);
 }
 
 // Pull level: every vertex not yet holding every source ORs the frontier
@@ -392,15 +432,7 @@ __global__ void __launch_bounds__(kMB) mbfs_contract_kernel(MbfsParams p) {
       anext |= 1u << b;
       nf += 1;
       mf += p.row_ptr[v + 1] - p.row_ptr[v];
-#pragma unroll
-      for (int j = 0; j < kMW; ++j) {
-        uint32_t x = nwv[j];
-        while (x) {
-          const uint32_t s = j * 32 + (__ffs(x) - 1);
-          x &= x - 1;
-          p.levels[(size_t)s * p.n + v] = nlevel;
-        }
-      }
+      mbfs_record(p, v, nwv, nlevel);
     }
     // |=: on a pull level the pull kernel has already set bits of this word
     if (anext) An[w] |= anext;
@@ -470,7 +502,7 @@ __global__ void mbfs_seed_kernel(MbfsParams p, const uint32_t *__restrict__ srcs
     const uint32_t v = srcs[s];
     atomicOr(&p.V[(size_t)v * kMW + s / 32], 1u << (s % 32));
     atomicOr(&F0[(size_t)v * kMW + s / 32], 1u << (s % 32));
-    p.levels[(size_t)s * p.n + v] = 0;
+    p.S[(size_t)v * p.sstride + s] = 0;  // level 0; distinct sources own distinct bytes
   }
   __syncthreads();
   // distinct source vertices form the level-0 frontier
@@ -495,34 +527,67 @@ __global__ void mbfs_seed_kernel(MbfsParams p, const uint32_t *__restrict__ srcs
   }
 }
 
-// Fill with 16-byte stores on the 16-byte-aligned body, scalar head/tail.
-__global__ void fill_i32_kernel(int32_t *__restrict__ p, int32_t v, size_t count) {
-  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * blockDim.x;
-  const uintptr_t addr = reinterpret_cast<uintptr_t>(p);
-  size_t head = ((16 - (addr & 15)) & 15) / 4;
-  if (head > count) head = count;
-  if (i < head) p[i] = v;
-  const size_t n4 = (count - head) / 4;
-  int4 *q = reinterpret_cast<int4 *>(p + head);
-  const int4 val = make_int4(v, v, v, v);
-  for (size_t j = i; j < n4; j += stride) q[j] = val;
-  for (size_t j = head + n4 * 4 + i; j < count; j += stride) p[j] = v;
+// Output for one group: levels[s * n + v] for s < k from the level bytes
+// (kInf where V lacks bit s).  A workgroup takes kLevTile consecutive
+// vertices: their rows of S are one contiguous block, staged in LDS (rows
+// padded by a dword so the per-source column reads are conflict-free);
+// each wave then stores kLevTile consecutive int32 of one source's output
+// row per step.
+constexpr int kLevTile = 64;
+
+__global__ void __launch_bounds__(kMB) mbfs_levels_kernel(MbfsParams p) {
+  __shared__ uint32_t s_row[kLevTile][kMW * 8 + 1];
+  __shared__ uint32_t s_vis[kLevTile][kMW + 1];
+  const uint32_t kw = p.sstride / 32;  // 32-source groups in a row
+  const uint32_t q4 = p.sstride / 16;  // uint4 per row
+  for (uint32_t v0 = blockIdx.x * kLevTile; v0 < p.n; v0 += gridDim.x * kLevTile) {
+    const uint32_t rows = min((uint32_t)kLevTile, p.n - v0);
+    const uint4 *src = reinterpret_cast<const uint4 *>(p.S + (size_t)v0 * p.sstride);
+    for (uint32_t q = threadIdx.x; q < rows * q4; q += kMB) {
+      const uint4 x = src[q];
+      uint32_t *d = &s_row[q / q4][(q % q4) * 4];
+      d[0] = x.x;
+      d[1] = x.y;
+      d[2] = x.z;
+      d[3] = x.w;
+    }
+    for (uint32_t q = threadIdx.x; q < rows * kw; q += kMB)
+      s_vis[q / kw][q % kw] = p.V[(size_t)(v0 + q / kw) * kMW + q % kw];
+    __syncthreads();
+    const uint32_t t = threadIdx.x % kLevTile;
+    if (t < rows) {
+      const uint32_t v = v0 + t;
+      for (uint32_t s = threadIdx.x / kLevTile; s < p.k; s += kMB / kLevTile) {
+        const bool seen = (s_vis[t][s >> 5] >> (s & 31)) & 1u;
+        const uint32_t b = (s_row[t][s >> 2] >> (8 * (s & 3))) & 0xFFu;
+        if (seen && b == 0xFFu) continue;  // level >= 255: already in the output
+        p.levels[(size_t)s * p.n + v] = seen ? (int32_t)b : kInf;
+      }
+    }
+    __syncthreads();
+  }
 }
+
+// Bytes per vertex row of S for groups of up to k sources.
+static inline uint32_t level_stride(uint32_t k) { return (std::min(k, 256u) + 31) / 32 * 32; }
 
 struct MbfsWork {
   uint32_t *V = nullptr, *F = nullptr, *N = nullptr, *A = nullptr, *T = nullptr, *D = nullptr, *queue = nullptr;
+  uint8_t *S = nullptr;
   uint2 *hubs = nullptr;
   MbfsCtl *ctl = nullptr, *h_ctl = nullptr;
-  uint32_t last_levels = 0;
+  uint32_t last_levels = 0, sstride = 0;
   ~MbfsWork() {
-    void *ps[] = {V, F, N, A, T, D, queue, hubs, ctl};
+    void *ps[] = {V, F, N, A, T, D, queue, S, hubs, ctl};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     if (h_ctl) (void)hipHostFree(h_ctl);
   }
-  int alloc(uint32_t n, uint32_t nwp, uint64_t m) {
+  // k: the largest group (at most 256 sources); S rows hold ceil(k/32)*32 bytes
+  int alloc(uint32_t n, uint32_t nwp, uint64_t m, uint32_t k) {
     const size_t nm = (size_t)n * kMW * 4;
+    sstride = level_stride(k);
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&S), (size_t)n * sstride));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&hubs), hub_capacity(m) * sizeof(uint2)));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&V), nm));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&F), nm * 2));
@@ -566,7 +631,9 @@ static MbfsParams mbfs_params(const olv_graph *g, MbfsWork &w, uint32_t k, int32
   p.D = w.D;
   p.queue = w.queue;
   p.hubs = w.hubs;
+  p.S = w.S;
   p.levels = levels;
+  p.sstride = w.sstride;
   p.ctl = w.ctl;
   p.m = g->m;
   p.wave_thr = g->tune.td_wave_thr;
@@ -581,8 +648,6 @@ static MbfsParams mbfs_params(const olv_graph *g, MbfsWork &w, uint32_t k, int32
 static int mbfs_enqueue_init(const MbfsParams &p, const uint32_t *d_srcs) {
   hipStream_t s = stream();
   hipLaunchKernelGGL(mbfs_reset_kernel, dim3(mgrid((size_t)p.n * kMW, 256)), dim3(256), 0, s, p);
-  hipLaunchKernelGGL(fill_i32_kernel, dim3(mgrid((size_t)p.k * p.n / 4 + 1, 256)), dim3(256), 0, s, p.levels, kInf,
-                     (size_t)p.k * p.n);
   hipLaunchKernelGGL(mbfs_seed_kernel, dim3(1), dim3(256), 0, s, p, d_srcs);
   OLV_HIP(hipGetLastError());
   return OLV_OK;
@@ -599,6 +664,14 @@ static int mbfs_enqueue_levels(const MbfsParams &p, const SegDev &d, uint32_t co
     hipLaunchKernelGGL(mbfs_contract_kernel, dim3(gw), dim3(kMB), 0, s, p);
     hipLaunchKernelGGL(mbfs_compact_kernel, dim3(gw), dim3(kMB), 0, s, p);
   }
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
+static int mbfs_enqueue_finish(const MbfsParams &p) {
+  const size_t tiles = ((size_t)p.n + kLevTile - 1) / kLevTile;
+  hipLaunchKernelGGL(mbfs_levels_kernel, dim3((unsigned)std::min<size_t>(tiles, (size_t)cu_count() * 64)), dim3(kMB),
+                     0, stream(), p);
   OLV_HIP(hipGetLastError());
   return OLV_OK;
 }
@@ -637,7 +710,7 @@ static int mbfs_run_group(const MbfsParams &p, const SegDev &d, MbfsWork &w, con
   }
   if (w.h_ctl->overflow) return fail(OLV_EINTERNAL, "olv_bfs_batched: more than n levels");
   w.last_levels = w.h_ctl->levels;
-  return OLV_OK;
+  return mbfs_enqueue_finish(p);
 }
 
 }  // namespace olv
@@ -716,7 +789,7 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
   OLV_TRY(mbfs_pull_plan(g, flags, &d));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   MbfsWork w;
-  OLV_TRY(w.alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m));
+  OLV_TRY(w.alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m, std::min(k, 256u)));
   DevBuf d_srcs, d_out;
   OLV_TRY(d_srcs.alloc((size_t)k * 4));
   OLV_HIP(hipMemcpy(d_srcs.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice));
@@ -945,7 +1018,7 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
   p->k = k;
   p->levels = levels;
   p->mw = new MbfsWork();
-  int s = p->mw->alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m);
+  int s = p->mw->alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m, std::min(k, 256u));
   if (s == OLV_OK) s = p->seeds.alloc((size_t)k * 4);
   if (s == OLV_OK && hipMemcpy(p->seeds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
     s = fail(OLV_EHIP, "olv_plan_bfs_batched: seed copy failed");
@@ -963,6 +1036,7 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
       for (size_t i = 0; i < p->groups.size(); ++i) {
         OLV_TRY(mbfs_enqueue_init(p->groups[i], p->seeds.as<uint32_t>() + 256 * i));
         OLV_TRY(mbfs_enqueue_levels(p->groups[i], p->seg, depth[i]));
+        OLV_TRY(mbfs_enqueue_finish(p->groups[i]));
       }
       return OLV_OK;
     });

@@ -47,6 +47,21 @@ def test_bfs_batched_duplicates_and_grid(olv, flags):
         assert np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(s)))
 
 
+@pytest.mark.parametrize("flags", [0, 0x2])
+def test_bfs_batched_deep(olv, flags):
+    """Levels of 255 and more (a 3 x 300 grid is 301 levels deep from a
+    corner) next to shallow ones in the same rows."""
+    say(f"batched bfs deep grid flags={flags}")
+    csr = olv.gen_grid(3, 300)
+    g = olv.Graph(csr)
+    srcs = np.array([0, 899, 450, 299, 1, 0, 600], np.uint32)
+    out, st = g.bfs_batched(srcs, flags=flags)
+    got = out.numpy().reshape(len(srcs), csr.n)
+    for i, s in enumerate(srcs):
+        assert np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(s))), (i, s)
+    assert got.max() >= 300
+
+
 def test_bfs_batched_rmat18_256(olv):
     say("batched bfs rmat18 x 256")
     csr = olv.gen_rmat(18, 16, seed=1)
