@@ -56,6 +56,9 @@ enum {
 /* weight dtypes for olv_graph_create */
 enum { OLV_W_NONE = 0, OLV_W_I32 = 1, OLV_W_F32 = 2 };
 
+/* text formats for olv_graph_load_file */
+enum { OLV_FMT_EDGE_LIST = 0, OLV_FMT_DIMACS = 1 };
+
 typedef struct olv_graph olv_graph;
 
 /* Per-call statistics (all optional: pass NULL). */
@@ -139,6 +142,38 @@ int olv_graph_device_arrays(const olv_graph *g, const uint32_t **row_ptr,
 /* Copy the in-edge CSR to the host (for tests). */
 int olv_graph_transpose_to_host(const olv_graph *g, uint32_t *in_row_ptr,
                                 uint32_t *in_col, uint32_t *in_eid);
+
+/* ---- CSR ingest on the device (replaces CsrGraph::fromEdgeListFile
+ *      csrGraph.h:106-171, CsrGraph::fromDimacsFile csrGraph.h:177-236 and
+ *      the source-order clustering of edgeTuple.h:42-67) ------------------ */
+/* Edge list in any order -> graph.  The edges are stable-sorted by source
+ * on the device (edges of one source keep their input order); weights
+ * (weight_dtype, or OLV_W_NONE) follow their edges.  Ids must be < n.
+ * OLV_PTR_HOST: the arrays are host pointers; OLV_BUILD_TRANSPOSE: also
+ * build the in-edge CSR on the device (in-lists ordered by source). */
+int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m,
+                       const uint32_t *src, const uint32_t *dst,
+                       const void *weights, int weight_dtype, unsigned flags);
+/* Edge-list text -> graph, parsed on the device: '#' lines skipped, the
+ * first other non-empty line is "n m", then exactly m "src dst [w]" lines.
+ * Unlike csrGraph.h:106-171 the tuples need not be sorted (they are
+ * stable-sorted by source) and a third column on every edge line is kept
+ * as int32 weights (OLV_W_I32).  Errors name the 1-based line. */
+int olv_graph_from_edge_list_text(olv_graph **out, const char *text,
+                                  size_t len, unsigned flags);
+/* DIMACS adjacency text -> graph, parsed on the device: header "n m", then
+ * line i after it lists the 1-based neighbours of vertex i-1; exactly 2m
+ * neighbour entries (csrGraph.h:177-236). */
+int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len,
+                               unsigned flags);
+/* Read a file (OLV_FMT_EDGE_LIST or OLV_FMT_DIMACS) into pinned host memory
+ * and ingest it as above. */
+int olv_graph_load_file(olv_graph **out, const char *path, int format,
+                        unsigned flags);
+/* Copy the out-CSR to the host (weights if present; weight_dtype gets the
+ * graph's weight dtype).  Any pointer may be NULL. */
+int olv_graph_csr_to_host(const olv_graph *g, uint32_t *row_ptr,
+                          uint32_t *col, void *weights, int *weight_dtype);
 
 /* ---- algorithms -------------------------------------------------------- */
 /* BFS levels from src, INF = 0x7fffffff for unreached.  Replaces the BFS.cu

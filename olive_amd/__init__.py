@@ -16,14 +16,16 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FORCE_BOTTOM_UP, OLV_HOST_LOOP, OLV_NO_DIRECTION_OPT, OLV_PERSISTENT,
-                   OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE, HostCsr, OliveError, Stats, check, lib)
+from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FMT_DIMACS, OLV_FMT_EDGE_LIST, OLV_FORCE_BOTTOM_UP, OLV_HOST_LOOP,
+                   OLV_NO_DIRECTION_OPT, OLV_PERSISTENT, OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE,
+                   HostCsr, OliveError, Stats, check, lib)
 
 INF = 0x7FFFFFFF
 
 __all__ = [
     "INF", "OliveError", "Stats", "CsrGraph", "Graph", "DeviceBuffer", "init", "gen_grid", "gen_rmat",
     "gen_uniform", "gen_dirichlet", "batch_scale", "batch_seed", "load_edge_list", "load_dimacs",
+    "ingest_coo", "ingest_edge_list_text", "ingest_dimacs_text", "ingest_file", "format_edge_list",
     "OLV_HOST_LOOP", "OLV_NO_DIRECTION_OPT", "OLV_FORCE_BOTTOM_UP", "OLV_PTR_HOST",
 ]
 
@@ -195,6 +197,100 @@ def batch_seed(seed, i):
     return int(lib().olv_gen_batch_seed(seed, i))
 
 
+# ---------------------------------------------------------- device ingest
+def _ndigits(a):
+    nd = np.ones(a.shape, np.int64)
+    for k in range(1, 19):
+        nd += a >= 10 ** k
+    return nd
+
+
+def format_edge_list(n: int, src, dst, weights=None, comment: str | None = None) -> bytes:
+    """Edge-list text in the reference's format (csrGraph.h:106-171): an
+    optional '#' comment line, the "n m" header, then one "src dst [w]" line
+    per edge in the given order.  Vectorised, for inputs of many millions of
+    edges."""
+    cols = [np.asarray(src, np.int64), np.asarray(dst, np.int64)]
+    if weights is not None:
+        cols.append(np.asarray(weights, np.int64))
+    m = cols[0].size
+    head = ((f"# {comment}\n" if comment else "") + f"{n} {m}\n").encode()
+    neg = [c < 0 for c in cols]
+    mag = [np.abs(c) for c in cols]
+    nds = [_ndigits(a) for a in mag]
+    length = sum(nd + ng for nd, ng in zip(nds, neg)) + len(cols)  # separators and newline
+    starts = np.empty(m, np.int64)
+    if m:
+        starts[0] = 0
+        np.cumsum(length[:-1], out=starts[1:])
+    total = int(starts[-1] + length[-1]) if m else 0
+    out = np.empty(total, np.uint8)
+    pos = starts
+    for c, (ng, a, nd) in enumerate(zip(neg, mag, nds)):
+        out[pos[ng]] = ord("-")
+        pos = pos + ng
+        for k in range(int(nd.max()) if m else 0):
+            sel = k < nd
+            out[pos[sel] + nd[sel] - 1 - k] = ord("0") + (a[sel] // 10 ** k) % 10
+        pos = pos + nd
+        out[pos] = ord(" ") if c + 1 < len(cols) else ord("\n")
+        pos = pos + 1
+    return head + out.tobytes()
+
+
+def ingest_coo(n: int, src, dst, weights=None, transpose: bool = True) -> Graph:
+    """Edge list in any order -> resident graph, built on the device
+    (olv_graph_from_coo: stable sort by source, edges of a source keep their
+    order; weights follow their edges)."""
+    init()
+    s = np.ascontiguousarray(src, dtype=np.uint32)
+    d = np.ascontiguousarray(dst, dtype=np.uint32)
+    if s.size != d.size:
+        raise ValueError("src and dst differ in length")
+    wd, wp, w = OLV_W_NONE, None, None
+    if weights is not None:
+        w = np.ascontiguousarray(weights)
+        if w.dtype not in (np.int32, np.float32) or w.size != s.size:
+            raise ValueError("weights must be int32 or float32, one per edge")
+        wd = OLV_W_I32 if w.dtype == np.int32 else OLV_W_F32
+        wp = w.ctypes.data
+    h = ctypes.c_void_p()
+    flags = OLV_PTR_HOST | (OLV_BUILD_TRANSPOSE if transpose else 0)
+    check(lib().olv_graph_from_coo(ctypes.byref(h), n, s.size, s.ctypes.data if s.size else None,
+                                   d.ctypes.data if d.size else None, wp, wd, flags), "olv_graph_from_coo")
+    return Graph._adopt(h)
+
+
+def _ingest_text(fn, name, text, transpose):
+    init()
+    buf = text.encode() if isinstance(text, str) else bytes(text)
+    h = ctypes.c_void_p()
+    flags = OLV_PTR_HOST | (OLV_BUILD_TRANSPOSE if transpose else 0)
+    check(fn(ctypes.byref(h), buf, len(buf), flags), name)
+    return Graph._adopt(h)
+
+
+def ingest_edge_list_text(text, transpose: bool = True) -> Graph:
+    """Edge-list text (the format of load_edge_list) parsed on the device."""
+    return _ingest_text(lib().olv_graph_from_edge_list_text, "olv_graph_from_edge_list_text", text, transpose)
+
+
+def ingest_dimacs_text(text, transpose: bool = True) -> Graph:
+    """DIMACS adjacency text (the format of load_dimacs) parsed on the device."""
+    return _ingest_text(lib().olv_graph_from_dimacs_text, "olv_graph_from_dimacs_text", text, transpose)
+
+
+def ingest_file(path, fmt: str = "edge_list", transpose: bool = True) -> Graph:
+    """A file read into pinned memory and parsed on the device
+    (olv_graph_load_file); fmt is "edge_list" or "dimacs"."""
+    init()
+    f = {"edge_list": OLV_FMT_EDGE_LIST, "dimacs": OLV_FMT_DIMACS}[fmt]
+    h = ctypes.c_void_p()
+    check(lib().olv_graph_load_file(ctypes.byref(h), str(path).encode(), f,
+                                    OLV_BUILD_TRANSPOSE if transpose else 0), "olv_graph_load_file")
+    return Graph._adopt(h)
+
+
 # ---------------------------------------------------------------- loaders
 def load_edge_list(path) -> CsrGraph:
     """Edge-list file (csrGraph.h:106-171 format: '#' comments, ``n m``
@@ -277,6 +373,42 @@ class Graph:
         check(lib().olv_graph_create(ctypes.byref(h), csr.n, csr.m, csr.row_ptr.ctypes.data,
                                      _np_ptr(csr.col), _np_ptr(csr.weights), wd, flags), "olv_graph_create")
         self.h = h
+
+    @classmethod
+    def _adopt(cls, h, csr: CsrGraph | None = None):
+        """Wrap a handle made by the library (device ingest); the host CSR is
+        downloaded unless given."""
+        g = cls.__new__(cls)
+        g.h = h
+        n, m = ctypes.c_uint32(), ctypes.c_uint32()
+        check(lib().olv_graph_counts(h, ctypes.byref(n), ctypes.byref(m)), "olv_graph_counts")
+        g.n, g.m = n.value, m.value
+        g.csr = csr if csr is not None else g.to_host_csr()
+        return g
+
+    def to_host_csr(self) -> CsrGraph:
+        """The resident out-CSR (and weights) copied to the host."""
+        rp = np.empty(self.n + 1, np.uint32)
+        col = np.empty(max(self.m, 1), np.uint32)
+        w = np.empty(max(self.m, 1), np.uint32)
+        wd = ctypes.c_int()
+        check(lib().olv_graph_csr_to_host(self.h, rp.ctypes.data, col.ctypes.data, w.ctypes.data, ctypes.byref(wd)),
+              "olv_graph_csr_to_host")
+        wt = None
+        if wd.value == OLV_W_I32:
+            wt = w[:self.m].view(np.int32).copy()
+        elif wd.value == OLV_W_F32:
+            wt = w[:self.m].view(np.float32).copy()
+        return CsrGraph(rp, col[:self.m].copy(), wt)
+
+    def transpose_to_host(self):
+        """(in_row_ptr, in_col, in_eid) of the resident in-CSR."""
+        irp = np.empty(self.n + 1, np.uint32)
+        icol = np.empty(max(self.m, 1), np.uint32)
+        ieid = np.empty(max(self.m, 1), np.uint32)
+        check(lib().olv_graph_transpose_to_host(self.h, irp.ctypes.data, icol.ctypes.data, ieid.ctypes.data),
+              "olv_graph_transpose_to_host")
+        return irp, icol[:self.m].copy(), ieid[:self.m].copy()
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:

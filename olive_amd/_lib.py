@@ -27,6 +27,7 @@ OLV_PERSISTENT = 0x20
 OLV_BUILD_TRANSPOSE = 0x100
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
+OLV_FMT_EDGE_LIST, OLV_FMT_DIMACS = 0, 1
 
 
 class OliveError(RuntimeError):
@@ -105,6 +106,11 @@ SIGNATURES = {
     "olv_graph_counts": (_int, [_vp, _u32p, _u32p]),
     "olv_graph_device_arrays": (_int, [_vp] + [ctypes.POINTER(_vp)] * 6),
     "olv_graph_transpose_to_host": (_int, [_vp, _vp, _vp, _vp]),
+    "olv_graph_from_coo": (_int, [ctypes.POINTER(_vp), _u32, _u32, _vp, _vp, _vp, _int, _uint]),
+    "olv_graph_from_edge_list_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint]),
+    "olv_graph_from_dimacs_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint]),
+    "olv_graph_load_file": (_int, [ctypes.POINTER(_vp), ctypes.c_char_p, _int, _uint]),
+    "olv_graph_csr_to_host": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_int)]),
     "olv_bfs": (_int, [_vp, _u32, _vp, _uint, _statp]),
     "olv_bfs_multisource": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),
     "olv_sssp": (_int, [_vp, _u32, _vp, _vp, _uint, _statp]),

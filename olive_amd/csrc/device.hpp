@@ -65,6 +65,16 @@ __device__ __forceinline__ T wave_max(T v) {
   return v;
 }
 
+template <class T>
+__device__ __forceinline__ T wave_min(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    T x = __shfl_xor(v, o, kWave);
+    v = x < v ? x : v;
+  }
+  return v;
+}
+
 // Inclusive prefix sum over the wave.
 template <class T>
 __device__ __forceinline__ T wave_inclusive_scan(T v) {

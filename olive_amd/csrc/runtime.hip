@@ -269,6 +269,12 @@ __global__ void inv_outdeg_kernel(const uint32_t *__restrict__ rp, uint32_t n, f
   }
 }
 
+int compute_inv_outdeg(const uint32_t *rp, uint32_t n, float *inv) {
+  hipLaunchKernelGGL(inv_outdeg_kernel, dim3(fill_grid(n)), dim3(256), 0, g_stream, rp, n, inv);
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
 void SegPlan::release() {
   void *ps[] = {list, hub_vertex, hub_first, chunk_hub, chunk_partial, hub_done};
   for (void *p : ps)

@@ -196,3 +196,20 @@ def test_gpus_flag_other_than_one_exits(tmp_path):
                        text=True, timeout=120)
     assert r.returncode == 2
     assert "one GPU" in json.loads(r.stdout.strip().splitlines()[-1])["error"]
+
+
+def test_format_edge_list_round_trip(tmp_path):
+    """format_edge_list writes the reference's text format: the oracle's
+    loader reads back the same edges (signed weights included)."""
+    rng = np.random.default_rng(11)
+    n, m = 5000, 20000
+    src = rng.integers(0, n, m)
+    dst = rng.integers(0, n, m)
+    w = rng.integers(-2 ** 31, 2 ** 31, m)
+    p = tmp_path / "e.txt"
+    p.write_bytes(olive_amd.format_edge_list(n, src, dst, w, comment="round trip"))
+    nn, rp, col, ww = oracle.load_edge_list(str(p))
+    order = np.argsort(src, kind="stable")
+    assert nn == n and rp[-1] == m
+    assert np.array_equal(col, dst[order]) and np.array_equal(ww, w[order].astype(np.int32))
+    assert olive_amd.format_edge_list(3, [], []) == b"3 0\n"
