@@ -179,30 +179,58 @@ __global__ void __launch_bounds__(kIB) radix_count_kernel(const uint32_t *__rest
   }
 }
 
+// Value sources of the first pass (later passes read the previous output).
+struct LoadIndex {
+  __device__ uint32_t operator()(size_t k) const { return (uint32_t)k; }
+};
+struct LoadArr {
+  const uint32_t *a;
+  __device__ uint32_t operator()(size_t k) const { return a[k]; }
+};
+struct LoadPair {
+  const uint32_t *a, *b;
+  __device__ uint2 operator()(size_t k) const { return make_uint2(a[k], b[k]); }
+};
+struct LoadIndexAnd {
+  const uint32_t *b;
+  __device__ uint2 operator()(size_t k) const { return make_uint2((uint32_t)k, b[k]); }
+};
+template <class V>
+struct LoadPrev {
+  const V *a;
+  __device__ V operator()(size_t k) const { return a[k]; }
+};
+
 // Stable scatter of one pass.  Wave w of tile t owns items
 // t*kITile + w*kITile/kIWaves + i*64 + lane (i = 0..kIItems-1): in that order
 // each item's rank among equal digits comes from 8 ballots (lanes with the
 // same digit) plus the wave's running digit count in LDS; waves of a tile
-// follow each other, tiles follow the scanned digit-major offsets.
-// vin == nullptr: the values are the item indices.
-__global__ void __launch_bounds__(kIB) radix_scatter_kernel(const uint32_t *__restrict__ kin,
-                                                            const uint32_t *__restrict__ vin,
-                                                            uint32_t *__restrict__ kout, uint32_t *__restrict__ vout,
+// follow each other, tiles follow the scanned digit-major offsets.  The tile
+// is first reordered by digit in LDS, so the stores of each digit's run go
+// out as consecutive addresses.
+static_assert(kIB == kRDigits, "one lane per digit in the offset step");
+template <class V, class Load>
+__global__ void __launch_bounds__(kIB) radix_scatter_kernel(const uint32_t *__restrict__ kin, Load load,
+                                                            uint32_t *__restrict__ kout, V *__restrict__ vout,
                                                             uint32_t m, uint32_t shift, uint32_t ntiles,
                                                             const uint32_t *__restrict__ offs) {
   __shared__ uint32_t s_cnt[kIWaves][kRDigits];
+  __shared__ uint32_t s_delta[kRDigits];
+  __shared__ uint32_t s_key[kITile];
+  __shared__ V s_val[kITile];
   for (uint32_t i = threadIdx.x; i < (uint32_t)(kIWaves * kRDigits); i += kIB) (&s_cnt[0][0])[i] = 0u;
   __syncthreads();
   const uint32_t w = threadIdx.x / kWave, l = lane_id();
   const size_t base = (size_t)blockIdx.x * kITile + (size_t)w * (kITile / kIWaves);
   const uint64_t below_mask = (1ull << l) - 1ull;
-  uint32_t key[kIItems], val[kIItems], rank[kIItems];
+  uint32_t key[kIItems], rank[kIItems];
+  V val[kIItems];
 #pragma unroll
   for (int i = 0; i < kIItems; ++i) {
     const size_t k = base + (size_t)i * kWave + l;
     const bool ok = k < m;
     key[i] = ok ? kin[k] : 0u;
-    val[i] = ok ? (vin ? vin[k] : (uint32_t)k) : 0u;
+    val[i] = ok ? load(k) : V{};
     const uint32_t d = (key[i] >> shift) & (kRDigits - 1);
     uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -221,63 +249,96 @@ __global__ void __launch_bounds__(kIB) radix_scatter_kernel(const uint32_t *__re
     rank[i] = r;
   }
   __syncthreads();
-  for (uint32_t d = threadIdx.x; d < (uint32_t)kRDigits; d += kIB) {
-    uint32_t run = offs[(size_t)d * ntiles + blockIdx.x];
+  {
+    // lane d: tile-local start of digit d, each wave's start inside it, and
+    // the shift from tile position to output position
+    const uint32_t d = threadIdx.x;
+    uint32_t c[kIWaves], tot = 0;
 #pragma unroll
     for (int x = 0; x < kIWaves; ++x) {
-      const uint32_t c = s_cnt[x][d];
-      s_cnt[x][d] = run;
-      run += c;
+      c[x] = s_cnt[x][d];
+      tot += c[x];
     }
+    uint32_t all;
+    const uint32_t start = block_excl_sum(tot, all);
+    uint32_t run = start;
+#pragma unroll
+    for (int x = 0; x < kIWaves; ++x) {
+      s_cnt[x][d] = run;
+      run += c[x];
+    }
+    s_delta[d] = offs[(size_t)d * ntiles + blockIdx.x] - start;
   }
   __syncthreads();
 #pragma unroll
   for (int i = 0; i < kIItems; ++i) {
     const size_t k = base + (size_t)i * kWave + l;
     if (k < m) {
-      const uint32_t pos = s_cnt[w][(key[i] >> shift) & (kRDigits - 1)] + rank[i];
-      kout[pos] = key[i];
-      vout[pos] = val[i];
+      const uint32_t lp = s_cnt[w][(key[i] >> shift) & (kRDigits - 1)] + rank[i];
+      s_key[lp] = key[i];
+      s_val[lp] = val[i];
     }
+  }
+  __syncthreads();
+  const uint32_t cnt = (uint32_t)min((size_t)kITile, (size_t)m - (size_t)blockIdx.x * kITile);
+  for (uint32_t j = threadIdx.x; j < cnt; j += kIB) {
+    const uint32_t k = s_key[j];
+    const uint32_t pos = s_delta[(k >> shift) & (kRDigits - 1)] + j;
+    kout[pos] = k;
+    vout[pos] = s_val[j];
   }
 }
 
 struct SortBufs {
   DevBuf k[2], v[2], counts;
+  int cur = 0;
+  uint32_t *keys() const { return k[cur].as<uint32_t>(); }
+  template <class V>
+  V *vals() const { return v[cur].as<V>(); }
 };
 
-// Stable sort of the item indices 0..m-1 by keys_in (keys < 2^key_bits).
-// keys_in is left unchanged; *sorted_keys and *perm point into b.
-int radix_sort_index(const uint32_t *keys_in, uint32_t m, uint32_t key_bits, SortBufs &b, uint32_t **sorted_keys,
-                     uint32_t **perm) {
+// Stable sort of items 0..m-1 by keys_in (keys < 2^key_bits), carrying a
+// value of type V read by load0(k) in the first pass.  keys_in is left
+// unchanged; the result is b.keys() / b.vals<V>().
+template <class V, class Load0>
+int radix_sort(const uint32_t *keys_in, Load0 load0, uint32_t m, uint32_t key_bits, SortBufs &b) {
   const uint32_t ntiles = (uint32_t)(((size_t)m + kITile - 1) / kITile);
   for (int i = 0; i < 2; ++i) {
     OLV_TRY(b.k[i].alloc((size_t)m * 4));
-    OLV_TRY(b.v[i].alloc((size_t)m * 4));
+    OLV_TRY(b.v[i].alloc((size_t)m * sizeof(V)));
   }
   OLV_TRY(b.counts.alloc(((size_t)kRDigits * ntiles + 1) * 4));
   const uint32_t passes = std::max(1u, (key_bits + kRBits - 1) / kRBits);
-  const uint32_t *kin = keys_in, *vin = nullptr;
   hipStream_t s = stream();
-  uint32_t cur = 0;
-  for (uint32_t p = 0; p < passes; ++p) {
-    cur = p & 1u;
+  uint32_t *counts = b.counts.as<uint32_t>();
+  for (uint32_t p = 0; p < passes && ntiles; ++p) {
+    const int out = (int)(p & 1u);
     const uint32_t shift = p * kRBits;
-    uint32_t *counts = b.counts.as<uint32_t>();
-    if (ntiles) {
-      hipLaunchKernelGGL(radix_count_kernel, dim3(ntiles), dim3(kIB), 0, s, kin, m, shift, ntiles, counts);
-      OLV_HIP(hipGetLastError());
-      OLV_TRY(exclusive_scan(counts, counts, (size_t)kRDigits * ntiles));
-      hipLaunchKernelGGL(radix_scatter_kernel, dim3(ntiles), dim3(kIB), 0, s, kin, vin, b.k[cur].as<uint32_t>(),
-                         b.v[cur].as<uint32_t>(), m, shift, ntiles, counts);
-      OLV_HIP(hipGetLastError());
-    }
-    kin = b.k[cur].as<uint32_t>();
-    vin = b.v[cur].as<uint32_t>();
+    const uint32_t *kin = p == 0 ? keys_in : b.k[out ^ 1].as<uint32_t>();
+    hipLaunchKernelGGL(radix_count_kernel, dim3(ntiles), dim3(kIB), 0, s, kin, m, shift, ntiles, counts);
+    OLV_HIP(hipGetLastError());
+    OLV_TRY(exclusive_scan(counts, counts, (size_t)kRDigits * ntiles));
+    if (p == 0)
+      hipLaunchKernelGGL((radix_scatter_kernel<V, Load0>), dim3(ntiles), dim3(kIB), 0, s, kin, load0,
+                         b.k[out].as<uint32_t>(), b.v[out].as<V>(), m, shift, ntiles, counts);
+    else
+      hipLaunchKernelGGL((radix_scatter_kernel<V, LoadPrev<V>>), dim3(ntiles), dim3(kIB), 0, s, kin,
+                         LoadPrev<V>{b.v[out ^ 1].as<V>()}, b.k[out].as<uint32_t>(), b.v[out].as<V>(), m, shift,
+                         ntiles, counts);
+    OLV_HIP(hipGetLastError());
+    b.cur = out;
   }
-  *sorted_keys = b.k[cur].as<uint32_t>();
-  *perm = b.v[cur].as<uint32_t>();
   return OLV_OK;
+}
+
+// a[i] = in[i].x, b[i] = in[i].y
+__global__ void split_kernel(const uint2 *__restrict__ in, uint32_t m, uint32_t *__restrict__ a,
+                             uint32_t *__restrict__ b) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x) {
+    const uint2 x = in[i];
+    a[i] = x.x;
+    b[i] = x.y;
+  }
 }
 
 uint32_t bits_for(uint32_t n) {  // bits of the largest id n - 1
@@ -295,13 +356,6 @@ __global__ void rows_from_sorted_kernel(const uint32_t *__restrict__ keys, uint3
     const int64_t lo = i > 0 ? (int64_t)keys[i - 1] : -1;
     for (int64_t v = lo + 1; v <= (int64_t)hi; ++v) rp[v] = (uint32_t)i;
   }
-}
-
-// out[i] = in[perm[i]] (4-byte items).
-__global__ void gather_kernel(const uint32_t *__restrict__ in, const uint32_t *__restrict__ perm, uint32_t m,
-                              uint32_t *__restrict__ out) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (size_t)gridDim.x * blockDim.x)
-    out[i] = in[perm[i]];
 }
 
 // src[e] = v for every edge e of row v.
@@ -386,13 +440,13 @@ int assemble(uint32_t n, uint32_t m, DevBuf &rp, DevBuf &col, DevBuf &w, int wdt
     OLV_TRY(icol.alloc((size_t)m * 4));
     OLV_TRY(ieid.alloc((size_t)m * 4));
     if (m) {
+      // CSR positions sorted by target, carrying (position, source)
       SortBufs b;
-      uint32_t *tgt = nullptr, *perm = nullptr;
-      OLV_TRY(radix_sort_index(col.as<uint32_t>(), m, bits_for(n), b, &tgt, &perm));
-      hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, tgt, m, n,
-                         irp.as<uint32_t>());
-      hipLaunchKernelGGL(gather_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, esrc, perm, m, icol.as<uint32_t>());
-      OLV_HIP(hipMemcpyAsync(ieid.p, perm, (size_t)m * 4, hipMemcpyDeviceToDevice, s));
+      OLV_TRY(radix_sort<uint2>(col.as<uint32_t>(), LoadIndexAnd{esrc}, m, bits_for(n), b));
+      hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, b.keys(), m,
+                         n, irp.as<uint32_t>());
+      hipLaunchKernelGGL(split_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, b.vals<uint2>(), m,
+                         ieid.as<uint32_t>(), icol.as<uint32_t>());
       OLV_HIP(hipGetLastError());
       OLV_HIP(hipStreamSynchronize(s));  // sort buffers are freed on return
     } else {
@@ -440,22 +494,26 @@ int build_from_coo(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *
   hipStream_t s = stream();
   DevBuf rp, col, wout;
   OLV_TRY(rp.alloc(((size_t)n + 1) * 4));
-  OLV_TRY(col.alloc((size_t)m * 4));
+  if (wdtype != OLV_W_NONE || m == 0) OLV_TRY(col.alloc((size_t)m * 4));
   if (wdtype != OLV_W_NONE) OLV_TRY(wout.alloc((size_t)m * 4));
   if (m == 0) {
     OLV_TRY(fill_u32(rp.as<uint32_t>(), 0u, (size_t)n + 1));
     return assemble(n, m, rp, col, wout, wdtype, nullptr, transpose, out);
   }
+  // edges sorted by source, carrying the target (and the weight)
   SortBufs b;
-  uint32_t *keys = nullptr, *perm = nullptr;
-  OLV_TRY(radix_sort_index(src, m, bits_for(n), b, &keys, &perm));
-  hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, keys, m, n,
+  if (wdtype == OLV_W_NONE) {
+    OLV_TRY(radix_sort<uint32_t>(src, LoadArr{dst}, m, bits_for(n), b));
+    std::swap(col.p, b.v[b.cur].p);  // the sorted targets are the column array
+  } else {
+    OLV_TRY(radix_sort<uint2>(src, LoadPair{dst, w}, m, bits_for(n), b));
+    hipLaunchKernelGGL(split_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, b.vals<uint2>(), m,
+                       col.as<uint32_t>(), wout.as<uint32_t>());
+  }
+  hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, b.keys(), m, n,
                      rp.as<uint32_t>());
-  hipLaunchKernelGGL(gather_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, dst, perm, m, col.as<uint32_t>());
-  if (wdtype != OLV_W_NONE)
-    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, w, perm, m, wout.as<uint32_t>());
   OLV_HIP(hipGetLastError());
-  return assemble(n, m, rp, col, wout, wdtype, keys, transpose, out);
+  return assemble(n, m, rp, col, wout, wdtype, b.keys(), transpose, out);
 }
 
 // ------------------------------------------------------------------- text
@@ -512,9 +570,25 @@ __global__ void __launch_bounds__(kIB) nl_write_kernel(const uint8_t *__restrict
     if (((wv[i / 4] >> (8 * (i % 4))) & 0xFFu) == '\n') starts[1 + (size_t)(r++)] = c0 + i + 1;
 }
 
+// Sequential reads of text bytes through aligned dword loads (the text
+// buffer is padded past its end, so the dword of any byte < len exists).
+struct ByteReader {
+  const uint32_t *w;
+  size_t cached = ~(size_t)0;
+  uint32_t word = 0;
+  __device__ explicit ByteReader(const uint8_t *t) : w(reinterpret_cast<const uint32_t *>(t)) {}
+  __device__ __forceinline__ uint8_t operator[](size_t p) {
+    if ((p >> 2) != cached) {
+      cached = p >> 2;
+      word = w[cached];
+    }
+    return (uint8_t)(word >> (8 * (p & 3)));
+  }
+};
+
 // Parse one token starting at t[p] (not whitespace, p < e) as an integer:
 // [+-]digits, at most 18 digits.  Advances p past the token.
-__device__ __forceinline__ bool parse_int(const uint8_t *t, size_t &p, size_t e, long long &val) {
+__device__ __forceinline__ bool parse_int(ByteReader &t, size_t &p, size_t e, long long &val) {
   bool neg = false;
   if (t[p] == '+' || t[p] == '-') {
     neg = t[p] == '-';
@@ -534,8 +608,9 @@ __device__ __forceinline__ bool parse_int(const uint8_t *t, size_t &p, size_t e,
 }
 
 // Line k spans [starts[k], starts[k+1] - 1).  Code: 0 comment, 1 blank, 2 content.
-__device__ __forceinline__ uint32_t line_code(const uint8_t *t, const unsigned long long *starts, size_t k,
+__device__ __forceinline__ uint32_t line_code(const uint8_t *text, const unsigned long long *starts, size_t k,
                                               size_t &b, size_t &e) {
+  ByteReader t(text);
   b = starts[k];
   e = starts[k + 1] - 1;
   while (b < e && is_space(t[b])) ++b;
@@ -601,14 +676,15 @@ __global__ void __launch_bounds__(kIB) edge_parse_kernel(EdgeParse a) {
     const uint32_t di = a.didx[k];
     if (a.didx[k + 1] == di || di == 0 || di > a.m) continue;  // not content, header, or beyond m
     size_t b = a.starts[k], e = a.starts[k + 1] - 1;
+    ByteReader t(a.t);
     long long v[3] = {0, 0, 0};
     uint32_t ntok = 0;
     bool ok = true;
     for (;;) {
-      while (b < e && is_space(a.t[b])) ++b;
+      while (b < e && is_space(t[b])) ++b;
       if (b >= e) break;
       long long x;
-      const bool okt = parse_int(a.t, b, e, x);
+      const bool okt = parse_int(t, b, e, x);
       if (ntok < 3) {  // columns after the third are not read
         ok = ok && okt;
         v[ntok] = x;
@@ -671,13 +747,14 @@ __global__ void __launch_bounds__(kIB) adj_parse_kernel(AdjParse a) {
     const uint32_t v = bi - a.hb - 1;
     if (v >= a.n) continue;
     size_t b = a.starts[k], e = a.starts[k + 1] - 1;
+    ByteReader t(a.t);
     uint32_t ntok = 0;
     uint32_t base = WRITE ? a.rp[v] : 0u;
     for (;;) {
-      while (b < e && is_space(a.t[b])) ++b;
+      while (b < e && is_space(t[b])) ++b;
       if (b >= e) break;
       long long x;
-      const bool ok = parse_int(a.t, b, e, x);
+      const bool ok = parse_int(t, b, e, x);
       if (!WRITE) {
         if (!ok) report(a.err, k, kErrToken);
         else if (x < 1 || x > a.n) report(a.err, k, kErrId);
@@ -905,9 +982,9 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   OLV_TRY(read_word(wr.as<uint32_t>(), &w_range));
   const bool has_w = m > 0 && min_tok != 0xffffffffu && min_tok >= 3;
   if (has_w && w_range) return fail(OLV_EINVAL, "%s: an edge weight does not fit int32", who);
-  L.text = DevBuf();  // release the text before the sort
-  L.starts = DevBuf();
-  didx = DevBuf();
+  L.text.reset();  // release the text before the sort
+  L.starts.reset();
+  didx.reset();
   return build_from_coo(n, m, src.as<uint32_t>(), dst.as<uint32_t>(), has_w ? w.as<uint32_t>() : nullptr,
                         has_w ? OLV_W_I32 : OLV_W_NONE, (flags & OLV_BUILD_TRANSPOSE) != 0, out);
 }
@@ -978,10 +1055,10 @@ int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, un
   hipLaunchKernelGGL(adj_parse_kernel<true>, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, s, a);
   OLV_HIP(hipGetLastError());
   OLV_HIP(hipStreamSynchronize(s));
-  L.text = DevBuf();
-  L.starts = DevBuf();
-  bidx = DevBuf();
-  deg = DevBuf();
+  L.text.reset();
+  L.starts.reset();
+  bidx.reset();
+  deg.reset();
   return assemble(n, m, rp, col, none, OLV_W_NONE, nullptr, (flags & OLV_BUILD_TRANSPOSE) != 0, out);
 }
 

@@ -90,8 +90,13 @@ void free_pr_work(olv_graph *g);
 // RAII device buffer for temporaries inside an entry point.
 struct DevBuf {
   void *p = nullptr;
-  ~DevBuf() {
+  DevBuf() = default;
+  DevBuf(const DevBuf &) = delete;
+  DevBuf &operator=(const DevBuf &) = delete;
+  ~DevBuf() { reset(); }
+  void reset() {
     if (p) (void)hipFree(p);
+    p = nullptr;
   }
   int alloc(size_t bytes) {
     if (hipMalloc(&p, bytes ? bytes : 4) != hipSuccess) {
