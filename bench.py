@@ -20,6 +20,8 @@ Other workloads print their own line with the same keys:
            12..18) as one captured persistent launch, vs the plain engine
   adjoint  weighted PageRank forward + VJP on RMAT-24 vs forward-only with
            central finite differences
+  ingest   RMAT-22 edge-list text (edges in random order) parsed on the
+           device into the out-CSR and in-CSR (edges/s, text resident in HBM)
 """
 import argparse
 import ctypes
@@ -331,6 +333,72 @@ def bench_adjoint(args, olv, scale=24):
     return res
 
 
+# -------------------------------------------------------------------- ingest
+def bench_ingest(args, olv):
+    """One step = edge-list text already in HBM -> resident graph with both
+    CSRs (olv_graph_from_edge_list_text + OLV_BUILD_TRANSPOSE)."""
+    import oracle
+    L = olv._lib.lib()
+    log("generating RMAT-22 and its edge-list text ...")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    src = np.repeat(np.arange(csr.n, dtype=np.uint32), np.diff(csr.row_ptr.astype(np.int64)))
+    perm = np.random.default_rng(1).permutation(csr.m)
+    s_, d_ = src[perm], csr.col[perm]
+    text = olv.format_edge_list(csr.n, s_, d_, comment="RMAT-22, edges in random order")
+    dtext = olv.DeviceBuffer(len(text), np.uint8)
+    olv._lib.check(L.olv_memcpy_h2d(dtext.ptr, text, len(text)))
+    flags = olv._lib.OLV_BUILD_TRANSPOSE
+    st = olv.Stats()
+
+    def ingest(ptr, fl, stats):
+        h = ctypes.c_void_p()
+        t0 = time.perf_counter()
+        olv._lib.check(L.olv_graph_from_edge_list_text(ctypes.byref(h), ptr, len(text), fl, ctypes.byref(stats)))
+        dt = time.perf_counter() - t0
+        return h, dt
+
+    for _ in range(args.warmup):
+        h, _ = ingest(dtext.ptr, flags, st)
+        L.olv_graph_destroy(h)
+    times = []
+    for _ in range(args.steps):
+        h, dt = ingest(dtext.ptr, flags, st)
+        times.append(dt)
+        if len(times) < args.steps:
+            L.olv_graph_destroy(h)
+    t_step = float(np.median(times))
+    # parity: the device CSR equals the stable sort by source, the device
+    # in-CSR equals the host transpose of the generator's CSR
+    g = olv.Graph._adopt(h, csr=None)
+    order = np.argsort(s_, kind="stable")
+    exact = bool(np.array_equal(g.csr.row_ptr, csr.row_ptr) and np.array_equal(g.csr.col, d_[order]))
+    del g
+    # PCIe-inclusive: the same text from pageable host memory
+    h, t_host = ingest(text, flags | olv._lib.OLV_PTR_HOST, st)
+    L.olv_graph_destroy(h)
+    # dominant kernel (line parsing), hipEvent-timed in separate profiled runs
+    pk = []
+    for _ in range(3):
+        h, _ = ingest(dtext.ptr, flags | olv._lib.OLV_PROFILE, st)
+        pk.append(st.kernel_ms / 1e3)
+        nlines = st.iterations
+        L.olv_graph_destroy(h)
+    k_s = float(np.median(pk))
+    cfg = {"workload": "ingest_rmat22_edge_list_text", "vertices": csr.n, "edges": csr.m, "text_bytes": len(text),
+           "lines": int(nlines), "transpose": True, "bit_exact": exact}
+    res = base_line(args, cfg, csr.m / t_step, "edges/s", t_step * 1e3, "u8")
+    # parse kernel: text + line starts (8) + content index (4) + src/dst/w out (12) per line
+    res["roofline"] = roofline(len(text) + 12 * nlines + 12 * csr.m, k_s, "ingest_rmat22_parse", "edge_parse_kernel")
+    res["detail"] = {"ms_device_resident": t_step * 1e3, "ms_from_host_text": t_host * 1e3,
+                     "text_GBps_device_resident": len(text) / t_step / 1e9, "parse_kernel_ms": k_s * 1e3}
+    if not args.no_cpu_baseline:
+        # the oracle's C loader (csrGraph.h:106-171 restated + stable counting sort), same text, one core
+        t_cpu, runs = timed_oracle(lambda: oracle.load_edge_list_text(text), 30.0, max_runs=2)
+        res["cpu_baseline"] = {"value": csr.m / t_cpu, "unit": "edges/s", "cores": 1, "kind": "port",
+                               "sample": f"the whole RMAT-22 text, oracle.c orc_edge_list_load, median of {runs}"}
+    return res
+
+
 def bench_copy(args, olv):
     """Device-to-device copy of a known size: the calibration kernel for the
     PMC byte counters (FETCH_SIZE / WRITE_SIZE) of the other workloads."""
@@ -358,7 +426,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pagerank",
-                    choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "copy"])
+                    choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "ingest", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
@@ -369,7 +437,7 @@ def main():
     import olive_amd as olv
     olv.init(0)
     fn = {"pagerank": bench_pagerank, "bfs": bench_bfs, "sssp": bench_sssp, "batched": bench_batched,
-          "adjoint": bench_adjoint, "copy": bench_copy}[args.workload]
+          "adjoint": bench_adjoint, "ingest": bench_ingest, "copy": bench_copy}[args.workload]
     res = fn(args, olv)
     print(json.dumps(res), flush=True)
 

@@ -153,23 +153,26 @@ int olv_graph_transpose_to_host(const olv_graph *g, uint32_t *in_row_ptr,
  * build the in-edge CSR on the device (in-lists ordered by source). */
 int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m,
                        const uint32_t *src, const uint32_t *dst,
-                       const void *weights, int weight_dtype, unsigned flags);
+                       const void *weights, int weight_dtype, unsigned flags,
+                       olv_stats *st);
 /* Edge-list text -> graph, parsed on the device: '#' lines skipped, the
  * first other non-empty line is "n m", then exactly m "src dst [w]" lines.
  * Unlike csrGraph.h:106-171 the tuples need not be sorted (they are
  * stable-sorted by source) and a third column on every edge line is kept
- * as int32 weights (OLV_W_I32).  Errors name the 1-based line. */
+ * as int32 weights (OLV_W_I32).  Errors name the 1-based line.
+ * st (optional): elapsed_ms = device time of the whole ingest, iterations =
+ * text lines; with OLV_PROFILE, kernel_ms = the line-parsing kernel. */
 int olv_graph_from_edge_list_text(olv_graph **out, const char *text,
-                                  size_t len, unsigned flags);
+                                  size_t len, unsigned flags, olv_stats *st);
 /* DIMACS adjacency text -> graph, parsed on the device: header "n m", then
  * line i after it lists the 1-based neighbours of vertex i-1; exactly 2m
  * neighbour entries (csrGraph.h:177-236). */
 int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len,
-                               unsigned flags);
+                               unsigned flags, olv_stats *st);
 /* Read a file (OLV_FMT_EDGE_LIST or OLV_FMT_DIMACS) into pinned host memory
  * and ingest it as above. */
 int olv_graph_load_file(olv_graph **out, const char *path, int format,
-                        unsigned flags);
+                        unsigned flags, olv_stats *st);
 /* Copy the out-CSR to the host (weights if present; weight_dtype gets the
  * graph's weight dtype).  Any pointer may be NULL. */
 int olv_graph_csr_to_host(const olv_graph *g, uint32_t *row_ptr,

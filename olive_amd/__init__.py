@@ -257,7 +257,7 @@ def ingest_coo(n: int, src, dst, weights=None, transpose: bool = True) -> Graph:
     h = ctypes.c_void_p()
     flags = OLV_PTR_HOST | (OLV_BUILD_TRANSPOSE if transpose else 0)
     check(lib().olv_graph_from_coo(ctypes.byref(h), n, s.size, s.ctypes.data if s.size else None,
-                                   d.ctypes.data if d.size else None, wp, wd, flags), "olv_graph_from_coo")
+                                   d.ctypes.data if d.size else None, wp, wd, flags, None), "olv_graph_from_coo")
     return Graph._adopt(h)
 
 
@@ -266,7 +266,7 @@ def _ingest_text(fn, name, text, transpose):
     buf = text.encode() if isinstance(text, str) else bytes(text)
     h = ctypes.c_void_p()
     flags = OLV_PTR_HOST | (OLV_BUILD_TRANSPOSE if transpose else 0)
-    check(fn(ctypes.byref(h), buf, len(buf), flags), name)
+    check(fn(ctypes.byref(h), buf, len(buf), flags, None), name)
     return Graph._adopt(h)
 
 
@@ -287,7 +287,7 @@ def ingest_file(path, fmt: str = "edge_list", transpose: bool = True) -> Graph:
     f = {"edge_list": OLV_FMT_EDGE_LIST, "dimacs": OLV_FMT_DIMACS}[fmt]
     h = ctypes.c_void_p()
     check(lib().olv_graph_load_file(ctypes.byref(h), str(path).encode(), f,
-                                    OLV_BUILD_TRANSPOSE if transpose else 0), "olv_graph_load_file")
+                                    OLV_BUILD_TRANSPOSE if transpose else 0, None), "olv_graph_load_file")
     return Graph._adopt(h)
 
 

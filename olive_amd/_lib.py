@@ -106,10 +106,10 @@ SIGNATURES = {
     "olv_graph_counts": (_int, [_vp, _u32p, _u32p]),
     "olv_graph_device_arrays": (_int, [_vp] + [ctypes.POINTER(_vp)] * 6),
     "olv_graph_transpose_to_host": (_int, [_vp, _vp, _vp, _vp]),
-    "olv_graph_from_coo": (_int, [ctypes.POINTER(_vp), _u32, _u32, _vp, _vp, _vp, _int, _uint]),
-    "olv_graph_from_edge_list_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint]),
-    "olv_graph_from_dimacs_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint]),
-    "olv_graph_load_file": (_int, [ctypes.POINTER(_vp), ctypes.c_char_p, _int, _uint]),
+    "olv_graph_from_coo": (_int, [ctypes.POINTER(_vp), _u32, _u32, _vp, _vp, _vp, _int, _uint, _statp]),
+    "olv_graph_from_edge_list_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint, _statp]),
+    "olv_graph_from_dimacs_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint, _statp]),
+    "olv_graph_load_file": (_int, [ctypes.POINTER(_vp), ctypes.c_char_p, _int, _uint, _statp]),
     "olv_graph_csr_to_host": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_int)]),
     "olv_bfs": (_int, [_vp, _u32, _vp, _uint, _statp]),
     "olv_bfs_multisource": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),

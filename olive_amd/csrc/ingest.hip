@@ -881,7 +881,7 @@ using namespace olv;
 extern "C" {
 
 int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *dst,
-                       const void *weights, int weight_dtype, unsigned flags) {
+                       const void *weights, int weight_dtype, unsigned flags, olv_stats *st) {
   if (!out) return fail(OLV_EINVAL, "olv_graph_from_coo: out is NULL");
   *out = nullptr;
   if (n == 0) return fail(OLV_EINVAL, "olv_graph_from_coo: graph has no vertices");
@@ -890,6 +890,9 @@ int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m, const uint32_t *
     return fail(OLV_EINVAL, "olv_graph_from_coo: unknown weight dtype %d", weight_dtype);
   if (weight_dtype != OLV_W_NONE && m > 0 && !weights) return fail(OLV_EINVAL, "olv_graph_from_coo: NULL weights");
   (void)current_device();
+  if (st) std::memset(st, 0, sizeof(*st));
+  Timer t;
+  t.start();
   const bool host = (flags & OLV_PTR_HOST) != 0;
   DevBuf ds, dd, dw, flag;
   const uint32_t *s = src, *d = dst, *w = static_cast<const uint32_t *>(weights);
@@ -916,15 +919,22 @@ int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m, const uint32_t *
     OLV_TRY(read_word(flag.as<uint32_t>(), &bad));
     if (bad) return fail(OLV_EINVAL, "olv_graph_from_coo: a vertex id is >= n (%u)", n);
   }
-  return build_from_coo(n, m, s, d, w, weight_dtype, (flags & OLV_BUILD_TRANSPOSE) != 0, out);
+  OLV_TRY(build_from_coo(n, m, s, d, w, weight_dtype, (flags & OLV_BUILD_TRANSPOSE) != 0, out));
+  t.stop();
+  if (st) st->elapsed_ms = t.ms();
+  return OLV_OK;
 }
 
-int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len, unsigned flags) {
+int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len, unsigned flags,
+                                  olv_stats *st) {
   static const char *who = "olv_graph_from_edge_list_text";
   if (!out) return fail(OLV_EINVAL, "%s: out is NULL", who);
   *out = nullptr;
   if (!text && len) return fail(OLV_EINVAL, "%s: NULL text", who);
   (void)current_device();
+  if (st) std::memset(st, 0, sizeof(*st));
+  Timer t;
+  t.start();
   hipStream_t s = stream();
   Lines L;
   OLV_TRY(split_lines(text, len, (flags & OLV_PTR_HOST) != 0, L));
@@ -971,9 +981,16 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   a.min_tok = mt.as<uint32_t>();
   a.w_range = wr.as<uint32_t>();
   a.err = err.as<unsigned long long>();
+  Timer tk;
+  tk.start();
   hipLaunchKernelGGL(edge_parse_kernel, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, s, a);
+  tk.stop();
   OLV_HIP(hipGetLastError());
   OLV_HIP(hipStreamSynchronize(s));
+  if (st && (flags & OLV_PROFILE)) {
+    st->kernel_ms = tk.ms();
+    st->kernel_launches = 1;
+  }
   unsigned long long e = 0;
   OLV_TRY(read_err(err, &e));
   if (e != ~0ull) return parse_error(who, e);
@@ -985,16 +1002,25 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   L.text.reset();  // release the text before the sort
   L.starts.reset();
   didx.reset();
-  return build_from_coo(n, m, src.as<uint32_t>(), dst.as<uint32_t>(), has_w ? w.as<uint32_t>() : nullptr,
-                        has_w ? OLV_W_I32 : OLV_W_NONE, (flags & OLV_BUILD_TRANSPOSE) != 0, out);
+  OLV_TRY(build_from_coo(n, m, src.as<uint32_t>(), dst.as<uint32_t>(), has_w ? w.as<uint32_t>() : nullptr,
+                         has_w ? OLV_W_I32 : OLV_W_NONE, (flags & OLV_BUILD_TRANSPOSE) != 0, out));
+  t.stop();
+  if (st) {
+    st->elapsed_ms = t.ms();
+    st->iterations = L.nl;  // text lines
+  }
+  return OLV_OK;
 }
 
-int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, unsigned flags) {
+int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, unsigned flags, olv_stats *st) {
   static const char *who = "olv_graph_from_dimacs_text";
   if (!out) return fail(OLV_EINVAL, "%s: out is NULL", who);
   *out = nullptr;
   if (!text && len) return fail(OLV_EINVAL, "%s: NULL text", who);
   (void)current_device();
+  if (st) std::memset(st, 0, sizeof(*st));
+  Timer t;
+  t.start();
   hipStream_t s = stream();
   Lines L;
   OLV_TRY(split_lines(text, len, (flags & OLV_PTR_HOST) != 0, L));
@@ -1059,10 +1085,16 @@ int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, un
   L.starts.reset();
   bidx.reset();
   deg.reset();
-  return assemble(n, m, rp, col, none, OLV_W_NONE, nullptr, (flags & OLV_BUILD_TRANSPOSE) != 0, out);
+  OLV_TRY(assemble(n, m, rp, col, none, OLV_W_NONE, nullptr, (flags & OLV_BUILD_TRANSPOSE) != 0, out));
+  t.stop();
+  if (st) {
+    st->elapsed_ms = t.ms();
+    st->iterations = L.nl;
+  }
+  return OLV_OK;
 }
 
-int olv_graph_load_file(olv_graph **out, const char *path, int format, unsigned flags) {
+int olv_graph_load_file(olv_graph **out, const char *path, int format, unsigned flags, olv_stats *st) {
   if (!out || !path) return fail(OLV_EINVAL, "olv_graph_load_file: NULL argument");
   *out = nullptr;
   if (format != OLV_FMT_EDGE_LIST && format != OLV_FMT_DIMACS)
@@ -1083,14 +1115,14 @@ int olv_graph_load_file(olv_graph **out, const char *path, int format, unsigned 
   }
   const size_t got = std::fread(buf, 1, (size_t)sz, f);
   std::fclose(f);
-  int st = OLV_OK;
-  if (got != (size_t)sz) st = fail(OLV_EINVAL, "olv_graph_load_file: short read of %s", path);
-  const unsigned fl = (flags & ~OLV_PTR_HOST) | OLV_PTR_HOST;
-  if (st == OLV_OK)
-    st = format == OLV_FMT_EDGE_LIST ? olv_graph_from_edge_list_text(out, buf, got, fl)
-                                     : olv_graph_from_dimacs_text(out, buf, got, fl);
+  int rc = OLV_OK;
+  if (got != (size_t)sz) rc = fail(OLV_EINVAL, "olv_graph_load_file: short read of %s", path);
+  const unsigned fl = flags | OLV_PTR_HOST;
+  if (rc == OLV_OK)
+    rc = format == OLV_FMT_EDGE_LIST ? olv_graph_from_edge_list_text(out, buf, got, fl, st)
+                                     : olv_graph_from_dimacs_text(out, buf, got, fl, st);
   (void)hipHostFree(buf);
-  return st;
+  return rc;
 }
 
 int olv_graph_csr_to_host(const olv_graph *g, uint32_t *row_ptr, uint32_t *col, void *weights, int *weight_dtype) {

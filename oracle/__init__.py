@@ -57,8 +57,13 @@ def lib():
                                             ctypes.c_double, ctypes.c_int, f64p]
         L.orc_pagerank_adjoint.argtypes = [ctypes.c_uint32, u32p, u32p, f64p, f64p, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_int, f64p]
+        i64p = ctypes.POINTER(ctypes.c_longlong)
+        L.orc_edge_list_header.argtypes = [ctypes.c_char_p, ctypes.c_size_t, i64p, i64p]
+        L.orc_edge_list_load.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
+                                         u32p, u32p, i32p, ctypes.POINTER(ctypes.c_int)]
         for f in ("orc_bfs", "orc_sssp_frontier", "orc_dijkstra", "orc_pagerank",
-                  "orc_pagerank_weighted", "orc_pagerank_adjoint"):
+                  "orc_pagerank_weighted", "orc_pagerank_adjoint", "orc_edge_list_header",
+                  "orc_edge_list_load"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -210,6 +215,24 @@ def load_edge_list(path, with_weights=True):
     np.add.at(row_ptr, src + 1, 1)
     row_ptr = np.cumsum(row_ptr, dtype=np.uint64).astype(np.uint32)
     return n, row_ptr, dst.astype(np.uint32), w
+
+
+def load_edge_list_text(text: bytes):
+    """The C restatement of the edge-list loader (oracle.c
+    orc_edge_list_load) over text in memory: (n, row_ptr, col, w|None)."""
+    n, m = ctypes.c_longlong(), ctypes.c_longlong()
+    if lib().orc_edge_list_header(text, len(text), ctypes.byref(n), ctypes.byref(m)) != 0:
+        raise ValueError("no header line")
+    n, m = n.value, m.value
+    rp = np.empty(n + 1, np.uint32)
+    col = np.empty(max(m, 1), np.uint32)
+    w = np.empty(max(m, 1), np.int32)
+    has_w = ctypes.c_int()
+    rc = lib().orc_edge_list_load(text, len(text), n, m, _p(rp, ctypes.c_uint32), _p(col, ctypes.c_uint32),
+                                  _p(w, ctypes.c_int32), ctypes.byref(has_w))
+    if rc != 0:
+        raise ValueError(f"orc_edge_list_load failed ({rc})")
+    return n, rp, col[:m], (w[:m] if has_w.value else None)
 
 
 def load_dimacs(path):

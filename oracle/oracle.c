@@ -284,3 +284,124 @@ int orc_pagerank_adjoint(uint32_t n, const uint32_t *row_ptr,
   free(nl);
   return it;
 }
+
+/*
+ * Edge-list text -> CSR, restating CsrGraph::fromEdgeListFile
+ * (csrGraph.h:106-171): lines whose first non-blank byte is '#' and blank
+ * lines are skipped, the first other line is "n m" (csrGraph.h:127), then
+ * one "src dst" line per edge read as two integers (csrGraph.h:134-136).
+ * Two differences, the loader's documented contract: the tuples are
+ * stable-sorted by source (counting sort; the sort csrGraph.h:149 leaves
+ * commented out) and a third column present on every edge line is kept as
+ * int32 weights (csrGraph.h:139 drops it).  Pinned by: equality with the
+ * Python restatement oracle.load_edge_list on the reference's data files
+ * and on generated text with comments, CRLF, tabs and signed weights.
+ *
+ * orc_edge_list_header: *n, *m from the header; returns 0, or -1 if none.
+ * orc_edge_list_load: fills row_ptr[n+1], col[m], w[m] (w may be NULL) and
+ * *has_w; returns 0, -(line number) for a malformed line, or -2^30 when the
+ * number of edge lines differs from m.
+ */
+static int orc_blank(char c) { return c == ' ' || c == '\t' || c == '\r' || c == '\v' || c == '\f'; }
+
+static const char *orc_next_content(const char *p, const char *end, const char **eol, long long *line) {
+  while (p < end) {
+    const char *e = memchr(p, '\n', (size_t)(end - p));
+    if (!e) e = end;
+    const char *q = p;
+    while (q < e && orc_blank(*q)) q++;
+    ++*line;
+    if (q < e && *q != '#') {
+      *eol = e;
+      return q;
+    }
+    p = e + 1;
+  }
+  return NULL;
+}
+
+/* Integers of [p, e): up to 3 values, returns the token count or -1. */
+static int orc_tokens(const char *p, const char *e, long long *v) {
+  int nt = 0;
+  while (p < e) {
+    while (p < e && orc_blank(*p)) p++;
+    if (p >= e) break;
+    const char *q = p;
+    int neg = 0, nd = 0, ok = 1;
+    long long x = 0;
+    if (*q == '+' || *q == '-') neg = *q++ == '-';
+    for (; q < e && !orc_blank(*q); q++, nd++) {
+      if (*q < '0' || *q > '9' || nd >= 18) ok = 0;
+      else x = x * 10 + (*q - '0');
+    }
+    if (nt < 3) {
+      if (!ok || nd == 0) return -1;
+      v[nt] = neg ? -x : x;
+    }
+    nt++;
+    p = q;
+  }
+  return nt;
+}
+
+int orc_edge_list_header(const char *text, size_t len, long long *n, long long *m) {
+  const char *eol;
+  long long line = 0, v[3];
+  const char *p = orc_next_content(text, text + len, &eol, &line);
+  if (!p || orc_tokens(p, eol, v) < 2) return -1;
+  *n = v[0];
+  *m = v[1];
+  return 0;
+}
+
+int orc_edge_list_load(const char *text, size_t len, uint32_t n, uint32_t m, uint32_t *row_ptr, uint32_t *col,
+                       int32_t *w, int *has_w) {
+  const char *end = text + len, *eol;
+  long long line = 0, v[3];
+  const char *p = orc_next_content(text, end, &eol, &line);
+  uint32_t *src = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+  uint32_t *dst = (uint32_t *)malloc(sizeof(uint32_t) * (m ? m : 1));
+  int32_t *wt = (int32_t *)malloc(sizeof(int32_t) * (m ? m : 1));
+  uint64_t e = 0;
+  int minw = 1, rc = 0;
+  if (!src || !dst || !wt) rc = -3;
+  while (rc == 0 && p && (p = orc_next_content(eol + (eol < end), end, &eol, &line))) {
+    const int nt = orc_tokens(p, eol, v);
+    if (nt < 2 || v[0] < 0 || v[0] >= n || v[1] < 0 || v[1] >= n) {
+      rc = (int)-line;
+      break;
+    }
+    if (e >= m) {
+      rc = -(1 << 30);
+      break;
+    }
+    src[e] = (uint32_t)v[0];
+    dst[e] = (uint32_t)v[1];
+    wt[e] = nt >= 3 ? (int32_t)v[2] : 0;
+    if (nt < 3) minw = 0;
+    e++;
+  }
+  if (rc == 0 && e != m) rc = -(1 << 30);
+  if (rc == 0) {
+    /* stable counting sort by source */
+    memset(row_ptr, 0, sizeof(uint32_t) * ((size_t)n + 1));
+    for (uint64_t i = 0; i < m; i++) row_ptr[src[i] + 1]++;
+    for (uint32_t i = 0; i < n; i++) row_ptr[i + 1] += row_ptr[i];
+    uint32_t *cur = (uint32_t *)malloc(sizeof(uint32_t) * ((size_t)n + 1));
+    if (!cur) rc = -3;
+    else {
+      memcpy(cur, row_ptr, sizeof(uint32_t) * ((size_t)n + 1));
+      for (uint64_t i = 0; i < m; i++) {
+        const uint32_t k = cur[src[i]]++;
+        col[k] = dst[i];
+        if (w) w[k] = wt[i];
+      }
+      free(cur);
+    }
+    *has_w = m > 0 && minw;
+  }
+  free(src);
+  free(dst);
+  free(wt);
+  return rc;
+}

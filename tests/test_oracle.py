@@ -283,3 +283,35 @@ def test_adjoint_plants_mistake():
     fd = (L(w + h * dw) - L(w - h * dw)) / (2 * h)
     assert abs(fd - gw @ dw) < 1e-6 * max(1, abs(fd))
     assert abs(fd - bad @ dw) > 1e-4 * max(1, abs(fd))
+
+
+def test_c_edge_list_loader_matches_python_restatement():
+    """oracle.c's edge-list loader (the CPU baseline of the ingest workload)
+    equals the Python restatement on the reference's data files and on
+    generated text with comments, blank lines, CRLF, tabs and weights."""
+    import olive_amd
+    from conftest import DATA_FILES, data_file
+
+    def same(a, b):
+        return (a[0] == b[0] and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+                and ((a[3] is None and b[3] is None) or np.array_equal(a[3], b[3])))
+
+    for name in DATA_FILES:
+        with open(data_file(name), "rb") as fh:
+            text = fh.read()
+        assert same(oracle.load_edge_list(data_file(name)), oracle.load_edge_list_text(text)), name
+    rng = np.random.default_rng(2)
+    n, m = 3000, 40000
+    for w in (None, rng.integers(-10 ** 9, 10 ** 9, m)):
+        text = olive_amd.format_edge_list(n, rng.integers(0, n, m), rng.integers(0, n, m), w)
+        lines = text.split(b"\n")
+        messy = b"# c\n\n" + b"\n".join((b"  " + ln.replace(b" ", b"\t") + b"\r") if i % 3 == 1 else ln
+                                        for i, ln in enumerate(lines))
+        import tempfile
+        with tempfile.NamedTemporaryFile(suffix=".txt", delete=False) as fh:
+            fh.write(messy)
+            path = fh.name
+        try:
+            assert same(oracle.load_edge_list(path), oracle.load_edge_list_text(messy))
+        finally:
+            os.unlink(path)
