@@ -59,6 +59,10 @@ enum { OLV_W_NONE = 0, OLV_W_I32 = 1, OLV_W_F32 = 2 };
 /* text formats for olv_graph_load_file */
 enum { OLV_FMT_EDGE_LIST = 0, OLV_FMT_DIMACS = 1 };
 
+/* value formats for olv_write_text: "%d\n" of int32, "%f\n" of float or
+ * double (printed as C's printf prints the double) */
+enum { OLV_TXT_I32 = 0, OLV_TXT_F32 = 1, OLV_TXT_F64 = 2 };
+
 typedef struct olv_graph olv_graph;
 
 /* Per-call statistics (all optional: pass NULL). */
@@ -177,6 +181,28 @@ int olv_graph_load_file(olv_graph **out, const char *path, int format,
  * graph's weight dtype).  Any pointer may be NULL. */
 int olv_graph_csr_to_host(const olv_graph *g, uint32_t *row_ptr,
                           uint32_t *col, void *weights, int *weight_dtype);
+
+/* ---- result egress (replaces Oliver::printVertices / printEdges
+ *      oliver.h:204-212 over GRD::persist grd.h:118-124 and the per-element
+ *      fprintf of GRD::print grd.h:130-134) ----------------------------- */
+/* One line per value in the programs' text formats (BFS.cu:41-43 "%d",
+ * PageRank.cu:42-44 "%f"), formatted on the device and streamed to `path`.
+ * values[i] sits at byte i*stride (stride 0 = the value size), so one field
+ * of a vertex struct can be written.  values is a device pointer unless
+ * OLV_PTR_HOST.  Doubles of magnitude >= 2^63 give OLV_EUNSUPPORTED. */
+int olv_write_text(const char *path, const void *values, size_t count,
+                   size_t stride, int fmt, unsigned flags);
+/* Binary dump: 24-byte header ("OLVR", version 1, dtype tag, element size,
+ * count; little-endian) then the values, copied through two pinned buffers
+ * so device-to-host copies overlap the file writes. */
+int olv_write_binary(const char *path, const void *values, size_t count,
+                     size_t elem_bytes, int dtype, unsigned flags);
+/* Read a dump written by olv_write_binary.  values == NULL reads only the
+ * header; else the values go to `values` (device unless OLV_PTR_HOST),
+ * OLV_ERANGE if they exceed capacity_bytes. */
+int olv_read_binary(const char *path, void *values, size_t capacity_bytes,
+                    uint64_t *count, uint32_t *elem_bytes, int *dtype,
+                    unsigned flags);
 
 /* ---- algorithms -------------------------------------------------------- */
 /* BFS levels from src, INF = 0x7fffffff for unreached.  Replaces the BFS.cu

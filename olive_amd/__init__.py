@@ -26,6 +26,7 @@ __all__ = [
     "INF", "OliveError", "Stats", "CsrGraph", "Graph", "DeviceBuffer", "init", "gen_grid", "gen_rmat",
     "gen_uniform", "gen_dirichlet", "batch_scale", "batch_seed", "load_edge_list", "load_dimacs",
     "ingest_coo", "ingest_edge_list_text", "ingest_dimacs_text", "ingest_file", "format_edge_list",
+    "write_text", "write_binary", "read_binary",
     "OLV_HOST_LOOP", "OLV_NO_DIRECTION_OPT", "OLV_FORCE_BOTTOM_UP", "OLV_PTR_HOST",
 ]
 
@@ -289,6 +290,49 @@ def ingest_file(path, fmt: str = "edge_list", transpose: bool = True) -> Graph:
     check(lib().olv_graph_load_file(ctypes.byref(h), str(path).encode(), f,
                                     OLV_BUILD_TRANSPOSE if transpose else 0, None), "olv_graph_load_file")
     return Graph._adopt(h)
+
+
+# ----------------------------------------------------------- result egress
+_TXT = {"%d": 0, "%f32": 1, "%f": 2}
+
+
+def write_text(path, values, fmt: str = "%d", stride: int = 0):
+    """One value per line in the reference programs' formats ("%d" for int32,
+    "%f" for float64, "%f32" for float32 printed as "%f"), formatted on the
+    device (olv_write_text).  values: a DeviceBuffer or a numpy array."""
+    from ._lib import OLV_PTR_HOST as H
+    code = _TXT[fmt]
+    if isinstance(values, DeviceBuffer):
+        ptr, count, flags = values.ptr, values.nbytes // (stride or values.dtype.itemsize), 0
+    else:
+        a = np.ascontiguousarray(values)
+        ptr, count, flags, values = a.ctypes.data, a.nbytes // (stride or a.dtype.itemsize), H, a
+    check(lib().olv_write_text(str(path).encode(), ptr, count, stride, code, flags), "olv_write_text")
+
+
+def write_binary(path, values, dtype_tag: int = 0):
+    """Binary dump of a DeviceBuffer or numpy array (olv_write_binary)."""
+    from ._lib import OLV_PTR_HOST as H
+    if isinstance(values, DeviceBuffer):
+        check(lib().olv_write_binary(str(path).encode(), values.ptr, values.count, values.dtype.itemsize, dtype_tag, 0),
+              "olv_write_binary")
+    else:
+        a = np.ascontiguousarray(values)
+        check(lib().olv_write_binary(str(path).encode(), a.ctypes.data, a.size, a.dtype.itemsize, dtype_tag, H),
+              "olv_write_binary")
+
+
+def read_binary(path, dtype) -> "DeviceBuffer":
+    """A dump written by write_binary, read into device memory."""
+    count, esz, tag = ctypes.c_uint64(), ctypes.c_uint32(), ctypes.c_int()
+    check(lib().olv_read_binary(str(path).encode(), None, 0, ctypes.byref(count), ctypes.byref(esz), ctypes.byref(tag),
+                                0), "olv_read_binary")
+    dt = np.dtype(dtype)
+    if esz.value != dt.itemsize:
+        raise ValueError(f"{path}: elements of {esz.value} bytes, asked for {dt}")
+    out = DeviceBuffer(count.value, dt)
+    check(lib().olv_read_binary(str(path).encode(), out.ptr, out.nbytes, None, None, None, 0), "olv_read_binary")
+    return out
 
 
 # ---------------------------------------------------------------- loaders

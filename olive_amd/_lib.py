@@ -28,6 +28,7 @@ OLV_BUILD_TRANSPOSE = 0x100
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
 OLV_FMT_EDGE_LIST, OLV_FMT_DIMACS = 0, 1
+OLV_TXT_I32, OLV_TXT_F32, OLV_TXT_F64 = 0, 1, 2
 
 
 class OliveError(RuntimeError):
@@ -111,6 +112,9 @@ SIGNATURES = {
     "olv_graph_from_dimacs_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint, _statp]),
     "olv_graph_load_file": (_int, [ctypes.POINTER(_vp), ctypes.c_char_p, _int, _uint, _statp]),
     "olv_graph_csr_to_host": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_int)]),
+    "olv_write_text": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint]),
+    "olv_write_binary": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint]),
+    "olv_read_binary": (_int, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_u64), _u32p, ctypes.POINTER(_int), _uint]),
     "olv_bfs": (_int, [_vp, _u32, _vp, _uint, _statp]),
     "olv_bfs_multisource": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),
     "olv_sssp": (_int, [_vp, _u32, _vp, _vp, _uint, _statp]),

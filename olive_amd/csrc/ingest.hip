@@ -138,6 +138,8 @@ __global__ void __launch_bounds__(kIB) scan_down_kernel(const uint32_t *in, uint
   if (blockIdx.x == nt - 1 && threadIdx.x == 0) out[n] = sums[nt];
 }
 
+}  // namespace
+
 // out[0..n) = exclusive prefix sums of in[0..n), out[n] = total (sum < 2^32).
 // out must hold n + 1 words; in == out is allowed.
 int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n) {
@@ -154,6 +156,8 @@ int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n) {
   OLV_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
   return OLV_OK;
 }
+
+namespace {
 
 // ------------------------------------------------------------- radix sort
 // counts[d * ntiles + t] = keys of tile t whose digit at `shift` is d.

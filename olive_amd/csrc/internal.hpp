@@ -84,6 +84,8 @@ int fill_u32(uint32_t *p, uint32_t v, size_t count);
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
 int ensure_in_head(olv_graph *g);
 int compute_inv_outdeg(const uint32_t *rp, uint32_t n, float *inv);  // 1/outdeg (0 for sinks)
+// Device exclusive prefix sum (ingest.hip): out[0..n), out[n] = total < 2^32.
+int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n);
 void free_bfs_work(olv_graph *g);
 void free_pr_work(olv_graph *g);
 

@@ -63,12 +63,8 @@ int main(int argc, char **argv) {
   }
   LOG(INFO) << "levels: " << levels << ", time: " << sw.getElapsedMillis() << "ms";
 
-  FILE *f = fopen(out, "w");
-  if (!f) return 3;
-  std::vector<Level> lv(graph.vertexCount);
-  ol.copyVertices(lv.data());
-  for (VertexId v = 0; v < graph.vertexCount; v++) fprintf(f, "%d\n", lv[v].level);
-  fclose(f);
+  // "%d\n" per vertex, as BFS.cu:41-43 prints, formatted on the device
+  ol.writeVertices(out, offsetof(Level, level), OLV_TXT_I32);
   frontier.del();
   touched.del();
   return 0;

@@ -92,12 +92,8 @@ int main(int argc, char **argv) {
     return 4;
   }
 
-  FILE *f = fopen(out, "w");
-  if (!f) return 3;
-  std::vector<Dist> dv(graph.vertexCount);
-  ol.copyVertices(dv.data());
-  for (VertexId v = 0; v < graph.vertexCount; v++) fprintf(f, "%d\n", dv[v].d);
-  fclose(f);
+  // "%d\n" per vertex, as SSSP.cu:41-43 prints, formatted on the device
+  ol.writeVertices(out, offsetof(Dist, d), OLV_TXT_I32);
   frontier.del();
   touched.del();
   return 0;

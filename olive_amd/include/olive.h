@@ -152,6 +152,20 @@ class Olive {
     edgeValues.print();
   }
 
+  /** The text printVertices() produces when VertexValue::print() is
+   *  fprintf("%d\n" or "%f\n", field) (BFS.cu:41-43, PageRank.cu:42-44),
+   *  formatted on the device: field_offset = offsetof(VertexValue, field),
+   *  fmt = OLV_TXT_I32 / OLV_TXT_F32 / OLV_TXT_F64. */
+  inline void writeVertices(const char *path, size_t field_offset, int fmt) {
+    OLV_CHECK(olv_write_text(path, reinterpret_cast<const char *>(vertexValues.elemsDevice) + field_offset,
+                             vertexCount, sizeof(VertexValue), fmt, 0));
+  }
+
+  /** Every vertex value as a binary dump (olv_write_binary). */
+  inline void dumpVertices(const char *path) {
+    OLV_CHECK(olv_write_binary(path, vertexValues.elemsDevice, vertexCount, sizeof(VertexValue), 0, 0));
+  }
+
   /** Copy every vertex value to `out` (vertexCount elements, host). */
   inline void copyVertices(VertexValue *out) {
     vertexValues.persist();
