@@ -22,6 +22,8 @@ Other workloads print their own line with the same keys:
            central finite differences
   ingest   RMAT-22 edge-list text (edges in random order) parsed on the
            device into the out-CSR and in-CSR (edges/s, text resident in HBM)
+  egress   RMAT-22 BFS levels written as "%d\n" text (BFS.cu's output file),
+           formatted on the device (lines/s, file in the temp directory)
 """
 import argparse
 import ctypes
@@ -399,6 +401,57 @@ def bench_ingest(args, olv):
     return res
 
 
+# -------------------------------------------------------------------- egress
+def bench_egress(args, olv):
+    """One step = the 4,194,304 BFS levels of RMAT-22 (HBM-resident) written
+    as one "%d\n" line each to a file, as BFS.cu:153-155 does."""
+    import tempfile
+    import oracle
+    log("generating RMAT-22 and its BFS levels ...")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    g = olv.Graph(csr)
+    src = int(sources_of(csr, 1, 4)[0])
+    levels, _ = g.bfs(src)
+    tmpd = tempfile.mkdtemp()
+    path = os.path.join(tmpd, "BFS.txt")
+    st = olv.Stats()
+    for _ in range(args.warmup):
+        olv.write_text(path, levels, "%d")
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        olv.write_text(path, levels, "%d", stats=st)
+        times.append(time.perf_counter() - t0)
+    t_step = float(np.median(times))
+    text_bytes = st.edges_inspected
+    lv = levels.numpy()
+    with open(path, "rb") as fh:
+        exact = fh.read() == "".join(f"{x}\n" for x in lv.tolist()).encode()
+    pk = []
+    for _ in range(3):
+        olv.write_text(path, levels, "%d", stats=st, profile=True)
+        pk.append(st.kernel_ms / max(st.kernel_launches, 1) / 1e3)
+    k_s = float(np.median(pk))
+    cfg = {"workload": "egress_rmat22_bfs_levels_text", "values": csr.n, "text_bytes": int(text_bytes),
+           "format": "%d\n", "bit_exact": bool(exact)}
+    res = base_line(args, cfg, csr.n / t_step, "lines/s", t_step * 1e3, "int32")
+    # write pass: value 4 + offset 4 in, the text out
+    res["roofline"] = roofline(8 * csr.n + text_bytes, k_s, "egress_rmat22_levels", "text_write_kernel")
+    res["detail"] = {"ms_per_file": t_step * 1e3, "write_kernel_ms": k_s * 1e3}
+    if not args.no_cpu_baseline:
+        cpath = os.path.join(tmpd, "BFS_cpu.txt")
+        t_cpu, runs = timed_oracle(lambda: oracle.print_values(cpath, lv), 30.0, max_runs=3)
+        with open(cpath, "rb") as a, open(path, "rb") as b:
+            res["config"]["bit_exact"] = bool(exact and a.read() == b.read())
+        os.unlink(cpath)
+        res["cpu_baseline"] = {"value": csr.n / t_cpu, "unit": "lines/s", "cores": 1, "kind": "port",
+                               "sample": f"the same levels, oracle.c orc_print_values (fprintf per vertex), "
+                                         f"median of {runs}"}
+    os.unlink(path)
+    os.rmdir(tmpd)
+    return res
+
+
 def bench_copy(args, olv):
     """Device-to-device copy of a known size: the calibration kernel for the
     PMC byte counters (FETCH_SIZE / WRITE_SIZE) of the other workloads."""
@@ -426,7 +479,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="pagerank",
-                    choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "ingest", "copy"])
+                    choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "ingest", "egress", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     args = ap.parse_args()
@@ -437,7 +490,8 @@ def main():
     import olive_amd as olv
     olv.init(0)
     fn = {"pagerank": bench_pagerank, "bfs": bench_bfs, "sssp": bench_sssp, "batched": bench_batched,
-          "adjoint": bench_adjoint, "ingest": bench_ingest, "copy": bench_copy}[args.workload]
+          "adjoint": bench_adjoint, "ingest": bench_ingest, "egress": bench_egress,
+          "copy": bench_copy}[args.workload]
     res = fn(args, olv)
     print(json.dumps(res), flush=True)
 

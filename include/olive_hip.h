@@ -189,9 +189,11 @@ int olv_graph_csr_to_host(const olv_graph *g, uint32_t *row_ptr,
  * PageRank.cu:42-44 "%f"), formatted on the device and streamed to `path`.
  * values[i] sits at byte i*stride (stride 0 = the value size), so one field
  * of a vertex struct can be written.  values is a device pointer unless
- * OLV_PTR_HOST.  Doubles of magnitude >= 2^63 give OLV_EUNSUPPORTED. */
+ * OLV_PTR_HOST.  Doubles of magnitude >= 2^63 give OLV_EUNSUPPORTED.
+ * st (optional): edges_inspected = text bytes written; with OLV_PROFILE,
+ * kernel_ms / kernel_launches = the formatting (write-pass) kernel. */
 int olv_write_text(const char *path, const void *values, size_t count,
-                   size_t stride, int fmt, unsigned flags);
+                   size_t stride, int fmt, unsigned flags, olv_stats *st);
 /* Binary dump: 24-byte header ("OLVR", version 1, dtype tag, element size,
  * count; little-endian) then the values, copied through two pinned buffers
  * so device-to-host copies overlap the file writes. */

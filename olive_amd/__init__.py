@@ -296,7 +296,7 @@ def ingest_file(path, fmt: str = "edge_list", transpose: bool = True) -> Graph:
 _TXT = {"%d": 0, "%f32": 1, "%f": 2}
 
 
-def write_text(path, values, fmt: str = "%d", stride: int = 0):
+def write_text(path, values, fmt: str = "%d", stride: int = 0, stats: Stats | None = None, profile: bool = False):
     """One value per line in the reference programs' formats ("%d" for int32,
     "%f" for float64, "%f32" for float32 printed as "%f"), formatted on the
     device (olv_write_text).  values: a DeviceBuffer or a numpy array."""
@@ -307,7 +307,10 @@ def write_text(path, values, fmt: str = "%d", stride: int = 0):
     else:
         a = np.ascontiguousarray(values)
         ptr, count, flags, values = a.ctypes.data, a.nbytes // (stride or a.dtype.itemsize), H, a
-    check(lib().olv_write_text(str(path).encode(), ptr, count, stride, code, flags), "olv_write_text")
+    if profile:
+        flags |= OLV_PROFILE
+    check(lib().olv_write_text(str(path).encode(), ptr, count, stride, code, flags,
+                               ctypes.byref(stats) if stats is not None else None), "olv_write_text")
 
 
 def write_binary(path, values, dtype_tag: int = 0):

@@ -112,7 +112,7 @@ SIGNATURES = {
     "olv_graph_from_dimacs_text": (_int, [ctypes.POINTER(_vp), _vp, _sz, _uint, _statp]),
     "olv_graph_load_file": (_int, [ctypes.POINTER(_vp), ctypes.c_char_p, _int, _uint, _statp]),
     "olv_graph_csr_to_host": (_int, [_vp, _vp, _vp, _vp, ctypes.POINTER(_int)]),
-    "olv_write_text": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint]),
+    "olv_write_text": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint, _statp]),
     "olv_write_binary": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint]),
     "olv_read_binary": (_int, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_u64), _u32p, ctypes.POINTER(_int), _uint]),
     "olv_bfs": (_int, [_vp, _u32, _vp, _uint, _statp]),

@@ -191,7 +191,8 @@ using namespace olv;
 
 extern "C" {
 
-int olv_write_text(const char *path, const void *values, size_t count, size_t stride, int fmt, unsigned flags) {
+int olv_write_text(const char *path, const void *values, size_t count, size_t stride, int fmt, unsigned flags,
+                   olv_stats *st) {
   if (!path || (!values && count)) return fail(OLV_EINVAL, "olv_write_text: NULL argument");
   if (fmt != OLV_TXT_I32 && fmt != OLV_TXT_F32 && fmt != OLV_TXT_F64)
     return fail(OLV_EINVAL, "olv_write_text: unknown format %d", fmt);
@@ -201,10 +202,12 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
     return fail(OLV_EINVAL, "olv_write_text: values must be %zu-byte aligned with a stride that is a multiple of %zu",
                 esz, esz);
   (void)current_device();
+  if (st) std::memset(st, 0, sizeof(*st));
   File file;
   file.f = std::fopen(path, "wb");
   if (!file.f) return fail(OLV_EINVAL, "olv_write_text: cannot open %s", path);
   if (count == 0) return OLV_OK;
+  uint64_t chars = 0;
   hipStream_t s = stream();
   const bool host = (flags & OLV_PTR_HOST) != 0;
   const size_t chunk = std::min(count, kTextChunkValues);
@@ -231,9 +234,17 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
     OLV_HIP(hipMemcpy(&total, len.as<uint32_t>() + c, 4, hipMemcpyDeviceToHost));
     OLV_HIP(hipMemcpy(&b, bad.p, 4, hipMemcpyDeviceToHost));
     if (b) return fail(OLV_EUNSUPPORTED, "olv_write_text: a value of magnitude >= 2^63 cannot be formatted");
+    Timer tk;
+    tk.start();
     hipLaunchKernelGGL(text_write_kernel, dim3(grid_for(c)), dim3(kEB), 0, s, src, c, stride, fmt,
                        len.as<uint32_t>(), text.as<char>());
+    tk.stop();
     OLV_HIP(hipGetLastError());
+    if (st && (flags & OLV_PROFILE)) {
+      st->kernel_ms += tk.ms();
+      st->kernel_launches += 1;
+    }
+    chars += total;
     OLV_HIP(hipMemcpyAsync(hbuf.p, text.p, total, hipMemcpyDeviceToHost, s));
     OLV_HIP(hipStreamSynchronize(s));
     if (std::fwrite(hbuf.p, 1, total, file.f) != total) return fail(OLV_EINVAL, "olv_write_text: write to %s failed", path);
@@ -243,6 +254,7 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
     return fail(OLV_EINVAL, "olv_write_text: closing %s failed", path);
   }
   file.f = nullptr;
+  if (st) st->edges_inspected = chars;  // text bytes written
   return OLV_OK;
 }
 

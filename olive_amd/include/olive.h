@@ -158,7 +158,7 @@ class Olive {
    *  fmt = OLV_TXT_I32 / OLV_TXT_F32 / OLV_TXT_F64. */
   inline void writeVertices(const char *path, size_t field_offset, int fmt) {
     OLV_CHECK(olv_write_text(path, reinterpret_cast<const char *>(vertexValues.elemsDevice) + field_offset,
-                             vertexCount, sizeof(VertexValue), fmt, 0));
+                             vertexCount, sizeof(VertexValue), fmt, 0, NULL));
   }
 
   /** Every vertex value as a binary dump (olv_write_binary). */

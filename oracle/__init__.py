@@ -61,9 +61,10 @@ def lib():
         L.orc_edge_list_header.argtypes = [ctypes.c_char_p, ctypes.c_size_t, i64p, i64p]
         L.orc_edge_list_load.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                          u32p, u32p, i32p, ctypes.POINTER(ctypes.c_int)]
+        L.orc_print_values.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
         for f in ("orc_bfs", "orc_sssp_frontier", "orc_dijkstra", "orc_pagerank",
                   "orc_pagerank_weighted", "orc_pagerank_adjoint", "orc_edge_list_header",
-                  "orc_edge_list_load"):
+                  "orc_edge_list_load", "orc_print_values"):
             getattr(L, f).restype = ctypes.c_int
         _lib = L
     return _lib
@@ -233,6 +234,16 @@ def load_edge_list_text(text: bytes):
     if rc != 0:
         raise ValueError(f"orc_edge_list_load failed ({rc})")
     return n, rp, col[:m], (w[:m] if has_w.value else None)
+
+
+def print_values(path, values):
+    """The reference programs' per-vertex fprintf loop (oracle.c
+    orc_print_values): "%d\n" for int32 values, "%f\n" for float64."""
+    a = np.ascontiguousarray(values)
+    if a.dtype not in (np.int32, np.float64):
+        raise ValueError("int32 or float64 values")
+    if lib().orc_print_values(str(path).encode(), a.ctypes.data, a.size, int(a.dtype == np.float64)) != 0:
+        raise OSError(f"cannot write {path}")
 
 
 def load_dimacs(path):

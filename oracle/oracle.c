@@ -13,6 +13,7 @@
  * header comment of each function and exercised by tests/test_oracle.py.
  */
 #include <math.h>
+#include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -404,4 +405,22 @@ int orc_edge_list_load(const char *text, size_t len, uint32_t n, uint32_t m, uin
   free(dst);
   free(wt);
   return rc;
+}
+
+/*
+ * The reference programs' output loop: GRD::print (grd.h:130-134) calling
+ * each vertex's print(), fprintf(outputFile, "%d\n", level) (BFS.cu:41-43)
+ * or fprintf(outputFile, "%f\n", rank) (PageRank.cu:42-44).  is_double
+ * selects "%f" over doubles, else "%d" over int32.  Pinned by: equality
+ * with Python's "%d" formatting and with libc snprintf("%f").
+ * Returns 0, or -1 if the file cannot be written.
+ */
+int orc_print_values(const char *path, const void *values, uint64_t count, int is_double) {
+  FILE *f = fopen(path, "w");
+  if (!f) return -1;
+  for (uint64_t i = 0; i < count; i++) {
+    if (is_double) fprintf(f, "%f\n", ((const double *)values)[i]);
+    else fprintf(f, "%d\n", ((const int32_t *)values)[i]);
+  }
+  return fclose(f) == 0 ? 0 : -1;
 }

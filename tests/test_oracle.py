@@ -315,3 +315,23 @@ def test_c_edge_list_loader_matches_python_restatement():
             assert same(oracle.load_edge_list(path), oracle.load_edge_list_text(messy))
         finally:
             os.unlink(path)
+
+
+def test_print_values_is_printf(tmp_path):
+    """oracle.print_values (the egress CPU baseline) writes what Python's
+    "%d" and libc's "%f" formatting give."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    libc.snprintf.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_double]
+    a = np.array([0, -1, 7, 2 ** 31 - 1, -2 ** 31], np.int32)
+    p = tmp_path / "i.txt"
+    oracle.print_values(p, a)
+    assert p.read_bytes() == "".join(f"{x}\n" for x in a.tolist()).encode()
+    x = np.array([0.0, -0.0, 1e-7, 0.0078125, 123.4567895, -2.5e-7])
+    oracle.print_values(p, x)
+    buf = ctypes.create_string_buffer(64)
+    want = b""
+    for v in x:
+        n = libc.snprintf(buf, 64, b"%f\n", float(v))
+        want += buf.raw[:n]
+    assert p.read_bytes() == want
