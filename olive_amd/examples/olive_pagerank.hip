@@ -12,6 +12,11 @@ struct Rank {
   void reduce(double &r) { r += fabs(delta); }
 };
 
+// |delta| of a vertex, for the device reduction of the convergence check
+struct AbsDelta {
+  __device__ inline double operator()(const Rank &r) const { return fabs(r.delta); }
+};
+
 struct Scatter {
   __device__ inline double gather(Rank src, EdgeId outdeg, Dump_Edge) { return src.rank / outdeg; }
   __device__ inline void reduce(double &acc, double x) { atomicAdd(&acc, x); }
@@ -63,7 +68,7 @@ int main(int argc, char **argv) {
     ol.edgeMap(all, Scatter());
     ol.vertexMap(all, Apply(0.85, (double)graph.vertexCount));
     updates++;
-    err = ol.vertexReduce();
+    err = ol.vertexReduce(AbsDelta());  // on the device; PageRank.cu:131 reduces on the host
     if (err < eps) break;
   }
   LOG(INFO) << "updates: " << updates << ", err: " << err << ", time: " << sw.getElapsedMillis() << "ms";

@@ -106,6 +106,65 @@ class Olive {
     return r;
   }
 
+  /**
+   * Device map-reduce: the sum over all vertices of f(val[v]), where f is a
+   * __device__ functor returning AccumValue.  The reduction runs on the GPU
+   * and only the result crosses to the host (the host vertexReduce() above
+   * copies every vertex value back, oliver.h:176-183).  The summation order
+   * is fixed, so repeated calls give identical results.
+   */
+  template <typename F>
+  AccumValue vertexReduce(F f) {
+    if (!vertexCount) return AccumValue(0);
+    const unsigned k = reduceGrid(vertexCount);
+    AccumValue *part = reinterpret_cast<AccumValue *>(reduceScratch);
+    hipLaunchKernelGGL((olive_detail::reduce_kernel<VertexValue, AccumValue, F>), dim3(k),
+                       dim3(olive_detail::kBlock), 0, stream(), vertexValues.elemsDevice, vertexCount, f, part);
+    hipLaunchKernelGGL(olive_detail::reduce_final_kernel<AccumValue>, dim3(1), dim3(olive_detail::kBlock), 0,
+                       stream(), part, k, part + k);
+    HIP_CHECK(hipGetLastError());
+    AccumValue r;
+    OLV_CHECK(olv_memcpy_d2h(&r, part + k, sizeof(AccumValue)));
+    return r;
+  }
+
+  /**
+   * Number of vertices in `s`: the queue length of a dense subset, the
+   * population count (on the device) of a sparse one.  Replaces the
+   * multi-GPU engine's getWorkqueueSize (olive.h:363-375), which copies
+   * and adds one counter per partition.
+   */
+  VertexId getWorkqueueSize(const VertexSubset &s) {
+    if (s.isDense) return s.size();
+    const uint32_t nw = s.flags.nwords;
+    if (!nw) return 0;
+    const unsigned k = reduceGrid(nw);
+    unsigned long long *part = reinterpret_cast<unsigned long long *>(reduceScratch);
+    hipLaunchKernelGGL(olive_detail::popcount_kernel, dim3(k), dim3(olive_detail::kBlock), 0, stream(),
+                       s.flags.words, nw, part);
+    hipLaunchKernelGGL(olive_detail::reduce_final_kernel<unsigned long long>, dim3(1), dim3(olive_detail::kBlock), 0,
+                       stream(), part, k, part + k);
+    HIP_CHECK(hipGetLastError());
+    unsigned long long c = 0;
+    OLV_CHECK(olv_memcpy_d2h(&c, part + k, sizeof(c)));
+    return (VertexId)c;
+  }
+
+  /** Whether `s` holds no vertex (olive.h:392-403, one flag per partition
+   *  there; one device count here). */
+  bool allVerticesInactive(const VertexSubset &s) { return getWorkqueueSize(s) == 0; }
+
+  /**
+   * f(globalId, value) for every vertex on the host, after copying the
+   * values back (olive.h:414-426).  On one GPU the global id of a vertex is
+   * its index.
+   */
+  template <typename F>
+  void vertexTransform(F f) {
+    vertexValues.persist();
+    for (VertexId v = 0; v < vertexCount; v++) f(v, vertexValues.elemsHost[v]);
+  }
+
   Olive(AccumValue accum) : defaultAccumValue(accum) { init(); }
   Olive() : defaultAccumValue(0) { init(); }
 
@@ -140,6 +199,8 @@ class Olive {
     mapQueue = reinterpret_cast<uint32_t *>(d);
     OLV_CHECK(olv_malloc(&d, sizeof(uint32_t)));
     mapCount = reinterpret_cast<uint32_t *>(d);
+    const size_t slot = sizeof(AccumValue) > 8 ? sizeof(AccumValue) : 8;
+    OLV_CHECK(olv_malloc(&reduceScratch, slot * ((size_t)cuCount * 4 + 1)));
   }
 
   inline void printVertices() {
@@ -186,6 +247,11 @@ class Olive {
 
   static hipStream_t stream() { return reinterpret_cast<hipStream_t>(olv_get_stream()); }
 
+  unsigned reduceGrid(size_t items) const {
+    const size_t want = (items + olive_detail::kBlock - 1) / olive_detail::kBlock;
+    return (unsigned)std::max<size_t>(1, std::min<size_t>(want, (size_t)cuCount * 4));
+  }
+
   unsigned grid(size_t items) const {
     const size_t want = (items + olive_detail::kBlock - 1) / olive_detail::kBlock;
     return (unsigned)std::max<size_t>(1, std::min<size_t>(want, (size_t)cuCount * 8));
@@ -198,6 +264,7 @@ class Olive {
     rowPtr = colIdx = NULL;
     buckets[0] = buckets[1] = buckets[2] = NULL;
     bucketCounts = mapQueue = mapCount = NULL;
+    reduceScratch = NULL;
     OLV_CHECK(olv_device_info(&cuCount, NULL, NULL));
   }
 
@@ -209,8 +276,10 @@ class Olive {
     if (bucketCounts) olv_free(bucketCounts);
     if (mapQueue) olv_free(mapQueue);
     if (mapCount) olv_free(mapCount);
+    if (reduceScratch) olv_free(reduceScratch);
     buckets[0] = buckets[1] = buckets[2] = NULL;
     bucketCounts = mapQueue = mapCount = NULL;
+    reduceScratch = NULL;
     vertexValues.del();
     accumulators.del();
     edgeValues.del();
@@ -248,6 +317,7 @@ class Olive {
   uint32_t *bucketCounts;
   uint32_t *mapQueue;
   uint32_t *mapCount;
+  void *reduceScratch;  // partial sums of the device reductions + the result
 
   GRD<VertexValue> vertexValues;
   GRD<AccumValue> accumulators;

@@ -162,6 +162,57 @@ __global__ void map_words_kernel(const uint32_t *words, uint32_t nwords, V *vals
     }
 }
 
+// ---- device reductions (vertexReduce with a device functor, subset sizes)
+// Partial sums per workgroup: grid-stride in a fixed order, then a tree in
+// LDS; reduce_final_kernel adds the partials in a fixed order, so a result
+// does not depend on scheduling.
+template <class V, class A, class F>
+__global__ void __launch_bounds__(kBlock) reduce_kernel(const V *__restrict__ vals, uint32_t n, F f,
+                                                        A *__restrict__ partials) {
+  __shared__ A s[kBlock];
+  A acc = A(0);
+  for (size_t v = (size_t)blockIdx.x * kBlock + threadIdx.x; v < n; v += (size_t)gridDim.x * kBlock)
+    acc += f(vals[v]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+// Number of set bits of words[0..nw) per workgroup.
+__global__ void __launch_bounds__(kBlock) popcount_kernel(const uint32_t *__restrict__ words, uint32_t nw,
+                                                          unsigned long long *__restrict__ partials) {
+  __shared__ unsigned long long s[kBlock];
+  unsigned long long acc = 0;
+  for (size_t w = (size_t)blockIdx.x * kBlock + threadIdx.x; w < nw; w += (size_t)gridDim.x * kBlock)
+    acc += __popc(words[w]);
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = s[0];
+}
+
+// out = partials[0] + ... + partials[k-1] (one workgroup).
+template <class A>
+__global__ void __launch_bounds__(kBlock) reduce_final_kernel(const A *__restrict__ partials, uint32_t k, A *out) {
+  __shared__ A s[kBlock];
+  A acc = A(0);
+  for (uint32_t i = threadIdx.x; i < k; i += kBlock) acc += partials[i];
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int o = kBlock / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = s[0];
+}
+
 }  // namespace olive_detail
 
 #endif  // OLIVE_KERNEL_H

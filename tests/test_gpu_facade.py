@@ -53,6 +53,30 @@ def test_example_bfs(name, tmp_path):
     np.testing.assert_array_equal(got, ref)
 
 
+def _helpers_expected(rp, col, s):
+    """Per BFS level: frontier size and the number of distinct out-neighbours
+    of the frontier (the touched set), then reached / level sum / max level."""
+    lv = oracle.bfs(rp, col, s).astype(np.int64)
+    reached = lv != INF
+    lines = []
+    for L in range(int(lv[reached].max()) + 1):
+        f = np.flatnonzero(lv == L)
+        nb = np.concatenate([col[rp[u]:rp[u + 1]] for u in f]) if f.size else np.zeros(0, np.uint32)
+        lines.append(f"level {L} frontier {f.size} touched {np.unique(nb).size}")
+    lines.append(f"reached {int(reached.sum())} levelsum {int(lv[reached].sum())} maxlevel {int(lv[reached].max())}")
+    return lines
+
+
+@pytest.mark.parametrize("name", ["gridGraph_15", "starGraph_1K", "disorderedGraph_9", "maxflowGraph_100"])
+def test_example_helpers(name, tmp_path):
+    """getWorkqueueSize (dense and sparse), allVerticesInactive, the device
+    vertexReduce and vertexTransform, checked against the oracle's BFS."""
+    n, rp, col, _ = _csr(name)
+    r = _run("olive_helpers", [data_file(name), "-s", "0"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines() == _helpers_expected(rp, col, 0)
+
+
 @pytest.mark.parametrize("name", DATA_FILES)
 def test_example_sssp_weighted(name, tmp_path):
     n, rp, col, w = _csr(name)
@@ -96,6 +120,9 @@ def test_example_bfs_rmat(tmp_path):
     ref = oracle.bfs(rp, col, s).astype(np.int64)
     ref[ref == INF] = -1
     np.testing.assert_array_equal(_read_ints(tmp_path / "out.txt"), ref)
+    r = _run("olive_helpers", [str(path), "-s", str(s)], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.strip().splitlines() == _helpers_expected(rp, col, s)
     r = _run("olive_pagerank", [str(path), "-max", "20", "-o", "pr.txt"], tmp_path)
     assert r.returncode == 0, r.stderr
     ref_pr, _, _ = oracle.pagerank(rp, col, 20)
