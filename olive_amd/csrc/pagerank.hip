@@ -29,7 +29,10 @@ struct PrWork {
   float *w_perm = nullptr;                                // m floats
   IterCtl *ctl = nullptr;
   IterCtl *h_ctl = nullptr;
+  double *part = nullptr;  // per-workgroup partial sums (kPartials)
 };
+
+constexpr uint32_t kPartials = 2048;
 
 void free_pr_work(olv_graph *g) {
   PrWork *w = g->pr;
@@ -39,6 +42,7 @@ void free_pr_work(olv_graph *g) {
   if (w->w_perm) (void)hipFree(w->w_perm);
   if (w->ctl) (void)hipFree(w->ctl);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
+  if (w->part) (void)hipFree(w->part);
   delete w;
   g->pr = nullptr;
 }
@@ -52,6 +56,7 @@ static int get_pr_work(olv_graph *g, bool need_wperm, PrWork **out) {
     for (auto &b : w->buf) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&b), (size_t)g->n * 4));
     OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(IterCtl)));
     OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(IterCtl), hipHostMallocDefault));
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->part), kPartials * sizeof(double)));
   }
   PrWork *w = g->pr;
   if (need_wperm && !w->w_perm) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->w_perm), std::max<size_t>(g->m, 1) * 4));
@@ -164,13 +169,23 @@ __global__ void pr_init_kernel(uint32_t n, float r0, const float *__restrict__ i
 
 __global__ void ctl_init_kernel(IterCtl *ctl, double eps) { ctl_reset(ctl, eps); }
 
-// eps = tol * sum |gbar| (single workgroup)
-__global__ void __launch_bounds__(1024) ctl_init_l1_kernel(IterCtl *ctl, const float *__restrict__ g, uint32_t n,
-                                                           double tol) {
-  __shared__ double s_red[1024 / kWave];
+// part[b] = sum |g| over workgroup b's grid-stride share
+__global__ void __launch_bounds__(256) l1_partial_kernel(const float *__restrict__ g, uint32_t n,
+                                                         double *__restrict__ part) {
+  __shared__ double s_red[256 / kWave];
   double s = 0.0;
-  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) s += fabs((double)g[i]);
-  s = block_sum<1024, double>(s, s_red);
+  for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) s += fabs((double)g[i]);
+  s = block_sum<256, double>(s, s_red);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// eps = tol * sum |gbar| from k partials, added in a fixed order (one workgroup)
+__global__ void __launch_bounds__(256) ctl_init_l1_kernel(IterCtl *ctl, const double *__restrict__ part, uint32_t k,
+                                                          double tol) {
+  __shared__ double s_red[256 / kWave];
+  double s = 0.0;
+  for (uint32_t i = threadIdx.x; i < k; i += 256) s += part[i];
+  s = block_sum<256, double>(s, s_red);
   ctl_reset(ctl, 0.0);
   if (threadIdx.x == 0) ctl->eps = tol * s;
 }
@@ -413,7 +428,11 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   // buffers 2/3 hold lambda (0/1 left free for a concurrent forward's state)
   float *lam[2] = {w->buf[2], w->buf[3]};
   OLV_HIP(hipMemcpyAsync(lam[0], dg, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
-  hipLaunchKernelGGL(ctl_init_l1_kernel, dim3(1), dim3(1024), 0, stream(), w->ctl, dg, g->n, (double)tol);
+  {
+    const uint32_t k = (uint32_t)std::min<size_t>(kPartials, ew_grid(g->n));
+    hipLaunchKernelGGL(l1_partial_kernel, dim3(k), dim3(256), 0, stream(), dg, g->n, w->part);
+    hipLaunchKernelGGL(ctl_init_l1_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, w->part, k, (double)tol);
+  }
   const uint32_t group = 8;
   uint32_t it = 0;
   IterResult res{0, 0.0, false};
