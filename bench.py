@@ -166,7 +166,7 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
     # the same sources through the other level drivers (same results)
     modes = {"host_loop": olv.OLV_HOST_LOOP, "persistent": olv.OLV_PERSISTENT, "push_only": olv.OLV_NO_DIRECTION_OPT}
     r["modes_ms_mean"] = {"pipeline": r["ms_mean"]}
-    for name, fl in modes.items():
+    for name, fl in ({} if args.no_modes else modes).items():
         ts = []
         for s in srcs[:4]:
             g.bfs(int(s), out=out, flags=fl)
@@ -179,7 +179,7 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
 
 
 def bench_bfs(args, olv):
-    r, csr, g, srcs = bfs_rmat22(olv, args, cpu=not args.no_cpu_baseline)
+    r, csr, g, srcs = bfs_rmat22(olv, args, cpu=not args.no_cpu_baseline, nsrc=args.sources)
     # roofline of the persistent BFS kernel (one launch per traversal)
     cfg = {"workload": "bfs_rmat22", "vertices": csr.n, "edges": csr.m, "sources": r["sources"],
            "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "bit_exact": r["bit_exact_last_source"]}
@@ -482,6 +482,8 @@ def main():
                     choices=["pagerank", "bfs", "sssp", "batched", "adjoint", "ingest", "egress", "copy"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
+    ap.add_argument("--sources", type=int, default=8, help="bfs: seeded sources timed (default 8)")
+    ap.add_argument("--no-modes", action="store_true", help="bfs: skip the host-loop/persistent/push-only drivers")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one GPU holding the whole graph; "
