@@ -87,6 +87,9 @@ int olv_init(int device);                 /* select device, query CU count   */
 int olv_set_stream(void *hip_stream);     /* NULL = the null stream          */
 void *olv_get_stream(void);
 int olv_synchronize(void);                /* wait for the current stream     */
+/* Return the device scratch blocks the library keeps between calls (ingest,
+ * egress and scan temporaries) to the device.  No reference counterpart. */
+int olv_free_cached_memory(void);
 int olv_device_info(int *cu_count, int *warp_size, size_t *total_mem);
 
 /* ---- device memory (replaces GRD<T>: grd.h:76-164) --------------------- */

@@ -87,6 +87,7 @@ SIGNATURES = {
     "olv_set_stream": (_int, [_vp]),
     "olv_get_stream": (_vp, []),
     "olv_synchronize": (_int, []),
+    "olv_free_cached_memory": (_int, []),
     "olv_device_info": (_int, [ctypes.POINTER(_int), ctypes.POINTER(_int), ctypes.POINTER(_sz)]),
     "olv_malloc": (_int, [ctypes.POINTER(_vp), _sz]),
     "olv_free": (_int, [_vp]),

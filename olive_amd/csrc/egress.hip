@@ -2,15 +2,15 @@
 //
 // * Text: one value per line in the reference programs' formats -- "%d\n"
 //   (BFS.cu:41-43, SSSP.cu:41-43) and "%f\n" (PageRank.cu:42-44) -- formatted
-//   on the device (a length pass, a scan, a write pass), copied through a
-//   pinned host buffer and written to the file.  This replaces
+//   on the device (a length pass, a scan, a write pass) and streamed to the
+//   file.  This replaces
 //   Oliver::printVertices (oliver.h:204-207: a blocking copy of every vertex
 //   value, GRD::persist grd.h:118-124, then one fprintf per vertex,
 //   grd.h:130-134).
 // * Binary: a 24-byte header (magic "OLVR", version, dtype, element size,
-//   count) and the raw values, streamed through two pinned buffers so the
-//   copy of one chunk overlaps the file write of the previous one; read back
-//   the same way.
+//   count) and the raw values.  Text and binary bytes leave the device
+//   through two 8 MB pinned buffers, so the copy of one piece overlaps the
+//   file write of the previous one; dumps are read back the same way.
 //
 // "%f" is formatted exactly as C's printf prints a double: the exact binary
 // value rounded to six decimals, ties to even, "-" for a set sign bit
@@ -28,7 +28,7 @@ namespace {
 
 constexpr int kEB = 256;
 constexpr size_t kTextChunkValues = size_t(1) << 23;   // values formatted per round
-constexpr size_t kBinChunkBytes = size_t(64) << 20;     // bytes per binary copy
+constexpr size_t kStageBytes = size_t(8) << 20;        // bytes per pinned staging buffer
 constexpr uint32_t kMaxFieldChars = 32;                 // longest line of any format
 
 unsigned grid_for(size_t items) {
@@ -177,6 +177,47 @@ struct Pinned {
   }
 };
 
+// Device bytes to a file through two pinned buffers: the copy of piece k+1
+// runs while piece k is written.
+struct Stager {
+  Pinned buf[2];
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  size_t cap = 0;
+  ~Stager() {
+    for (auto &e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+  int init(size_t bytes) {
+    cap = std::max<size_t>(bytes, 1);
+    OLV_TRY(buf[0].alloc(cap));
+    OLV_TRY(buf[1].alloc(cap));
+    for (auto &e : ev) OLV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    return OLV_OK;
+  }
+  int write(const uint8_t *src, size_t total, FILE *f, const char *who) {
+    hipStream_t s = stream();
+    int st = OLV_OK;
+    size_t off = 0, k = 0;
+    if (total && (hipMemcpyAsync(buf[0].p, src, std::min(cap, total), hipMemcpyDeviceToHost, s) != hipSuccess ||
+                  hipEventRecord(ev[0], s) != hipSuccess))
+      st = fail(OLV_EHIP, "%s: copy failed", who);
+    while (st == OLV_OK && off < total) {
+      const size_t c = std::min(cap, total - off);
+      const size_t next = off + c;
+      if (next < total && (hipMemcpyAsync(buf[(k + 1) & 1].p, src + next, std::min(cap, total - next),
+                                          hipMemcpyDeviceToHost, s) != hipSuccess ||
+                           hipEventRecord(ev[(k + 1) & 1], s) != hipSuccess))
+        st = fail(OLV_EHIP, "%s: copy failed", who);
+      if (st == OLV_OK && hipEventSynchronize(ev[k & 1]) != hipSuccess) st = fail(OLV_EHIP, "%s: copy failed", who);
+      if (st == OLV_OK && std::fwrite(buf[k & 1].p, 1, c, f) != c) st = fail(OLV_EINVAL, "%s: file write failed", who);
+      off = next;
+      ++k;
+    }
+    (void)hipStreamSynchronize(s);  // no copy into the buffers outlives a failure
+    return st;
+  }
+};
+
 struct BinHeader {
   char magic[4];
   uint32_t version, dtype, elem_bytes;
@@ -211,13 +252,13 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
   hipStream_t s = stream();
   const bool host = (flags & OLV_PTR_HOST) != 0;
   const size_t chunk = std::min(count, kTextChunkValues);
-  DevBuf staged, len, text, bad;
+  ScratchBuf staged, len, text, bad;
+  size_t text_cap = 0;
   OLV_TRY(len.alloc((chunk + 1) * 4));
-  OLV_TRY(text.alloc(chunk * kMaxFieldChars));
   OLV_TRY(bad.alloc(4));
   if (host) OLV_TRY(staged.alloc(chunk * stride));
-  Pinned hbuf;
-  OLV_TRY(hbuf.alloc(chunk * kMaxFieldChars));
+  Stager out;
+  OLV_TRY(out.init(std::min(kStageBytes, chunk * kMaxFieldChars)));
   for (size_t i0 = 0; i0 < count; i0 += chunk) {
     const size_t c = std::min(chunk, count - i0);
     const uint8_t *src = static_cast<const uint8_t *>(values) + i0 * stride;
@@ -234,6 +275,11 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
     OLV_HIP(hipMemcpy(&total, len.as<uint32_t>() + c, 4, hipMemcpyDeviceToHost));
     OLV_HIP(hipMemcpy(&b, bad.p, 4, hipMemcpyDeviceToHost));
     if (b) return fail(OLV_EUNSUPPORTED, "olv_write_text: a value of magnitude >= 2^63 cannot be formatted");
+    if (total > text_cap) {  // the text buffer is sized by the scan
+      text.reset();
+      OLV_TRY(text.alloc(total));
+      text_cap = total;
+    }
     Timer tk;
     tk.start();
     hipLaunchKernelGGL(text_write_kernel, dim3(grid_for(c)), dim3(kEB), 0, s, src, c, stride, fmt,
@@ -245,9 +291,7 @@ int olv_write_text(const char *path, const void *values, size_t count, size_t st
       st->kernel_launches += 1;
     }
     chars += total;
-    OLV_HIP(hipMemcpyAsync(hbuf.p, text.p, total, hipMemcpyDeviceToHost, s));
-    OLV_HIP(hipStreamSynchronize(s));
-    if (std::fwrite(hbuf.p, 1, total, file.f) != total) return fail(OLV_EINVAL, "olv_write_text: write to %s failed", path);
+    OLV_TRY(out.write(text.as<uint8_t>(), total, file.f, "olv_write_text"));
   }
   if (std::fclose(file.f) != 0) {
     file.f = nullptr;
@@ -278,37 +322,9 @@ int olv_write_binary(const char *path, const void *values, size_t count, size_t 
     if (total && std::fwrite(src, 1, total, file.f) != total)
       return fail(OLV_EINVAL, "olv_write_binary: write to %s failed", path);
   } else if (total) {
-    // two pinned buffers: the copy of chunk k+1 runs while chunk k is written
-    hipStream_t s = stream();
-    const size_t chunk = std::min(total, kBinChunkBytes);
-    Pinned buf[2];
-    OLV_TRY(buf[0].alloc(chunk));
-    OLV_TRY(buf[1].alloc(chunk));
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    for (auto &e : ev) OLV_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-    int st = OLV_OK;
-    size_t off = 0, k = 0;
-    const size_t c0 = std::min(chunk, total);
-    if (hipMemcpyAsync(buf[0].p, src, c0, hipMemcpyDeviceToHost, s) != hipSuccess || hipEventRecord(ev[0], s) != hipSuccess)
-      st = fail(OLV_EHIP, "olv_write_binary: copy failed");
-    while (st == OLV_OK && off < total) {
-      const size_t c = std::min(chunk, total - off);
-      const size_t next = off + c;
-      if (next < total) {
-        const size_t cn = std::min(chunk, total - next);
-        if (hipMemcpyAsync(buf[(k + 1) & 1].p, src + next, cn, hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipEventRecord(ev[(k + 1) & 1], s) != hipSuccess)
-          st = fail(OLV_EHIP, "olv_write_binary: copy failed");
-      }
-      if (st == OLV_OK && hipEventSynchronize(ev[k & 1]) != hipSuccess) st = fail(OLV_EHIP, "olv_write_binary: copy failed");
-      if (st == OLV_OK && std::fwrite(buf[k & 1].p, 1, c, file.f) != c)
-        st = fail(OLV_EINVAL, "olv_write_binary: write to %s failed", path);
-      off = next;
-      ++k;
-    }
-    (void)hipStreamSynchronize(s);  // no copy into the buffers outlives them
-    for (auto &e : ev) (void)hipEventDestroy(e);
-    if (st != OLV_OK) return st;
+    Stager out;
+    OLV_TRY(out.init(std::min(total, kStageBytes)));
+    OLV_TRY(out.write(src, total, file.f, "olv_write_binary"));
   }
   if (std::fclose(file.f) != 0) {
     file.f = nullptr;
@@ -341,7 +357,7 @@ int olv_read_binary(const char *path, void *values, size_t capacity_bytes, uint6
   }
   (void)current_device();
   hipStream_t s = stream();
-  const size_t chunk = std::min(std::max<size_t>(total, 1), kBinChunkBytes);
+  const size_t chunk = std::min(std::max<size_t>(total, 1), kStageBytes);
   Pinned buf[2];
   OLV_TRY(buf[0].alloc(chunk));
   OLV_TRY(buf[1].alloc(chunk));

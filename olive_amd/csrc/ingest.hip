@@ -145,7 +145,7 @@ __global__ void __launch_bounds__(kIB) scan_down_kernel(const uint32_t *in, uint
 int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n) {
   const size_t nt = std::max<size_t>(1, (n + kITile - 1) / kITile);
   if (nt >= (1ull << 32)) return fail(OLV_ERANGE, "scan of %zu items", n);
-  DevBuf sums;
+  ScratchBuf sums;
   OLV_TRY(sums.alloc((nt + 1) * 4));
   hipStream_t s = stream();
   hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)nt), dim3(kIB), 0, s, in, n, sums.as<uint32_t>());
@@ -153,8 +153,7 @@ int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n) {
   hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)nt), dim3(kIB), 0, s, in, out, n, sums.as<uint32_t>(),
                      (uint32_t)nt);
   OLV_HIP(hipGetLastError());
-  OLV_HIP(hipStreamSynchronize(s));  // `sums` is freed on return
-  return OLV_OK;
+  return OLV_OK;  // `sums` goes back to the pool after this thread's stream drains
 }
 
 namespace {
@@ -294,7 +293,7 @@ __global__ void __launch_bounds__(kIB) radix_scatter_kernel(const uint32_t *__re
 }
 
 struct SortBufs {
-  DevBuf k[2], v[2], counts;
+  ScratchBuf k[2], v[2], counts;
   int cur = 0;
   uint32_t *keys() const { return k[cur].as<uint32_t>(); }
   template <class V>
@@ -414,7 +413,7 @@ int read_word(const uint32_t *d, uint32_t *h) {
 }
 
 int max_row(const uint32_t *rp, uint32_t n, uint32_t *out) {
-  DevBuf d;
+  ScratchBuf d;
   OLV_TRY(d.alloc(4));
   OLV_TRY(fill_u32(d.as<uint32_t>(), 0u, 1));
   hipLaunchKernelGGL(max_row_kernel, dim3(grid_for(n, kIB)), dim3(kIB), 0, stream(), rp, n, d.as<uint32_t>());
@@ -427,7 +426,8 @@ int max_row(const uint32_t *rp, uint32_t n, uint32_t *out) {
 int assemble(uint32_t n, uint32_t m, DevBuf &rp, DevBuf &col, DevBuf &w, int wdtype, const uint32_t *esrc,
              bool transpose, olv_graph **out) {
   hipStream_t s = stream();
-  DevBuf inv, irp, icol, ieid, noin, esrc_buf;
+  DevBuf inv, irp, icol, ieid, noin;
+  ScratchBuf esrc_buf;
   uint32_t maxout = 0, maxin = 0;
   OLV_TRY(max_row(rp.as<uint32_t>(), n, &maxout));
   OLV_TRY(inv.alloc((size_t)n * 4));
@@ -508,7 +508,8 @@ int build_from_coo(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *
   SortBufs b;
   if (wdtype == OLV_W_NONE) {
     OLV_TRY(radix_sort<uint32_t>(src, LoadArr{dst}, m, bits_for(n), b));
-    std::swap(col.p, b.v[b.cur].p);  // the sorted targets are the column array
+    OLV_TRY(col.alloc((size_t)m * 4));  // the sorted targets are the column array
+    OLV_HIP(hipMemcpyAsync(col.p, b.vals<uint32_t>(), (size_t)m * 4, hipMemcpyDeviceToDevice, s));
   } else {
     OLV_TRY(radix_sort<uint2>(src, LoadPair{dst, w}, m, bits_for(n), b));
     hipLaunchKernelGGL(split_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, b.vals<uint2>(), m,
@@ -802,7 +803,7 @@ bool parse_header(const std::string &s, long long &a, long long &b) {
 // Device copy of the text (zero padded to whole lane chunks) and its line
 // starts: starts[k] for k < nl, starts[nl] = len + 1.
 struct Lines {
-  DevBuf text, starts;
+  ScratchBuf text, starts;
   size_t len = 0;
   uint32_t nl = 0;
   const uint8_t *t() const { return text.as<uint8_t>(); }
@@ -821,7 +822,7 @@ int split_lines(const char *text, size_t len, bool host, Lines &L) {
   L.len = len;
   const size_t nt = std::max<size_t>(1, (len + kTextTile - 1) / kTextTile);
   if (nt >= (1ull << 31)) return fail(OLV_ERANGE, "text of %zu bytes", len);
-  DevBuf cnt;
+  ScratchBuf cnt;
   OLV_TRY(cnt.alloc((nt + 1) * 4));
   hipLaunchKernelGGL(nl_count_kernel, dim3((unsigned)nt), dim3(kIB), 0, s, L.t(), len, cnt.as<uint32_t>());
   OLV_HIP(hipGetLastError());
@@ -851,7 +852,7 @@ int fetch_line(const Lines &L, uint32_t k, std::string &out) {
 }
 
 int first_content_line(const Lines &L, uint32_t *k) {
-  DevBuf d;
+  ScratchBuf d;
   OLV_TRY(d.alloc(4));
   OLV_TRY(fill_u32(d.as<uint32_t>(), 0xffffffffu, 1));
   hipLaunchKernelGGL(first_content_kernel, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, stream(), L.t(), L.s(), L.nl,
@@ -865,12 +866,12 @@ int parse_error(const char *who, unsigned long long e) {
   return fail(OLV_EINVAL, "%s: line %llu: %s", who, (e >> 4) + 1, why[std::min<unsigned long long>(e & 15, 3)]);
 }
 
-int read_err(const DevBuf &err, unsigned long long *h) {
+int read_err(const ScratchBuf &err, unsigned long long *h) {
   OLV_HIP(hipMemcpy(h, err.p, 8, hipMemcpyDeviceToHost));
   return OLV_OK;
 }
 
-int init_err(DevBuf &err) {
+int init_err(ScratchBuf &err) {
   OLV_TRY(err.alloc(8));
   const unsigned long long none = ~0ull;
   OLV_HIP(hipMemcpy(err.p, &none, 8, hipMemcpyHostToDevice));
@@ -898,7 +899,7 @@ int olv_graph_from_coo(olv_graph **out, uint32_t n, uint32_t m, const uint32_t *
   Timer t;
   t.start();
   const bool host = (flags & OLV_PTR_HOST) != 0;
-  DevBuf ds, dd, dw, flag;
+  ScratchBuf ds, dd, dw, flag;
   const uint32_t *s = src, *d = dst, *w = static_cast<const uint32_t *>(weights);
   if (host && m) {
     OLV_TRY(ds.alloc((size_t)m * 4));
@@ -943,7 +944,7 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   Lines L;
   OLV_TRY(split_lines(text, len, (flags & OLV_PTR_HOST) != 0, L));
   // content lines (code 2) numbered in order; the first is the header
-  DevBuf didx;
+  ScratchBuf didx;
   OLV_TRY(didx.alloc(((size_t)L.nl + 1) * 4));
   hipLaunchKernelGGL(line_flags_kernel, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, s, L.t(), L.s(), L.nl, 1u << 2,
                      didx.as<uint32_t>());
@@ -963,7 +964,7 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   if ((long long)ncontent - 1 != hm)
     return fail(OLV_EINVAL, "%s: header says %lld edges, found %u edge lines", who, hm, ncontent - 1);
   const uint32_t m = (uint32_t)hm;
-  DevBuf src, dst, w, mt, wr, err;
+  ScratchBuf src, dst, w, mt, wr, err;
   OLV_TRY(src.alloc((size_t)m * 4));
   OLV_TRY(dst.alloc((size_t)m * 4));
   OLV_TRY(w.alloc((size_t)m * 4));
@@ -1029,7 +1030,7 @@ int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, un
   Lines L;
   OLV_TRY(split_lines(text, len, (flags & OLV_PTR_HOST) != 0, L));
   // non-comment lines (blank or content) numbered in order
-  DevBuf bidx;
+  ScratchBuf bidx;
   OLV_TRY(bidx.alloc(((size_t)L.nl + 1) * 4));
   hipLaunchKernelGGL(line_flags_kernel, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, s, L.t(), L.s(), L.nl,
                      (1u << 1) | (1u << 2), bidx.as<uint32_t>());
@@ -1047,7 +1048,8 @@ int olv_graph_from_dimacs_text(olv_graph **out, const char *text, size_t len, un
   const uint32_t n = (uint32_t)hn, m = (uint32_t)(2 * hm);
   uint32_t hb = 0;
   OLV_TRY(read_word(bidx.as<uint32_t>() + hline, &hb));
-  DevBuf deg, rp, col, err, none;
+  ScratchBuf deg, err;
+  DevBuf rp, col, none;
   OLV_TRY(deg.alloc(((size_t)n + 1) * 4));
   OLV_TRY(fill_u32(deg.as<uint32_t>(), 0u, (size_t)n + 1));
   OLV_TRY(init_err(err));

@@ -111,6 +111,35 @@ struct DevBuf {
   T *as() const { return static_cast<T *>(p); }
 };
 
+// Device scratch blocks kept for reuse by later calls (runtime.hip): a
+// process-wide pool per device.  scratch_get returns a cached block of at
+// least `bytes` (at most twice as large, or 1 MiB larger) or a new one;
+// scratch_put waits for the calling thread's stream, then keeps the block.
+// olv_free_cached_memory() returns the cached blocks to the device.
+void *scratch_get(size_t bytes);
+void scratch_put(void *p);
+
+// RAII scratch buffer from the pool (for temporaries inside entry points).
+struct ScratchBuf {
+  void *p = nullptr;
+  ScratchBuf() = default;
+  ScratchBuf(const ScratchBuf &) = delete;
+  ScratchBuf &operator=(const ScratchBuf &) = delete;
+  ~ScratchBuf() { reset(); }
+  void reset() {
+    if (p) scratch_put(p);
+    p = nullptr;
+  }
+  int alloc(size_t bytes) {
+    reset();
+    p = scratch_get(bytes ? bytes : 4);
+    if (!p) return fail(OLV_ENOMEM, "device scratch of %zu bytes failed", bytes);
+    return OLV_OK;
+  }
+  template <class T>
+  T *as() const { return static_cast<T *>(p); }
+};
+
 // Event-timed region on the current stream.
 struct Timer {
   hipEvent_t a = nullptr, b = nullptr;

@@ -5,7 +5,10 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 #include "device.hpp"
@@ -355,6 +358,69 @@ static void graph_free(olv_graph *g) {
 
 using namespace olv;
 
+// ------------------------------------------------------------ scratch pool
+namespace olv {
+namespace {
+struct ScratchPool {
+  std::mutex mu;
+  std::multimap<size_t, std::pair<int, void *>> free_blocks;  // size -> (device, block)
+  std::unordered_map<void *, std::pair<int, size_t>> owned;    // block -> (device, size)
+};
+ScratchPool &pool() {
+  static ScratchPool *p = new ScratchPool();  // never destroyed: blocks outlive static teardown
+  return *p;
+}
+void release_cached(int device) {  // caller holds the lock
+  ScratchPool &P = pool();
+  for (auto it = P.free_blocks.begin(); it != P.free_blocks.end();) {
+    if (device >= 0 && it->second.first != device) {
+      ++it;
+      continue;
+    }
+    (void)hipFree(it->second.second);
+    P.owned.erase(it->second.second);
+    it = P.free_blocks.erase(it);
+  }
+}
+}  // namespace
+
+void *scratch_get(size_t bytes) {
+  const int dev = current_device();
+  bytes = (bytes + 255) / 256 * 256;
+  ScratchPool &P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  const size_t limit = std::max(2 * bytes, bytes + (size_t(1) << 20));
+  for (auto it = P.free_blocks.lower_bound(bytes); it != P.free_blocks.end() && it->first <= limit; ++it) {
+    if (it->second.first != dev) continue;
+    void *p = it->second.second;
+    P.free_blocks.erase(it);
+    return p;
+  }
+  void *p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    (void)hipGetLastError();
+    release_cached(dev);  // give the cached blocks back and try once more
+    if (hipMalloc(&p, bytes) != hipSuccess) {
+      (void)hipGetLastError();
+      return nullptr;
+    }
+  }
+  P.owned[p] = {dev, bytes};
+  return p;
+}
+
+void scratch_put(void *p) {
+  if (!p) return;
+  (void)hipStreamSynchronize(stream());  // no queued work of this thread still uses the block
+  ScratchPool &P = pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  auto it = P.owned.find(p);
+  if (it == P.owned.end()) return;
+  P.free_blocks.insert({it->second.second, {it->second.first, p}});
+}
+
+}  // namespace olv
+
 extern "C" {
 
 const char *olv_last_error(void) { return g_err; }
@@ -378,6 +444,13 @@ int olv_set_stream(void *s) {
 }
 
 void *olv_get_stream(void) { return reinterpret_cast<void *>(g_stream); }
+
+int olv_free_cached_memory(void) {
+  olv::ScratchPool &P = olv::pool();
+  std::lock_guard<std::mutex> lk(P.mu);
+  olv::release_cached(-1);
+  return OLV_OK;
+}
 
 int olv_synchronize(void) {
   OLV_HIP(hipStreamSynchronize(g_stream));
