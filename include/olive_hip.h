@@ -239,6 +239,16 @@ int olv_pagerank_weighted_f32(const olv_graph *g, const float *w,
                               const float *t, float damping, float tol,
                               uint32_t max_iters, float *ranks, unsigned flags,
                               olv_stats *st);
+/* Jacobian-vector product (forward mode) of the same fixed point: for a
+ * tangent (dw, dt) at the solution r,
+ *   dr = d * P^T dr + d * dP^T r + (1-d) dt,   (dP)[u][v] = dw_uv,
+ * solved like the forward with teleport s/(1-d), s = d dP^T r + (1-d) dt.
+ * <gbar, dr> equals <grad_w, dw> + <grad_t, dt> of olv_pagerank_vjp_f32
+ * (the dot-product test).  No reference counterpart; damping in [0, 1). */
+int olv_pagerank_jvp_f32(const olv_graph *g, const float *w, const float *r,
+                         const float *dw, const float *dt, float damping,
+                         float tol, uint32_t max_iters, float *dr,
+                         unsigned flags, olv_stats *st);
 /* Vector-Jacobian product of the fixed point above for L = <gbar, r>:
  *   lambda = gbar + d * P lambda          (adjoint solve on the out-CSR)
  *   grad_t = (1-d) lambda,   grad_w[e=(u,v)] = d * r_u * lambda_v.

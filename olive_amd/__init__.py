@@ -534,6 +534,16 @@ class Graph:
                                          ctypes.byref(st)), "olv_pagerank_vjp_f32")
         return gw, gt, lam, st
 
+    def pagerank_jvp(self, w: DeviceBuffer, r: DeviceBuffer, dw: DeviceBuffer, dt: DeviceBuffer, damping=0.85,
+                     tol=1e-6, max_iters=1000):
+        """Tangent dr of the weighted fixed point for (dw, dt) at r
+        (olv_pagerank_jvp_f32)."""
+        dr = DeviceBuffer(self.n, np.float32)
+        st = Stats()
+        check(lib().olv_pagerank_jvp_f32(self.h, w.c, r.c, dw.c, dt.c, damping, tol, max_iters, dr.c, 0,
+                                         ctypes.byref(st)), "olv_pagerank_jvp_f32")
+        return dr, st
+
     def transpose_host(self):
         rp = np.empty(self.n + 1, np.uint32)
         ci = np.empty(max(self.m, 1), np.uint32)

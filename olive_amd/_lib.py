@@ -122,6 +122,7 @@ SIGNATURES = {
     "olv_pagerank_f32": (_int, [_vp, _f32, _u32, _f64, _vp, _uint, _statp]),
     "olv_pagerank_weighted_f32": (_int, [_vp, _vp, _vp, _f32, _f32, _u32, _vp, _uint, _statp]),
     "olv_pagerank_vjp_f32": (_int, [_vp, _vp, _vp, _vp, _f32, _f32, _u32, _vp, _vp, _vp, _uint, _statp]),
+    "olv_pagerank_jvp_f32": (_int, [_vp, _vp, _vp, _vp, _vp, _f32, _f32, _u32, _vp, _uint, _statp]),
     "olv_bfs_batched": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),
     "olv_sssp_batched": (_int, [_vp, _vp, _u32, _vp, _vp, _uint, _statp]),
     "olv_bfs_multigraph": (_int, [ctypes.POINTER(_vp), _u32, _vp, ctypes.POINTER(_vp), _uint, _statp]),

@@ -291,6 +291,30 @@ struct Stage {
   }
 };
 
+// Fixed point r = d * P^T r + (1-d) t over the in-slot weights w->w_perm,
+// iterated from r0 = t until the L1 change drops below tol; the result is
+// left in w->buf[res->iters & 1].  t must not be w->buf[0] or w->buf[1].
+static int weighted_solve(const olv_graph *g, const SegDev &d, PrWork *w, const float *t, float damping, float tol,
+                          uint32_t max_iters, Timer &timer, IterResult *res) {
+  OLV_HIP(hipMemcpyAsync(w->buf[0], t, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
+  const uint32_t group = 8;
+  uint32_t it = 0;
+  *res = IterResult{0, 0.0, false};
+  for (; it < max_iters;) {
+    const uint32_t stop = std::min(max_iters, it + group);
+    for (; it < stop; ++it) {
+      WprOp op{g->in_col, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
+      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it));
+    }
+    timer.stop();
+    OLV_TRY(read_ctl(w));
+    *res = iter_result(*w->h_ctl, it);
+    if (res->converged) break;
+  }
+  return OLV_OK;
+}
+
 }  // namespace olv
 
 using namespace olv;
@@ -363,22 +387,8 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   Timer t;
   t.start();
   if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
-  OLV_HIP(hipMemcpyAsync(w->buf[0], dt, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
-  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
-  const uint32_t group = 8;
-  uint32_t it = 0;
-  IterResult res{0, 0.0, false};
-  for (; it < max_iters;) {
-    const uint32_t stop = std::min(max_iters, it + group);
-    for (; it < stop; ++it) {
-      WprOp op{g->in_col, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], dt, damping, 1.0f - damping};
-      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it));
-    }
-    t.stop();
-    OLV_TRY(read_ctl(w));
-    res = iter_result(*w->h_ctl, it);
-    if (res.converged) break;
-  }
+  IterResult res;
+  OLV_TRY(weighted_solve(g, d, w, dt, damping, tol, max_iters, t, &res));
   const double ms = t.ms();
   const uint32_t iters = res.iters;
   OLV_HIP(hipMemcpyAsync(r, w->buf[iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
@@ -396,6 +406,62 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   if (!res.converged)
     return fail(OLV_ENOCONV, "olv_pagerank_weighted_f32: residual %.3e after %u iterations (tol %.3e)", res.residual,
                 iters, (double)tol);
+  return OLV_OK;
+}
+
+int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ranks, const float *dwts,
+                         const float *dtele, float damping, float tol, uint32_t max_iters, float *dranks, unsigned flags,
+                         olv_stats *st) {
+  if (!cg || !wts || !ranks || !dwts || !dtele || !dranks) return fail(OLV_EINVAL, "olv_pagerank_jvp_f32: NULL argument");
+  if (!(damping >= 0.0f && damping < 1.0f)) return fail(OLV_EINVAL, "olv_pagerank_jvp_f32: damping must be in [0, 1)");
+  olv_graph *g = const_cast<olv_graph *>(cg);
+  OLV_TRY(ensure_plan(g, true));
+  const SegDev d = make_seg_dev(g->plan_in);
+  PrWork *w = nullptr;
+  OLV_TRY(get_pr_work(g, true, &w));
+  const bool host = (flags & OLV_PTR_HOST) != 0;
+  int s = OLV_OK;
+  Stage sw, sr, sdw, sdt, so;
+  const float *dw = sw.in(wts, g->m, host, &s);
+  OLV_TRY(s);
+  const float *dr_in = sr.in(ranks, g->n, host, &s);
+  OLV_TRY(s);
+  const float *ddw = sdw.in(dwts, g->m, host, &s);
+  OLV_TRY(s);
+  const float *ddt = sdt.in(dtele, g->n, host, &s);
+  OLV_TRY(s);
+  float *out = so.out(dranks, g->n, host, &s);
+  OLV_TRY(s);
+  Timer t;
+  t.start();
+  // source term s = d * dP^T r + (1-d) dt: one forward iteration with the
+  // weights dw, applied to r, and the teleport dt
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), ddw, g->in_eid, g->m, w->w_perm);
+  hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, 0.0);
+  WprOp src_op{g->in_col, w->w_perm, dr_in, w->buf[2], ddt, damping, 1.0f - damping};
+  OLV_TRY(enqueue_iteration(d, g->in_row_ptr, src_op, w, 0));
+  // dr = d * P^T dr + s, i.e. the forward fixed point with teleport s / (1-d)
+  hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), w->buf[2], 1.0f / (1.0f - damping),
+                     g->n, w->buf[3]);
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
+  IterResult res;
+  OLV_TRY(weighted_solve(g, d, w, w->buf[3], damping, tol, max_iters, t, &res));
+  const double ms = t.ms();
+  OLV_HIP(hipMemcpyAsync(out, w->buf[res.iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  if (host) OLV_HIP(hipMemcpy(dranks, out, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+  else OLV_HIP(hipStreamSynchronize(stream()));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = ms;
+    st->iterations = res.iters;
+    st->residual = res.residual;
+    st->edges_traversed = (uint64_t)g->m * (res.iters + 1);
+    st->edges_inspected = st->edges_traversed;
+    st->vertices_reached = g->n;
+  }
+  if (!res.converged)
+    return fail(OLV_ENOCONV, "olv_pagerank_jvp_f32: residual %.3e after %u iterations (tol %.3e)", res.residual,
+                res.iters, (double)tol);
   return OLV_OK;
 }
 

@@ -161,3 +161,57 @@ def test_vjp_dense_scale12(olv):
     gw_ref = 0.85 * r_ref[src] * lam_ref[csr.col.astype(np.int64)]
     assert np.linalg.norm(gw.numpy() - gw_ref) / np.linalg.norm(gw_ref) <= 1e-4
     assert np.linalg.norm(gt.numpy() - 0.15 * lam_ref) / np.linalg.norm(0.15 * lam_ref) <= 1e-4
+
+
+def test_jvp_dense_and_dot_product(olv):
+    """Forward mode: the tangent dr for (dw, dt) against the dense fp64
+    tangent solve (I - d P^T) dr = d dP^T r + (1-d) dt, and the dot-product
+    test <gbar, J (dw, dt)> = <J^T gbar, (dw, dt)> with the VJP."""
+    csr, t = weighted_case(olv, 12, 24)
+    n = csr.n
+    rng = np.random.default_rng(5)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    T = olv.DeviceBuffer.from_numpy(t)
+    r, _ = g.pagerank_weighted(W, T, tol=1e-8)
+    gbar = rng.standard_normal(n).astype(np.float32)
+    gw, gt, _, _ = g.pagerank_vjp(W, r, olv.DeviceBuffer.from_numpy(gbar), tol=1e-7)
+    src = np.repeat(np.arange(n), csr.outdeg())
+    dst = csr.col.astype(np.int64)
+    P = np.zeros((n, n))
+    np.add.at(P, (src, dst), csr.weights.astype(np.float64))
+    r_ref = np.linalg.solve(np.eye(n) - 0.85 * P.T, 0.15 * t.astype(np.float64))
+    for k in range(3):
+        dw = rng.standard_normal(csr.m).astype(np.float32)
+        dt = rng.standard_normal(n).astype(np.float32)
+        dr, st = g.pagerank_jvp(W, r, olv.DeviceBuffer.from_numpy(dw), olv.DeviceBuffer.from_numpy(dt), tol=1e-7)
+        dP = np.zeros((n, n))
+        np.add.at(dP, (src, dst), dw.astype(np.float64))
+        rhs = 0.85 * dP.T @ r_ref + 0.15 * dt.astype(np.float64)
+        dr_ref = np.linalg.solve(np.eye(n) - 0.85 * P.T, rhs)
+        got = dr.numpy().astype(np.float64)
+        rel = np.linalg.norm(got - dr_ref) / np.linalg.norm(dr_ref)
+        lhs = float(gbar.astype(np.float64) @ got)
+        rhs_dot = float(gw.numpy().astype(np.float64) @ dw + gt.numpy().astype(np.float64) @ dt)
+        say(f"  jvp {k}: iters {st.iterations} rel {rel:.2e} dot {lhs:.6e} vs {rhs_dot:.6e}")
+        assert rel <= 1e-4
+        assert abs(lhs - rhs_dot) <= 1e-4 * max(abs(lhs), abs(rhs_dot))
+
+
+def test_jvp_against_central_differences(olv):
+    """dr against fp64 central differences of the oracle's fixed point."""
+    csr, t = weighted_case(olv, 12, 6)
+    rng = np.random.default_rng(7)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    r, _ = g.pagerank_weighted(W, olv.DeviceBuffer.from_numpy(t), tol=1e-8)
+    w64, t64 = csr.weights.astype(np.float64), t.astype(np.float64)
+    dw = rng.standard_normal(csr.m).astype(np.float32)
+    dt = rng.standard_normal(csr.n).astype(np.float32)
+    dr, _ = g.pagerank_jvp(W, r, olv.DeviceBuffer.from_numpy(dw), olv.DeviceBuffer.from_numpy(dt), tol=1e-7)
+    h = 1e-5
+    f = lambda s: oracle.pagerank_weighted(csr.row_ptr, csr.col, w64 + s * dw, t64 + s * dt, tol=1e-15)[0]
+    fd = (f(h) - f(-h)) / (2 * h)
+    rel = np.linalg.norm(dr.numpy().astype(np.float64) - fd) / np.linalg.norm(fd)
+    say(f"  jvp vs fd rel {rel:.2e}")
+    assert rel <= 1e-4
