@@ -291,13 +291,37 @@ def bench_batched(args, olv):
     t_plain_multi = time.perf_counter() - t0
     L.olv_plan_destroy(uplan)
     L.olv_graph_destroy(uni)
+    # 16-source SSSP on the uniform 2^22 x 2^25 graph, batched vs plain
+    log("generating uniform 2^22 x 2^25 ...")
+    ucsr = olv.with_weights_i32(olv.gen_uniform(1 << 22, 1 << 25, seed=1), 1, 64, seed=1)
+    ug = olv.Graph(ucsr, transpose=False)
+    ssrc = sources_of(ucsr, 16, 5)
+    sout, sst = ug.sssp_batched(ssrc)
+    ts = []
+    for _ in range(max(3, args.steps // 4)):
+        sout, sst = ug.sssp_batched(ssrc, out=sout)
+        ts.append(sst.elapsed_ms / 1e3)
+    t_sssp_b = float(np.median(ts))
+    one = olv.DeviceBuffer(ucsr.n, np.int32)
+    ug.sssp(int(ssrc[0]), out=one)
+    t_sssp_p = 0.0
+    sexact = True
+    sb = sout.numpy().reshape(16, ucsr.n)
+    for i, s_ in enumerate(ssrc):
+        _, s1 = ug.sssp(int(s_), out=one)
+        t_sssp_p += s1.elapsed_ms / 1e3
+        if i in (0, 15):
+            sexact = sexact and bool(np.array_equal(sb[i], one.numpy()))
+    del sout, one
     cfg = {"workload": "batched_bfs_rmat22_256src_plus_64_graphs", "vertices": csr.n, "edges": csr.m,
            "sources": 256, "bit_exact_sampled": bool(exact and uexact),
            "multigraph_vertices": int(offs[64]), "multigraph_edges": int(sum(c.m for c in csrs))}
     res = base_line(args, cfg, er_tot / t_batched / 1e9, "GTEPS", t_batched * 1e3, "int32")
     res["detail"] = {"batched_256_ms": t_batched * 1e3, "plain_256_ms": t_plain * 1e3,
                      "speedup_256": t_plain / t_batched, "multigraph_ms": t_union * 1e3,
-                     "plain_64_graphs_ms": t_plain_multi * 1e3, "speedup_64_graphs": t_plain_multi / t_union}
+                     "plain_64_graphs_ms": t_plain_multi * 1e3, "speedup_64_graphs": t_plain_multi / t_union,
+                     "sssp_batched_16_uniform_ms": t_sssp_b * 1e3, "sssp_plain_16_uniform_ms": t_sssp_p * 1e3,
+                     "speedup_sssp_16": t_sssp_p / t_sssp_b, "sssp_batched_equals_plain": sexact}
     return res
 
 

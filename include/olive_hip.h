@@ -272,8 +272,9 @@ int olv_bfs_multisource(const olv_graph *g, const uint32_t *srcs, uint32_t k,
  * = levels from srcs[s]. */
 int olv_bfs_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
                     int32_t *levels, unsigned flags, olv_stats *st);
-/* k-source SSSP as k concurrent instances in one persistent launch (equal to
- * k runs of olv_sssp); dist is k x n. */
+/* k-source SSSP as k instances of the graph (virtual vertex s*n + v)
+ * relaxed together in every round of the level pipeline (equal to k runs of
+ * olv_sssp); dist is k x n. */
 int olv_sssp_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
                      const int32_t *weights, int32_t *dist, unsigned flags,
                      olv_stats *st);
