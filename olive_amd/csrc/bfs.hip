@@ -113,16 +113,23 @@ static int get_work(const olv_graph *cg, BfsWork **out) {
 // are pre-claimed (visited byte and snapshot bit) when bottom-up steps may
 // run -- no parent can claim them, so bottom-up steps skip them;
 // bfs_seed_kernel releases the seeds among them -- else nothing is.
+// With a single seed (`seed1`, device) the seeding is done here as well:
+// the seed's level, visited byte and snapshot bit are written by the lanes
+// that own them, and workgroup 0 builds the level-0 frontier after clearing
+// the control block -- one launch instead of two.
 __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint8_t *__restrict__ visited,
                                  uint32_t *__restrict__ snap, const uint32_t *__restrict__ no_in,
                                  uint32_t *__restrict__ bms, uint32_t nwp, BfsCtl *__restrict__ ctl,
-                                 GridBar *__restrict__ bar) {
+                                 GridBar *__restrict__ bar, const uint32_t *__restrict__ seed1, BfsArgs a,
+                                 unsigned long long m) {
   const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t nw = ((size_t)n + 31) / 32;
-  for (size_t i = tid; i < n; i += stride) levels[i] = kInf;
+  const uint32_t s = seed1 ? *seed1 : 0xffffffffu;
+  for (size_t i = tid; i < n; i += stride) levels[i] = i == s ? 0 : kInf;
   for (size_t i = tid; i < nwp; i += stride) {
-    const uint32_t d = (no_in && i < nw) ? no_in[i] : 0u;
+    uint32_t d = (no_in && i < nw) ? no_in[i] : 0u;
+    if (s != 0xffffffffu && i == (s >> 5)) d |= 1u << (s & 31);  // the seed is claimed
     snap[i] = d;
     uint32_t b[8];
 #pragma unroll
@@ -135,8 +142,19 @@ __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint8
     o[1] = make_uint4(b[4], b[5], b[6], b[7]);
   }
   for (size_t i = tid; i < (size_t)nwp * 2; i += stride) bms[i] = 0u;
-  for (size_t i = tid; i < sizeof(BfsCtl) / 4; i += stride) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
-  for (size_t i = tid; i < sizeof(GridBar) / 4; i += stride) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
+  if (blockIdx.x != 0) return;
+  for (uint32_t i = threadIdx.x; i < sizeof(BfsCtl) / 4; i += blockDim.x) reinterpret_cast<uint32_t *>(ctl)[i] = 0u;
+  for (uint32_t i = threadIdx.x; i < sizeof(GridBar) / 4; i += blockDim.x) reinterpret_cast<uint32_t *>(bar)[i] = 0u;
+  if (!seed1) return;
+  __syncthreads();
+  unsigned long long deg = 0;
+  seed_frontier(seed1, 1u, a.row_ptr, a.hub_thr, a.queue, &ctl->qfill[0], a.hubs, &ctl->hfill[0],
+                [&](uint32_t) -> uint32_t { return s; }, IdRow(), &deg);
+  if (threadIdx.x == 0) {
+    ctl->st[0].mu = m > deg ? m - deg : 0ull;
+    ctl->st[0].mode = 0;
+    ctl->st[0].nbu = 0;
+  }
 }
 
 // One workgroup: distinct seeds become level 0 and the level-0 frontier.
@@ -358,10 +376,13 @@ int bfs_prepare(const olv_graph *g, bool can_bu, BfsWork **w) {
 int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, uint32_t k, int32_t *levels,
                      bool can_bu) {
   const uint32_t *no_in = can_bu ? g->no_in_bm : nullptr;
+  const BfsArgs a = make_args(g, w, levels);
   hipLaunchKernelGGL(bfs_reset_kernel, dim3(grid_for(std::max<size_t>(g->n, (size_t)w->nwp * 2), 256, 8)), dim3(256),
-                     0, stream(), levels, g->n, w->visited, w->snap, no_in, w->bms, w->nwp, w->ctl, w->bar);
-  hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, make_args(g, w, levels),
-                     (unsigned long long)g->m, w->ctl);
+                     0, stream(), levels, g->n, w->visited, w->snap, no_in, w->bms, w->nwp, w->ctl, w->bar,
+                     k == 1 ? d_seeds : nullptr, a, (unsigned long long)g->m);
+  if (k != 1)
+    hipLaunchKernelGGL(bfs_seed_kernel, dim3(1), dim3(256), 0, stream(), d_seeds, k, a, (unsigned long long)g->m,
+                       w->ctl);
   OLV_HIP(hipGetLastError());
   return OLV_OK;
 }
