@@ -67,80 +67,32 @@ class CsrGraph {
 
   /**
    * Edge-list file: lines starting with '#' are comments; the first other
-   * line is "n m"; then m lines "src dst [value]".
+   * line is "n m"; then m lines "src dst [value]".  The file is parsed and
+   * stable-sorted by source on the GPU (olv_graph_load_file) and the CSR
+   * copied into this host graph; an unreadable or malformed file logs the
+   * library's message and leaves the graph empty (csrGraph.h:107-111 of the
+   * reference logs and returns too).
    */
   void fromEdgeListFile(const char *path) {
-    FILE *file = fopen(path, "r");
-    if (file == NULL) {
-      LOG(ERROR) << "Can not open graph file: " << path;
-      return;
-    }
     Stopwatch sw;
     sw.start();
-    std::vector<char> buf;
-    {
-      char chunk[1 << 16];
-      size_t r;
-      while ((r = fread(chunk, 1, sizeof(chunk), file)) > 0) buf.insert(buf.end(), chunk, chunk + r);
-    }
-    fclose(file);
-    buf.push_back('\0');
-    long long n = -1, m = -1;
-    std::vector<VertexId> src, dst;
-    std::vector<long long> val;
-    bool allValues = true;
-    char *p = buf.data();
-    while (*p) {
-      char *eol = strchr(p, '\n');
-      if (eol) *eol = '\0';
-      char *s = p;
-      while (*s == ' ' || *s == '\t' || *s == '\r') ++s;
-      if (*s && *s != '#') {
-        char *q = s;
-        long long a = strtoll(q, &q, 10), b = strtoll(q, &q, 10);
-        if (n < 0) {
-          n = a;
-          m = b;
-          src.reserve((size_t)m);
-          dst.reserve((size_t)m);
-        } else {
-          char *before = q;
-          long long c = strtoll(q, &q, 10);
-          const bool has = q != before;
-          src.push_back((VertexId)a);
-          dst.push_back((VertexId)b);
-          val.push_back(has ? c : 0);
-          allValues = allValues && has;
-        }
-      }
-      if (!eol) break;
-      p = eol + 1;
-    }
-    if (n <= 0 || (long long)src.size() != m) {
-      LOG(ERROR) << "Malformed edge list " << path << ": header n=" << n << " m=" << m << ", " << src.size()
-                 << " edges";
+    olv_graph *g = NULL;
+    if (olv_graph_load_file(&g, path, OLV_FMT_EDGE_LIST, 0, NULL) != OLV_OK) {
+      LOG(ERROR) << olv_last_error();
       return;
     }
+    uint32_t n = 0, m = 0;
+    OLV_CHECK(olv_graph_counts(g, &n, &m));
     initGraph((VertexId)n, (EdgeId)m);
-    // stable counting sort by source
-    std::vector<EdgeId> count((size_t)n + 1, 0);
-    for (size_t e = 0; e < src.size(); ++e) {
-      if (src[e] >= (VertexId)n || dst[e] >= (VertexId)n) {
-        LOG(ERROR) << "Vertex id out of range in " << path;
-        release();
-        return;
-      }
-      count[src[e] + 1]++;
-    }
-    for (size_t v = 0; v < (size_t)n; ++v) count[v + 1] += count[v];
-    for (size_t v = 0; v <= (size_t)n; ++v) vertices[v] = count[v];
-    for (size_t e = 0; e < src.size(); ++e) {
-      const EdgeId pos = count[src[e]]++;
-      edges[pos] = dst[e];
-      if constexpr (std::is_arithmetic<EdgeValue>::value)
-        if (allValues) edgeValues[pos] = (EdgeValue)val[e];
-    }
-    hasEdgeValues = std::is_arithmetic<EdgeValue>::value && allValues && m > 0;
+    std::vector<int32_t> w(m ? m : 1);
+    int wd = OLV_W_NONE;
+    OLV_CHECK(olv_graph_csr_to_host(g, reinterpret_cast<uint32_t *>(vertices), reinterpret_cast<uint32_t *>(edges),
+                                    w.data(), &wd));
+    OLV_CHECK(olv_graph_destroy(g));
+    hasEdgeValues = std::is_arithmetic<EdgeValue>::value && wd == OLV_W_I32 && m > 0;
+    if constexpr (std::is_arithmetic<EdgeValue>::value)
+      if (hasEdgeValues)
+        for (uint32_t e = 0; e < m; ++e) edgeValues[e] = (EdgeValue)w[e];
     LOG(INFO) << "Parsed " << path << ": " << n << " vertices, " << m << " edges in " << sw.getElapsedMillis()
               << "ms";
   }
