@@ -6,6 +6,9 @@ Forward:  r = d * P^T r + (1 - d) * t,  P[u, v] = w[e] for edge e = (u, v),
 Backward: lambda = gbar + d * P lambda (the adjoint fixed point, solved on the
           out-CSR), then dL/dt = (1 - d) * lambda and
           dL/dw[e] = d * r[u] * lambda[v] for edge e = (u, v).
+Forward mode (torch.autograd.forward_ad): the tangent
+          dr = d * P^T dr + d * dP^T r + (1 - d) * dt
+          (olv_pagerank_jvp_f32), for tangents of w and/or t.
 
 The gradient is taken with respect to the weights as given (no chain rule
 through a per-source normalisation the caller may have applied).  With
@@ -79,6 +82,7 @@ def _make_function():
             ctx.graph = graph
             ctx.hyper = (damping, tol, max_iters)
             ctx.save_for_backward(w, r)
+            ctx.save_for_forward(w, r)
             return r
 
         @staticmethod
@@ -95,6 +99,20 @@ def _make_function():
                                                  max_iters, gw.data_ptr(), gt.data_ptr(), lam.data_ptr(), 0, None),
                       "olv_pagerank_vjp_f32")
             return None, gw, gt, None, None, None
+
+        @staticmethod
+        def jvp(ctx, _graph_t, dw, dt, *_):
+            w, r = ctx.saved_tensors
+            graph = ctx.graph
+            damping, tol, max_iters = ctx.hyper
+            dw = torch.zeros_like(w) if dw is None else dw.detach().to(torch.float32).contiguous()
+            dt = torch.zeros_like(r) if dt is None else dt.detach().to(torch.float32).contiguous()
+            dr = torch.empty_like(r)
+            with _torch_stream(r.device):
+                check(lib().olv_pagerank_jvp_f32(graph.h, w.data_ptr(), r.data_ptr(), dw.data_ptr(), dt.data_ptr(),
+                                                 damping, tol, max_iters, dr.data_ptr(), 0, None),
+                      "olv_pagerank_jvp_f32")
+            return dr
 
     return _PageRank
 

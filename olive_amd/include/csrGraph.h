@@ -99,61 +99,23 @@ class CsrGraph {
 
   /**
    * DIMACS/METIS adjacency file: header "n m" (m undirected edges, 2m
-   * arcs), then line i lists the 1-based neighbours of vertex i-1.
+   * arcs), then line i lists the 1-based neighbours of vertex i-1.  Parsed
+   * on the GPU (olv_graph_load_file); a malformed file (the neighbour count
+   * is not 2m: csrGraph.h:229-232 of the reference asserts) logs the
+   * library's message and leaves the graph empty.
    */
   void fromDimacsFile(const char *path) {
-    FILE *file = fopen(path, "r");
-    if (file == NULL) {
-      LOG(ERROR) << "Can not open graph file: " << path;
+    olv_graph *g = NULL;
+    if (olv_graph_load_file(&g, path, OLV_FMT_DIMACS, 0, NULL) != OLV_OK) {
+      LOG(ERROR) << olv_last_error();
       return;
     }
-    std::vector<std::vector<VertexId>> rows;
-    long long n = -1, m = -1;
-    char *line = NULL;
-    size_t cap = 0;
-    ssize_t len;
-    while ((len = getline(&line, &cap, file)) >= 0) {
-      char *s = line;
-      while (*s == ' ' || *s == '\t') ++s;
-      if (*s == '#') continue;
-      if (n < 0) {
-        if (*s == '\n' || *s == '\0') continue;
-        char *q = s;
-        n = strtoll(q, &q, 10);
-        m = strtoll(q, &q, 10);
-        continue;
-      }
-      std::vector<VertexId> r;
-      char *q = s;
-      for (;;) {
-        char *before = q;
-        long long x = strtoll(q, &q, 10);
-        if (q == before) break;
-        r.push_back((VertexId)(x - 1));
-      }
-      rows.push_back(r);
-      if ((long long)rows.size() == n) break;
-    }
-    free(line);
-    fclose(file);
-    if (n <= 0) {
-      LOG(ERROR) << "Malformed DIMACS file " << path;
-      return;
-    }
-    rows.resize((size_t)n);
-    size_t total = 0;
-    for (auto &r : rows) total += r.size();
-    if ((long long)total != 2 * m) {
-      LOG(ERROR) << total << " != " << 2 * m;
-      assert(0);
-    }
-    initGraph((VertexId)n, (EdgeId)total);
-    EdgeId e = 0;
-    for (size_t v = 0; v < (size_t)n; ++v) {
-      vertices[v] = e;
-      for (VertexId d : rows[v]) edges[e++] = d;
-    }
-    vertices[n] = e;
+    uint32_t n = 0, m = 0;
+    OLV_CHECK(olv_graph_counts(g, &n, &m));
+    initGraph((VertexId)n, (EdgeId)m);
+    OLV_CHECK(olv_graph_csr_to_host(g, reinterpret_cast<uint32_t *>(vertices), reinterpret_cast<uint32_t *>(edges),
+                                    NULL, NULL));
+    OLV_CHECK(olv_graph_destroy(g));
   }
 
   /** Degree histogram (and the adjacency when verbose). */

@@ -105,3 +105,39 @@ def test_autograd_rejects_bad_input(olv):
         autograd.pagerank(g, torch.tensor(csr.weights), torch.tensor(t, device=dev))
     with pytest.raises(ValueError):
         autograd.pagerank(g, torch.tensor(csr.weights[:-1], device=dev), torch.tensor(t, device=dev))
+
+
+def test_autograd_forward_mode(olv):
+    """torch.autograd.forward_ad through the wrapper: the tangent of r for
+    tangents of w and t against the dense fp64 tangent solve, and a tangent
+    of t alone (w's tangent zero)."""
+    import torch.autograd.forward_ad as fwAD
+
+    from olive_amd import autograd
+
+    csr, t = _case(olv, 11, 25)
+    n = csr.n
+    rng = np.random.default_rng(3)
+    g = olv.Graph(csr)
+    dw = rng.standard_normal(csr.m).astype(np.float32)
+    dt = rng.standard_normal(n).astype(np.float32)
+    src = np.repeat(np.arange(n), csr.outdeg())
+    dst = csr.col.astype(np.int64)
+    P = np.zeros((n, n))
+    np.add.at(P, (src, dst), csr.weights.astype(np.float64))
+    dP = np.zeros((n, n))
+    np.add.at(dP, (src, dst), dw.astype(np.float64))
+    M = np.eye(n) - 0.85 * P.T
+    r_ref = np.linalg.solve(M, 0.15 * t.astype(np.float64))
+    w_t = torch.tensor(csr.weights, device="cuda")
+    t_t = torch.tensor(t, device="cuda")
+    with fwAD.dual_level():
+        r = autograd.pagerank(g, fwAD.make_dual(w_t, torch.tensor(dw, device="cuda")),
+                              fwAD.make_dual(t_t, torch.tensor(dt, device="cuda")), tol=1e-8)
+        dr = fwAD.unpack_dual(r).tangent.cpu().numpy()
+    ref = np.linalg.solve(M, 0.85 * dP.T @ r_ref + 0.15 * dt.astype(np.float64))
+    assert _rel(dr, ref) <= 1e-4
+    with fwAD.dual_level():
+        r = autograd.pagerank(g, w_t, fwAD.make_dual(t_t, torch.tensor(dt, device="cuda")), tol=1e-8)
+        dr = fwAD.unpack_dual(r).tangent.cpu().numpy()
+    assert _rel(dr, np.linalg.solve(M, 0.15 * dt.astype(np.float64))) <= 1e-4
