@@ -306,7 +306,8 @@ __global__ void __launch_bounds__(kMB) mbfs_pull_kernel(MbfsParams p, SegDev d) 
   const uint32_t *rp = p.in_row_ptr;
   uint32_t nf = 0;
   unsigned long long mf = 0, insp = 0;
-  for (uint32_t b = blockIdx.x; b < d.role_start[5]; b += gridDim.x) {
+  // rows without in-edges (listed last) can gain nothing from a pull
+  for (uint32_t b = blockIdx.x; b < d.live_end; b += gridDim.x) {
     uint32_t acc[kMW] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (b < d.role_start[1]) {
       // piece of a very long in-list: OR into N, touch T
