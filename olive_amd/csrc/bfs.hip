@@ -71,7 +71,8 @@ struct BfsWork {
   BfsCtl *h_ctl = nullptr;          // pinned mirror
   GridBar *bar = nullptr;           // persistent mode
   unsigned long long *stat = nullptr;
-  uint32_t last_levels = 0;         // depth of the previous run (chunk sizing)
+  uint32_t last_levels = 0;         // depth of the previous run (plan capture depth)
+  uint32_t recent[4] = {0, 0, 0, 0};  // depths of the last runs (first chunk = their maximum)
   bool force_bu = false;            // OLV_FORCE_BOTTOM_UP of the current run
 };
 
@@ -500,7 +501,11 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
     OLV_TRY(enqueue_persistent(g, w, out, can_bu));
     t.stop();
   } else {
-    const uint32_t chunk0 = (flags & OLV_HOST_LOOP) ? 1u : std::max(4u, w->last_levels + 1);
+    // first chunk: the deepest of the last few runs (traversals of one graph
+    // from different sources have similar depths); deeper runs continue in
+    // doubling chunks after one host check
+    const uint32_t recent_max = std::max(std::max(w->recent[0], w->recent[1]), std::max(w->recent[2], w->recent[3]));
+    const uint32_t chunk0 = (flags & OLV_HOST_LOOP) ? 1u : std::max(4u, recent_max);
     OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0, t));
   }
   const double ms = t.ms();
@@ -513,6 +518,10 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
   }
   if (w->h_ctl->overflow) return fail(OLV_EINTERNAL, "olv_bfs: more than n levels");
   w->last_levels = levels_run;
+  w->recent[3] = w->recent[2];
+  w->recent[2] = w->recent[1];
+  w->recent[1] = w->recent[0];
+  w->recent[0] = levels_run;
   if (host) OLV_HIP(hipMemcpy(levels, out, (size_t)g->n * 4, hipMemcpyDeviceToHost));
   if (st) {
     std::memset(st, 0, sizeof(*st));
