@@ -67,6 +67,7 @@ struct olv_graph {
   int32_t wmin = 0;
   double wmean = 0.0;
   uint32_t max_outdeg = 0, max_indeg = 0;
+  uint32_t sssp_rounds[4] = {0, 0, 0, 0};  // rounds of the last SSSP runs (first chunk = their maximum)
   float *inv_outdeg = nullptr;     // 1/outdeg (0 for sinks), fp32
   olv::Tuning tune;
   olv::SegPlan plan_in;   // buckets over the in-CSR (pull PageRank)

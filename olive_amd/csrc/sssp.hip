@@ -455,7 +455,12 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
   } else {
     const unsigned gr = (unsigned)cu_count() * 8;
     const unsigned gc = sgrid(w.nw, kSBlock);
-    uint32_t chunk = (flags & OLV_HOST_LOOP) ? 1u : 8u;
+    // first chunk: the longest of the last few runs on this graph (8 at
+    // first); longer runs continue in doubling chunks after a host check
+    olv_graph *gm = const_cast<olv_graph *>(g);
+    uint32_t hist = 8;
+    for (uint32_t r : gm->sssp_rounds) hist = std::max(hist, std::min(r, 4096u));
+    uint32_t chunk = (flags & OLV_HOST_LOOP) ? 1u : hist;
     uint64_t enq = 0;
     for (;;) {
       for (uint32_t i = 0; i < chunk; ++i) {
@@ -475,6 +480,11 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
     }
   }
   const double ms = t.ms();
+  {
+    olv_graph *gm = const_cast<olv_graph *>(g);
+    for (int i = 3; i > 0; --i) gm->sssp_rounds[i] = gm->sssp_rounds[i - 1];
+    gm->sssp_rounds[0] = w.h_ctl->rounds;
+  }
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->elapsed_ms = ms;
