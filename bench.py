@@ -100,6 +100,12 @@ def bench_pagerank(args, olv):
     out = olv.DeviceBuffer(n, np.float32)
     L = olv._lib.lib()
     st = olv.Stats()
+    # the first call also builds the in-degree relabelled in-CSR and its
+    # bucket plan (once per graph)
+    t_first = time.perf_counter()
+    olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c, 0, ctypes.byref(st)))
+    L.olv_synchronize()
+    t_first = time.perf_counter() - t_first
     for _ in range(args.warmup):
         olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c, 0, ctypes.byref(st)))
     L.olv_synchronize()
@@ -118,9 +124,23 @@ def bench_pagerank(args, olv):
         kl += st.kernel_launches
     # accuracy of the last run against the fp64 oracle (not timed)
     ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0)
-    err = float(np.max(np.abs(out.numpy().astype(np.float64) - ref) / ref))
+    got = out.numpy().copy()
+    err = float(np.max(np.abs(got.astype(np.float64) - ref) / ref))
+    # the same iteration in the graph's own vertex order (no relabelling):
+    # per-launch time for comparison, and the ranks must be bit-identical
+    gms, gl = 0.0, 0
+    if not args.no_modes:
+        for _ in range(3):
+            olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c,
+                                              olv._lib.OLV_PROFILE | olv._lib.OLV_GRAPH_ORDER, ctypes.byref(st)))
+            gms += st.kernel_ms
+            gl += st.kernel_launches
     cfg = {"workload": "pagerank_rmat20", "vertices": n, "edges": m, "iterations": iters, "damping": 0.85,
-           "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "max_rel_err_vs_fp64": err}
+           "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "max_rel_err_vs_fp64": err,
+           "first_call_ms": t_first * 1e3}
+    if gl:
+        cfg["graph_order_launch_us"] = gms / gl * 1e3
+        cfg["graph_order_bit_identical"] = bool(np.array_equal(out.numpy().view(np.uint32), got.view(np.uint32)))
     res = base_line(args, cfg, m * iters * args.steps / dt, "edges/s", dt / args.steps * 1e3, "f32")
     res["roofline"] = roofline(8 * m + 12 * n, kms / kl / 1e3, "pagerank_rmat20", "seg_iter_kernel<PrOp>")
     if not args.no_cpu_baseline:
@@ -507,7 +527,7 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-secondary", action="store_true")
     ap.add_argument("--sources", type=int, default=8, help="bfs: seeded sources timed (default 8)")
-    ap.add_argument("--no-modes", action="store_true", help="bfs: skip the host-loop/persistent/push-only drivers")
+    ap.add_argument("--no-modes", action="store_true", help="bfs: skip the host-loop/persistent/push-only drivers; pagerank: skip the graph-order comparison")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs one GPU holding the whole graph; "

@@ -51,6 +51,9 @@ enum {
                                         (slower on MI355X; kept to compare)      */
 #define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
                                         with hipEvents (olv_stats.kernel_ms)     */
+#define OLV_GRAPH_ORDER      0x0040u /* olv_pagerank_f32: iterate in the graph's own
+                                        vertex order instead of the in-degree
+                                        relabelling (same ranks, bit for bit)    */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
 
 /* weight dtypes for olv_graph_create */

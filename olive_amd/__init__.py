@@ -510,10 +510,11 @@ class Graph:
 
     # ------------------------------------------------------------- PageRank
     def pagerank(self, iters: int = 20, damping: float = 0.85, epsilon: float = 1e-7,
-                 out: DeviceBuffer | None = None):
+                 out: DeviceBuffer | None = None, flags: int = 0):
         out = out or DeviceBuffer(self.n, np.float32)
         st = Stats()
-        check(lib().olv_pagerank_f32(self.h, damping, iters, epsilon, out.c, 0, ctypes.byref(st)), "olv_pagerank_f32")
+        check(lib().olv_pagerank_f32(self.h, damping, iters, epsilon, out.c, flags, ctypes.byref(st)),
+              "olv_pagerank_f32")
         return out, st
 
     def pagerank_weighted(self, w: DeviceBuffer, t: DeviceBuffer, damping=0.85, tol=1e-6, max_iters=1000,

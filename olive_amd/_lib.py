@@ -24,6 +24,7 @@ OLV_FORCE_BOTTOM_UP = 0x4
 OLV_HOST_LOOP = 0x8
 OLV_PROFILE = 0x10
 OLV_PERSISTENT = 0x20
+OLV_GRAPH_ORDER = 0x40
 OLV_BUILD_TRANSPOSE = 0x100
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2

@@ -83,6 +83,7 @@ namespace olv {
 // Device helpers implemented in runtime.hip
 int fill_u32(uint32_t *p, uint32_t v, size_t count);
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
+int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan);  // rp: host row offsets (n + 1)
 int ensure_in_head(olv_graph *g);
 int compute_inv_outdeg(const uint32_t *rp, uint32_t n, float *inv);  // 1/outdeg (0 for sinks)
 // Device exclusive prefix sum (ingest.hip): out[0..n), out[n] = total < 2^32.

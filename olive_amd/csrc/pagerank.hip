@@ -23,8 +23,36 @@
 
 namespace olv {
 
+// The plain pull iterates over a relabelled copy of the in-CSR: vertices
+// numbered by descending in-degree (ties by id), each row's entries mapped to
+// the new numbers in their original slot order.  On a skewed graph the
+// vertices most gathered from (large out-degree, which follows in-degree
+// there) then sit together in a few cache lines of the contribution vector,
+// and the rows of one degree bucket are consecutive.  Every vertex keeps its
+// bucket and its summation order, so the ranks are bit-identical to an
+// iteration in the graph's own order.
+struct PrRelabel {
+  uint32_t *pi = nullptr;     // n: vertex -> new number
+  uint32_t *order = nullptr;  // n: new number -> vertex
+  uint32_t *rp = nullptr;     // n + 1: row offsets of the relabelled in-CSR
+  uint32_t *col = nullptr;    // m: new numbers of the in-neighbours
+  float *inv = nullptr;       // n: 1/outdeg by new number
+  SegPlan plan;
+  bool ready = false;
+  void release() {
+    void *ps[] = {pi, order, rp, col, inv};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    pi = order = rp = col = nullptr;
+    inv = nullptr;
+    plan.release();
+    ready = false;
+  }
+};
+
 struct PrWork {
   uint32_t n = 0, m = 0;
+  PrRelabel rl;
   float *buf[4] = {nullptr, nullptr, nullptr, nullptr};  // n floats each
   float *w_perm = nullptr;                                // m floats
   IterCtl *ctl = nullptr;
@@ -43,6 +71,7 @@ void free_pr_work(olv_graph *g) {
   if (w->ctl) (void)hipFree(w->ctl);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
   if (w->part) (void)hipFree(w->part);
+  w->rl.release();
   delete w;
   g->pr = nullptr;
 }
@@ -75,6 +104,72 @@ static int ensure_plan(olv_graph *g, bool in_edges) {
     OLV_TRY(build_seg_plan(g, false, g->plan_out));
     g->plan_out_ready = true;
   }
+  return OLV_OK;
+}
+
+// Row i of the relabelled in-CSR is row order[i] of the in-CSR with every
+// entry u replaced by pi[u]; one wave per row.
+__global__ void relabel_rows_kernel(const uint32_t *__restrict__ order, const uint32_t *__restrict__ pi,
+                                    const uint32_t *__restrict__ in_rp, const uint32_t *__restrict__ in_col,
+                                    const uint32_t *__restrict__ rp, uint32_t n, uint32_t *__restrict__ col) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t lane = threadIdx.x % kWave;
+  for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < n; i += waves) {
+    const uint32_t v = order[i];
+    const uint32_t b = in_rp[v], len = in_rp[v + 1] - b, o = rp[i];
+    for (uint32_t j = lane; j < len; j += kWave) col[o + j] = pi[in_col[b + j]];
+  }
+}
+
+// out[i] = in[idx[i]]
+template <class T>
+__global__ void gather_kernel(const T *__restrict__ in, const uint32_t *__restrict__ idx, uint32_t n,
+                              T *__restrict__ out) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) out[i] = in[idx[i]];
+}
+
+static unsigned ew_grid(size_t n);
+
+static int ensure_relabel(olv_graph *g, PrWork *w) {
+  PrRelabel &rl = w->rl;
+  if (rl.ready) return OLV_OK;
+  rl.release();
+  const uint32_t n = g->n, m = g->m;
+  std::vector<uint32_t> in_rp((size_t)n + 1);
+  OLV_HIP(hipMemcpy(in_rp.data(), g->in_row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  // counting sort by in-degree, descending, stable in the vertex id
+  uint32_t maxd = 0;
+  for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, in_rp[v + 1] - in_rp[v]);
+  std::vector<uint32_t> start((size_t)maxd + 2, 0);
+  for (uint32_t v = 0; v < n; ++v) ++start[maxd - (in_rp[v + 1] - in_rp[v]) + 1];
+  for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+  std::vector<uint32_t> order(n), pi(n), rp((size_t)n + 1);
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t i = start[maxd - (in_rp[v + 1] - in_rp[v])]++;
+    order[i] = v;
+    pi[v] = i;
+  }
+  rp[0] = 0;
+  for (uint32_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (in_rp[order[i] + 1] - in_rp[order[i]]);
+  const size_t nb = std::max<size_t>(n, 1) * 4;
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.pi), nb));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.order), nb));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.rp), ((size_t)n + 1) * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.col), std::max<size_t>(m, 1) * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.inv), nb));
+  OLV_HIP(hipMemcpy(rl.pi, pi.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  OLV_HIP(hipMemcpy(rl.order, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  OLV_HIP(hipMemcpy(rl.rp, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+  if (n) {
+    hipLaunchKernelGGL(relabel_rows_kernel, dim3(ew_grid((size_t)n * kWave)), dim3(256), 0, stream(), rl.order, rl.pi,
+                       g->in_row_ptr, g->in_col, rl.rp, n, rl.col);
+    hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(n)), dim3(256), 0, stream(),
+                       (const float *)g->inv_outdeg, rl.order, n, rl.inv);
+    OLV_HIP(hipGetLastError());
+  }
+  OLV_TRY(build_seg_plan_rp(rp, rl.plan));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  rl.ready = true;
   return OLV_OK;
 }
 
@@ -325,26 +420,37 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
                      unsigned flags, olv_stats *st) {
   if (!cg || !ranks) return fail(OLV_EINVAL, "olv_pagerank_f32: NULL argument");
   olv_graph *g = const_cast<olv_graph *>(cg);
-  OLV_TRY(ensure_plan(g, true));
-  const SegDev d = make_seg_dev(g->plan_in);
+  const bool relabel = (flags & OLV_GRAPH_ORDER) == 0;
+  if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+  if (!relabel) OLV_TRY(ensure_plan(g, true));
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, false, &w));
+  if (relabel) OLV_TRY(ensure_relabel(g, w));
+  const SegDev dr = make_seg_dev(relabel ? w->rl.plan : g->plan_in);
+  const uint32_t *rp = relabel ? w->rl.rp : g->in_row_ptr;
+  const uint32_t *ci = relabel ? w->rl.col : g->in_col;
+  const float *inv = relabel ? w->rl.inv : g->inv_outdeg;
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage so;
   float *r = so.out(ranks, g->n, host, &s);
   OLV_TRY(s);
+  float *ri = relabel ? w->buf[2] : r;  // iterate (new numbering when relabelled)
   KernelProfile prof;
   if (flags & OLV_PROFILE) OLV_TRY(prof.reserve(max_iters));
   Timer t;
   t.start();
-  hipLaunchKernelGGL(pr_init_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), g->n, 1.0f / (float)g->n,
-                     (const float *)g->inv_outdeg, r, w->buf[0], w->ctl, epsilon);
+  hipLaunchKernelGGL(pr_init_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), g->n, 1.0f / (float)g->n, inv, ri,
+                     w->buf[0], w->ctl, epsilon);
   const float base = (float)((1.0 - (double)damping) / (double)g->n);
   for (uint32_t it = 0; it < max_iters; ++it) {
-    PrOp op{g->in_col, w->buf[it & 1], w->buf[(it + 1) & 1], r, g->inv_outdeg, damping, base};
-    OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it, &prof));
+    PrOp op{ci, w->buf[it & 1], w->buf[(it + 1) & 1], ri, inv, damping, base};
+    OLV_TRY(enqueue_iteration(dr, rp, op, w, it, &prof));
   }
+  if (relabel && g->n)
+    hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(g->n)), dim3(256), 0, stream(), (const float *)ri, w->rl.pi,
+                       g->n, r);
+  OLV_HIP(hipGetLastError());
   t.stop();
   const double ms = t.ms();
   OLV_TRY(read_ctl(w));

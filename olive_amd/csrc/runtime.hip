@@ -289,12 +289,17 @@ void SegPlan::release() {
 
 // Group vertices by degree for the pull/push segment kernels.
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan) {
-  plan.release();
   const uint32_t n = g->n;
   const uint32_t *d_rp = in_edges ? g->in_row_ptr : g->row_ptr;
   if (!d_rp) return fail(OLV_EINVAL, "segment plan needs the %s CSR", in_edges ? "in-edge" : "out-edge");
   std::vector<uint32_t> rp((size_t)n + 1);
   OLV_HIP(hipMemcpy(rp.data(), d_rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  return build_seg_plan_rp(rp, plan);
+}
+
+int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan) {
+  plan.release();
+  const uint32_t n = (uint32_t)(rp.size() - 1);
   // lanes per vertex: 1 (deg<=b0), 8 (<=b1), 64 (<=b2), workgroup (<=chunk), chunked hubs
   plan.bounds[0] = 8;
   plan.bounds[1] = 96;

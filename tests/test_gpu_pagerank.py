@@ -78,6 +78,33 @@ def test_pagerank_bit_reproducible(olv):
     assert np.array_equal(a, b)
 
 
+def star_in_hub(olv, n):
+    """Every vertex but 0 and n-1 points at 0 (in-degree n-2: a chunked
+    row), 0 points at the even vertices below n-1, and n-1 has no edges."""
+    hub_out = np.arange(2, n - 1, 2)
+    src = np.concatenate([np.arange(1, n - 1), np.zeros(hub_out.size, np.int64)])
+    dst = np.concatenate([np.zeros(n - 2, np.int64), hub_out])
+    o = np.argsort(src, kind="stable")
+    rp = np.zeros(n + 1, np.uint32)
+    rp[1:] = np.cumsum(np.bincount(src, minlength=n))
+    return olv.CsrGraph(rp, dst[o].astype(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["rmat16", "grid", "star"])
+def test_pagerank_relabel_bit_identical(olv, case):
+    """The in-degree relabelled iteration gives the same fp32 ranks, bit for
+    bit, as the iteration in the graph's own order, and both match fp64."""
+    csr = {"rmat16": lambda: olv.gen_rmat(16, 16, seed=7), "grid": lambda: olv.gen_grid(64, 64),
+           "star": lambda: star_in_hub(olv, 20011)}[case]()
+    g = olv.Graph(csr)
+    for iters in (1, 2, 20):
+        a = g.pagerank(iters, epsilon=0.0)[0].numpy()
+        b = g.pagerank(iters, epsilon=0.0, flags=olv._lib.OLV_GRAPH_ORDER)[0].numpy()
+        assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (case, iters)
+    ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, 20, epsilon=0.0)
+    assert maxrel(a, ref) <= 1e-5
+
+
 # ------------------------------------------------ weighted + adjoint
 def weighted_case(olv, scale, seed):
     csr = olv.with_weights_f32_normalized(olv.gen_rmat(scale, 16, seed=seed), seed=seed)
