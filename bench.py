@@ -184,7 +184,8 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
          "edges": csr.m, "sources": len(srcs), "levels": [p[3] for p in per], "bottom_up_levels": [p[4] for p in per],
          "bit_exact_last_source": ok}
     # the same sources through the other level drivers (same results)
-    modes = {"host_loop": olv.OLV_HOST_LOOP, "persistent": olv.OLV_PERSISTENT, "push_only": olv.OLV_NO_DIRECTION_OPT}
+    modes = {"host_loop": olv.OLV_HOST_LOOP, "persistent": olv.OLV_PERSISTENT, "push_only": olv.OLV_NO_DIRECTION_OPT,
+             "graph_order": olv.OLV_GRAPH_ORDER}
     r["modes_ms_mean"] = {"pipeline": r["ms_mean"]}
     for name, fl in ({} if args.no_modes else modes).items():
         ts = []
@@ -304,9 +305,15 @@ def bench_batched(args, olv):
     uo = uout.numpy()
     uexact = all(np.array_equal(uo[offs[i]:offs[i + 1]], oracle.bfs(csrs[i].row_ptr, csrs[i].col, int(bsrc[i])))
                  for i in (0, 31, 63))
+    # plain engine, one graph at a time: outputs allocated and every graph
+    # run once beforehand (work buffers, renumbered twin), then timed
+    outs = [olv.DeviceBuffer(c.n, np.int32) for c in csrs]
+    for x, s, o in zip(gs, bsrc, outs):
+        x.bfs(int(s), out=o)
+    L.olv_synchronize()
     t0 = time.perf_counter()
-    for x, s in zip(gs, bsrc):
-        x.bfs(int(s))
+    for x, s, o in zip(gs, bsrc, outs):
+        x.bfs(int(s), out=o)
     L.olv_synchronize()
     t_plain_multi = time.perf_counter() - t0
     L.olv_plan_destroy(uplan)

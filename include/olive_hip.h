@@ -51,9 +51,11 @@ enum {
                                         (slower on MI355X; kept to compare)      */
 #define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
                                         with hipEvents (olv_stats.kernel_ms)     */
-#define OLV_GRAPH_ORDER      0x0040u /* olv_pagerank_f32: iterate in the graph's own
-                                        vertex order instead of the in-degree
-                                        relabelling (same ranks, bit for bit)    */
+#define OLV_GRAPH_ORDER      0x0040u /* olv_pagerank_f32, olv_bfs,
+                                        olv_bfs_multisource: run in the graph's
+                                        own vertex numbering instead of the
+                                        in-degree renumbering (same results,
+                                        bit for bit)                             */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
 
 /* weight dtypes for olv_graph_create */
@@ -215,7 +217,10 @@ int olv_read_binary(const char *path, void *values, size_t capacity_bytes,
 /* ---- algorithms -------------------------------------------------------- */
 /* BFS levels from src, INF = 0x7fffffff for unreached.  Replaces the BFS.cu
  * level loop (BFS.cu:124-148) over edgeFilter/vertexFilter
- * (oliver.h:56-113, oliverKernel.h:45-141) run to convergence. */
+ * (oliver.h:56-113, oliverKernel.h:45-141) run to convergence.  The first
+ * direction-optimising call on a graph also builds its in-degree renumbered
+ * twin (a second copy of the graph and its transpose, kept until
+ * olv_graph_destroy); later calls run on the twin. */
 int olv_bfs(const olv_graph *g, uint32_t src, int32_t *levels, unsigned flags,
             olv_stats *st);
 

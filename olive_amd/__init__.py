@@ -16,8 +16,8 @@ import ctypes
 
 import numpy as np
 
-from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FMT_DIMACS, OLV_FMT_EDGE_LIST, OLV_FORCE_BOTTOM_UP, OLV_HOST_LOOP,
-                   OLV_NO_DIRECTION_OPT, OLV_PERSISTENT, OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE,
+from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FMT_DIMACS, OLV_FMT_EDGE_LIST, OLV_FORCE_BOTTOM_UP, OLV_GRAPH_ORDER,
+                   OLV_HOST_LOOP, OLV_NO_DIRECTION_OPT, OLV_PERSISTENT, OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE,
                    HostCsr, OliveError, Stats, check, lib)
 
 INF = 0x7FFFFFFF
@@ -27,7 +27,7 @@ __all__ = [
     "gen_uniform", "gen_dirichlet", "batch_scale", "batch_seed", "load_edge_list", "load_dimacs",
     "ingest_coo", "ingest_edge_list_text", "ingest_dimacs_text", "ingest_file", "format_edge_list",
     "write_text", "write_binary", "read_binary",
-    "OLV_HOST_LOOP", "OLV_NO_DIRECTION_OPT", "OLV_FORCE_BOTTOM_UP", "OLV_PTR_HOST",
+    "OLV_HOST_LOOP", "OLV_NO_DIRECTION_OPT", "OLV_FORCE_BOTTOM_UP", "OLV_PTR_HOST", "OLV_GRAPH_ORDER",
 ]
 
 _init_done = False

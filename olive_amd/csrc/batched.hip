@@ -953,7 +953,8 @@ int olv_bfs_multigraph(olv_graph *const *gs, uint32_t k, const uint32_t *srcs, i
   for (uint32_t i = 0; i < k; ++i) seeds[i] = voff[i] + srcs[i];
   DevBuf lv;
   int s = lv.alloc((size_t)u->n * 4);
-  if (s == OLV_OK) s = bfs_multisource(u, seeds.data(), k, true, lv.as<int32_t>(), flags & ~OLV_PTR_HOST, st);
+  if (s == OLV_OK)  // the union is built per call: no renumbered twin for it
+    s = bfs_multisource(u, seeds.data(), k, true, lv.as<int32_t>(), (flags & ~OLV_PTR_HOST) | OLV_GRAPH_ORDER, st);
   if (s == OLV_OK) {
     const bool host = (flags & OLV_PTR_HOST) != 0;
     for (uint32_t i = 0; i < k && s == OLV_OK; ++i) {
@@ -985,7 +986,8 @@ int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k, int32_t *
   p->levels = levels;
   p->can_bu = !(flags & OLV_NO_DIRECTION_OPT);
   int s = bfs_prepare(g, p->can_bu, &p->bw);
-  if (s == OLV_OK) s = bfs_multisource(g, srcs, k, true, levels, flags & ~(OLV_PTR_HOST | OLV_PERSISTENT), nullptr);
+  if (s == OLV_OK)  // on g itself: its work buffers keep the depth the capture uses
+    s = bfs_multisource(g, srcs, k, true, levels, (flags & ~(OLV_PTR_HOST | OLV_PERSISTENT)) | OLV_GRAPH_ORDER, nullptr);
   if (s == OLV_OK) s = p->seeds.alloc((size_t)k * 4);
   if (s == OLV_OK && hipMemcpy(p->seeds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
     s = fail(OLV_EHIP, "olv_plan_bfs: seed copy failed");

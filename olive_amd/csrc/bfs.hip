@@ -127,7 +127,8 @@ __global__ void bfs_reset_kernel(int32_t *__restrict__ levels, uint32_t n, uint8
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   const size_t nw = ((size_t)n + 31) / 32;
   const uint32_t s = seed1 ? *seed1 : 0xffffffffu;
-  for (size_t i = tid; i < n; i += stride) levels[i] = i == s ? 0 : kInf;
+  const uint32_t so = (seed1 && a.omap) ? a.omap[s] : s;  // the seed's levels index
+  for (size_t i = tid; i < n; i += stride) levels[i] = i == so ? 0 : kInf;
   for (size_t i = tid; i < nwp; i += stride) {
     uint32_t d = (no_in && i < nw) ? no_in[i] : 0u;
     if (s != 0xffffffffu && i == (s >> 5)) d |= 1u << (s & 31);  // the seed is claimed
@@ -173,7 +174,7 @@ __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, 
     const uint32_t old = atomicOr(&a.snap[s >> 5], 1u << (s & 31));
     if (old & (1u << (s & 31))) return 0xffffffffu;
     a.visited[s] = 1;
-    a.levels[s] = 0;
+    a.set_level(s, 0);
     return s;
   };
   unsigned long long deg = 0;
@@ -340,6 +341,7 @@ static BfsArgs make_args(const olv_graph *g, BfsWork *w, int32_t *levels) {
   a.in_col = g->in_col;
   a.in_head = g->in_head;
   a.levels = levels;
+  a.omap = g->out_map;
   a.visited = w->visited;
   a.snap = w->snap;
   a.queue = w->queue;
@@ -470,19 +472,34 @@ static int collect_stats(const olv_graph *g, BfsWork *w, const int32_t *levels, 
 
 // Multi-source BFS on g (levels = distance to the nearest seed).
 // seeds_host: seeds is a host array (levels follow OLV_PTR_HOST in flags).
-int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool seeds_host, int32_t *levels,
+// Direction-optimising runs go to g's twin -- g renumbered by descending
+// in-degree, so hubs and their visited bytes sit together, and in-lists
+// start with the hubs -- whose level stores go through the twin's out_map
+// straight to g's numbering.  BFS levels do not depend on the numbering, so
+// they are the same values.  OLV_GRAPH_ORDER (or OLV_NO_DIRECTION_OPT) runs
+// on g itself.
+int bfs_multisource(const olv_graph *g0, const uint32_t *seeds, uint32_t k, bool seeds_host, int32_t *levels,
                     unsigned flags, olv_stats *st) {
-  if (!g || !seeds || !levels) return fail(OLV_EINVAL, "olv_bfs: NULL argument");
+  if (!g0 || !seeds || !levels) return fail(OLV_EINVAL, "olv_bfs: NULL argument");
   if (k == 0) return fail(OLV_EINVAL, "olv_bfs: no source");
   const bool host = (flags & OLV_PTR_HOST) != 0;
   std::vector<uint32_t> h_seeds(k);
   if (seeds_host) std::memcpy(h_seeds.data(), seeds, (size_t)k * 4);
   else OLV_HIP(hipMemcpy(h_seeds.data(), seeds, (size_t)k * 4, hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < k; ++i)
-    if (h_seeds[i] >= g->n) return fail(OLV_EINVAL, "olv_bfs: source %u out of range (n=%u)", h_seeds[i], g->n);
+    if (h_seeds[i] >= g0->n) return fail(OLV_EINVAL, "olv_bfs: source %u out of range (n=%u)", h_seeds[i], g0->n);
   const bool can_bu = !(flags & OLV_NO_DIRECTION_OPT);
   if (!can_bu && (flags & OLV_FORCE_BOTTOM_UP))
     return fail(OLV_EINVAL, "olv_bfs: OLV_FORCE_BOTTOM_UP and OLV_NO_DIRECTION_OPT together");
+  const bool twin = can_bu && !(flags & OLV_GRAPH_ORDER);
+  const olv_graph *g = g0;
+  if (twin) {
+    olv_graph *gm = const_cast<olv_graph *>(g0);
+    OLV_TRY(ensure_twin(gm));
+    gm->twin->tune = gm->tune;
+    for (uint32_t &s : h_seeds) s = gm->h_rl_pi[s];
+    g = gm->twin;
+  }
   BfsWork *w = nullptr;
   OLV_TRY(bfs_prepare(g, can_bu, &w));
   w->force_bu = (flags & OLV_FORCE_BOTTOM_UP) != 0;
@@ -529,7 +546,7 @@ int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool 
     st->iterations = levels_run;
     st->bottom_up_levels = bu_levels;
     st->edges_inspected = w->h_ctl->inspected;
-    OLV_TRY(collect_stats(g, w, out, st));
+    OLV_TRY(collect_stats(g0, w, out, st));  // levels are in g0's numbering
   }
   return OLV_OK;
 }

@@ -80,6 +80,7 @@ struct BfsArgs {
   const uint32_t *in_col;
   const uint2 *in_head;  // first two in-neighbours (0xffffffff where absent)
   int32_t *levels;
+  const uint32_t *omap;  // levels index of each vertex (renumbered twin), or nullptr: the vertex id
   uint8_t *visited;    // one byte per vertex: nonzero = claimed (padded to 32-vertex words)
   uint32_t *snap;      // bitmap of the claims as of the last frontier extraction
   uint32_t *queue;
@@ -88,6 +89,7 @@ struct BfsArgs {
   uint32_t wave_thr;   // degree >= wave_thr: one wave per vertex
   uint32_t block_thr;  // degree >= block_thr: whole workgroup per vertex
   uint32_t hub_thr;    // degree >= hub_thr: kHubChunk-edge hub items
+  __device__ __forceinline__ void set_level(uint32_t v, int32_t l) const { levels[omap ? omap[v] : v] = l; }
 };
 
 // Hub items needed for the worst case of m edges.
@@ -254,7 +256,7 @@ struct BfsVisitor {
   __device__ __forceinline__ void apply(uint32_t v, uint32_t seen, uint32_t) {
     if (seen) return;
     a.visited[v] = 1;
-    a.levels[v] = nlevel;
+    a.set_level(v, nlevel);
     nf += 1;
     mf += a.row_ptr[v + 1] - a.row_ptr[v];
   }
@@ -390,7 +392,7 @@ __device__ void bu_step(const BfsArgs &a, const uint32_t *front_bm, int32_t nlev
       const uint32_t v = (uint32_t)(base + j * stride) + lane;
       if (found[j]) {
         a.visited[v] = 1;
-        a.levels[v] = nlevel;
+        a.set_level(v, nlevel);
         nf += 1;
         mf += a.row_ptr[v + 1] - a.row_ptr[v];
       }

@@ -351,13 +351,19 @@ uint32_t bits_for(uint32_t n) {  // bits of the largest id n - 1
 }
 
 // ------------------------------------------------------------ CSR pieces
-// rp[v] = first position whose key is >= v, for sorted keys < n (rp[n] = m).
+// rp[v] = first position whose key is >= v, for sorted keys < n (rp[n] = m):
+// a binary search per vertex, so long runs of rows without entries (ids
+// that no key holds) cost no more than any other rows.
 __global__ void rows_from_sorted_kernel(const uint32_t *__restrict__ keys, uint32_t m, uint32_t n,
                                         uint32_t *__restrict__ rp) {
-  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m; i += (size_t)gridDim.x * blockDim.x) {
-    const uint32_t hi = i < m ? keys[i] : n;
-    const int64_t lo = i > 0 ? (int64_t)keys[i - 1] : -1;
-    for (int64_t v = lo + 1; v <= (int64_t)hi; ++v) rp[v] = (uint32_t)i;
+  for (size_t v = (size_t)blockIdx.x * blockDim.x + threadIdx.x; v <= n; v += (size_t)gridDim.x * blockDim.x) {
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (keys[mid] < (uint32_t)v) lo = mid + 1;
+      else hi = mid;
+    }
+    rp[v] = lo;
   }
 }
 
@@ -447,7 +453,7 @@ int assemble(uint32_t n, uint32_t m, DevBuf &rp, DevBuf &col, DevBuf &w, int wdt
       // CSR positions sorted by target, carrying (position, source)
       SortBufs b;
       OLV_TRY(radix_sort<uint2>(col.as<uint32_t>(), LoadIndexAnd{esrc}, m, bits_for(n), b));
-      hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, b.keys(), m,
+      hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)n + 1, kIB)), dim3(kIB), 0, s, b.keys(), m,
                          n, irp.as<uint32_t>());
       hipLaunchKernelGGL(split_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, b.vals<uint2>(), m,
                          ieid.as<uint32_t>(), icol.as<uint32_t>());
@@ -515,7 +521,7 @@ int build_from_coo(uint32_t n, uint32_t m, const uint32_t *src, const uint32_t *
     hipLaunchKernelGGL(split_kernel, dim3(grid_for(m, kIB)), dim3(kIB), 0, s, b.vals<uint2>(), m,
                        col.as<uint32_t>(), wout.as<uint32_t>());
   }
-  hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)m + 1, kIB)), dim3(kIB), 0, s, b.keys(), m, n,
+  hipLaunchKernelGGL(rows_from_sorted_kernel, dim3(grid_for((size_t)n + 1, kIB)), dim3(kIB), 0, s, b.keys(), m, n,
                      rp.as<uint32_t>());
   OLV_HIP(hipGetLastError());
   return assemble(n, m, rp, col, wout, wdtype, b.keys(), transpose, out);

@@ -75,6 +75,13 @@ struct olv_graph {
   bool plan_in_ready = false, plan_out_ready = false;
   olv::BfsWork *bfs = nullptr;
   olv::PrWork *pr = nullptr;
+  // In-degree numbering: vertices by descending in-degree, ties by id
+  // (built on first use; shared by the PageRank pull and the BFS twin).
+  uint32_t *rl_pi = nullptr;       // vertex -> new number
+  uint32_t *rl_order = nullptr;    // new number -> vertex
+  std::vector<uint32_t> h_rl_pi;   // host copy of rl_pi (seed mapping)
+  olv_graph *twin = nullptr;       // this graph renumbered by rl_pi (BFS runs on it)
+  const uint32_t *out_map = nullptr;  // a twin: its vertex -> the original graph's (not owned)
   uint32_t *union_map = nullptr;   // multigraph: unused here
 };
 
@@ -84,6 +91,11 @@ namespace olv {
 int fill_u32(uint32_t *p, uint32_t v, size_t count);
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
 int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan);  // rp: host row offsets (n + 1)
+// In-degree numbering of g (g->rl_pi, g->rl_order, g->h_rl_pi), built
+// once; needs the in-CSR.  order (optional) receives new number -> vertex.
+int indeg_order(olv_graph *g, std::vector<uint32_t> *order);
+// g->twin: g renumbered by its in-degree numbering, with its transpose.
+int ensure_twin(olv_graph *g);
 int ensure_in_head(olv_graph *g);
 int compute_inv_outdeg(const uint32_t *rp, uint32_t n, float *inv);  // 1/outdeg (0 for sinks)
 // Device exclusive prefix sum (ingest.hip): out[0..n), out[n] = total < 2^32.

@@ -31,19 +31,18 @@ namespace olv {
 // and the rows of one degree bucket are consecutive.  Every vertex keeps its
 // bucket and its summation order, so the ranks are bit-identical to an
 // iteration in the graph's own order.
+// (The numbering itself, g->rl_pi / g->rl_order, belongs to the graph.)
 struct PrRelabel {
-  uint32_t *pi = nullptr;     // n: vertex -> new number
-  uint32_t *order = nullptr;  // n: new number -> vertex
   uint32_t *rp = nullptr;     // n + 1: row offsets of the relabelled in-CSR
   uint32_t *col = nullptr;    // m: new numbers of the in-neighbours
   float *inv = nullptr;       // n: 1/outdeg by new number
   SegPlan plan;
   bool ready = false;
   void release() {
-    void *ps[] = {pi, order, rp, col, inv};
+    void *ps[] = {rp, col, inv};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    pi = order = rp = col = nullptr;
+    rp = col = nullptr;
     inv = nullptr;
     plan.release();
     ready = false;
@@ -117,7 +116,14 @@ __global__ void relabel_rows_kernel(const uint32_t *__restrict__ order, const ui
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < n; i += waves) {
     const uint32_t v = order[i];
     const uint32_t b = in_rp[v], len = in_rp[v + 1] - b, o = rp[i];
-    for (uint32_t j = lane; j < len; j += kWave) col[o + j] = pi[in_col[b + j]];
+    for (uint32_t j0 = lane; j0 < len; j0 += 4 * kWave) {  // four loads in flight per lane
+      uint32_t c[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) c[q] = j0 + q * kWave < len ? in_col[b + j0 + q * kWave] : 0u;
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        if (j0 + q * kWave < len) col[o + j0 + q * kWave] = pi[c[q]];
+    }
   }
 }
 
@@ -135,36 +141,21 @@ static int ensure_relabel(olv_graph *g, PrWork *w) {
   if (rl.ready) return OLV_OK;
   rl.release();
   const uint32_t n = g->n, m = g->m;
-  std::vector<uint32_t> in_rp((size_t)n + 1);
+  std::vector<uint32_t> order;
+  OLV_TRY(indeg_order(g, &order));
+  std::vector<uint32_t> in_rp((size_t)n + 1), rp((size_t)n + 1);
   OLV_HIP(hipMemcpy(in_rp.data(), g->in_row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
-  // counting sort by in-degree, descending, stable in the vertex id
-  uint32_t maxd = 0;
-  for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, in_rp[v + 1] - in_rp[v]);
-  std::vector<uint32_t> start((size_t)maxd + 2, 0);
-  for (uint32_t v = 0; v < n; ++v) ++start[maxd - (in_rp[v + 1] - in_rp[v]) + 1];
-  for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
-  std::vector<uint32_t> order(n), pi(n), rp((size_t)n + 1);
-  for (uint32_t v = 0; v < n; ++v) {
-    const uint32_t i = start[maxd - (in_rp[v + 1] - in_rp[v])]++;
-    order[i] = v;
-    pi[v] = i;
-  }
   rp[0] = 0;
   for (uint32_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (in_rp[order[i] + 1] - in_rp[order[i]]);
-  const size_t nb = std::max<size_t>(n, 1) * 4;
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.pi), nb));
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.order), nb));
   OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.rp), ((size_t)n + 1) * 4));
   OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.col), std::max<size_t>(m, 1) * 4));
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.inv), nb));
-  OLV_HIP(hipMemcpy(rl.pi, pi.data(), (size_t)n * 4, hipMemcpyHostToDevice));
-  OLV_HIP(hipMemcpy(rl.order, order.data(), (size_t)n * 4, hipMemcpyHostToDevice));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.inv), std::max<size_t>(n, 1) * 4));
   OLV_HIP(hipMemcpy(rl.rp, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
   if (n) {
-    hipLaunchKernelGGL(relabel_rows_kernel, dim3(ew_grid((size_t)n * kWave)), dim3(256), 0, stream(), rl.order, rl.pi,
-                       g->in_row_ptr, g->in_col, rl.rp, n, rl.col);
+    hipLaunchKernelGGL(relabel_rows_kernel, dim3(ew_grid((size_t)n * kWave)), dim3(256), 0, stream(), g->rl_order,
+                       g->rl_pi, g->in_row_ptr, g->in_col, rl.rp, n, rl.col);
     hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(n)), dim3(256), 0, stream(),
-                       (const float *)g->inv_outdeg, rl.order, n, rl.inv);
+                       (const float *)g->inv_outdeg, g->rl_order, n, rl.inv);
     OLV_HIP(hipGetLastError());
   }
   OLV_TRY(build_seg_plan_rp(rp, rl.plan));
@@ -448,7 +439,7 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
     OLV_TRY(enqueue_iteration(dr, rp, op, w, it, &prof));
   }
   if (relabel && g->n)
-    hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(g->n)), dim3(256), 0, stream(), (const float *)ri, w->rl.pi,
+    hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(g->n)), dim3(256), 0, stream(), (const float *)ri, g->rl_pi,
                        g->n, r);
   OLV_HIP(hipGetLastError());
   t.stop();

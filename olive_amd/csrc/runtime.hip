@@ -352,11 +352,83 @@ static void graph_free(olv_graph *g) {
   free_pr_work(g);
   g->plan_in.release();
   g->plan_out.release();
+  graph_free(g->twin);
   void *ps[] = {g->row_ptr, g->col, g->weights, g->in_row_ptr, g->in_col, g->in_eid, g->no_in_bm, g->in_head,
-                g->inv_outdeg, g->union_map};
+                g->inv_outdeg, g->union_map, g->rl_pi, g->rl_order};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   delete g;
+}
+
+int indeg_order(olv_graph *g, std::vector<uint32_t> *order_out) {
+  if (g->rl_pi && !order_out) return OLV_OK;
+  if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
+  const uint32_t n = g->n;
+  std::vector<uint32_t> in_rp((size_t)n + 1);
+  OLV_HIP(hipMemcpy(in_rp.data(), g->in_row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  // counting sort by in-degree, descending, stable in the vertex id
+  uint32_t maxd = 0;
+  for (uint32_t v = 0; v < n; ++v) maxd = std::max(maxd, in_rp[v + 1] - in_rp[v]);
+  std::vector<uint32_t> start((size_t)maxd + 2, 0);
+  for (uint32_t v = 0; v < n; ++v) ++start[maxd - (in_rp[v + 1] - in_rp[v]) + 1];
+  for (size_t k = 1; k < start.size(); ++k) start[k] += start[k - 1];
+  std::vector<uint32_t> order(n), pi(n);
+  for (uint32_t v = 0; v < n; ++v) {
+    const uint32_t i = start[maxd - (in_rp[v + 1] - in_rp[v])]++;
+    order[i] = v;
+    pi[v] = i;
+  }
+  if (!g->rl_pi) {
+    OLV_TRY(upload(&g->rl_pi, pi.data(), n));
+    OLV_TRY(upload(&g->rl_order, order.data(), n));
+    g->h_rl_pi = std::move(pi);
+  }
+  if (order_out) *order_out = std::move(order);
+  return OLV_OK;
+}
+
+// s[e] = pi[u], d[e] = pi[col[e]] for every edge e = (u, col[e]); one wave
+// per row.
+__global__ void coo_relabel_kernel(const uint32_t *__restrict__ rp, const uint32_t *__restrict__ col,
+                                   const uint32_t *__restrict__ pi, uint32_t n, uint32_t *__restrict__ s,
+                                   uint32_t *__restrict__ d) {
+  const uint32_t waves = gridDim.x * (blockDim.x / kWave);
+  const uint32_t lane = threadIdx.x % kWave;
+  for (uint32_t u = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; u < n; u += waves) {
+    const uint32_t b = rp[u], e = rp[u + 1], pu = pi[u];
+    for (uint32_t k0 = b + lane; k0 < e; k0 += 4 * kWave) {  // four loads in flight per lane
+      uint32_t c[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) c[i] = k0 + i * kWave < e ? col[k0 + i * kWave] : 0u;
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+        if (k0 + i * kWave < e) {
+          s[k0 + i * kWave] = pu;
+          d[k0 + i * kWave] = pi[c[i]];
+        }
+    }
+  }
+}
+
+int ensure_twin(olv_graph *g) {
+  if (g->twin) return OLV_OK;
+  OLV_TRY(indeg_order(g, nullptr));
+  ScratchBuf s, d;
+  OLV_TRY(s.alloc(std::max<size_t>(g->m, 1) * 4));
+  OLV_TRY(d.alloc(std::max<size_t>(g->m, 1) * 4));
+  if (g->m) {
+    hipLaunchKernelGGL(coo_relabel_kernel, dim3(fill_grid((size_t)g->n * kWave)), dim3(256), 0, stream(), g->row_ptr,
+                       g->col, g->rl_pi, g->n, s.as<uint32_t>(), d.as<uint32_t>());
+    OLV_HIP(hipGetLastError());
+  }
+  olv_graph *t = nullptr;
+  OLV_TRY(olv_graph_from_coo(&t, g->n, g->m, s.as<uint32_t>(), d.as<uint32_t>(), g->weights, g->wdtype,
+                             OLV_BUILD_TRANSPOSE, nullptr));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  t->device = g->device;
+  t->out_map = g->rl_order;
+  g->twin = t;
+  return OLV_OK;
 }
 
 }  // namespace olv

@@ -23,6 +23,12 @@ MODES = {
     "persistent_push_only": 0x20 | 0x2,
     "bottom_up": 0x4,                   # every level after the first pulls
     "persistent_bottom_up": 0x20 | 0x4,
+    # the same drivers in the graph's own numbering (the others run on the
+    # graph renumbered by descending in-degree)
+    "graph_order": 0x40,
+    "graph_order_host_loop": 0x40 | 0x8,
+    "graph_order_persistent": 0x40 | 0x20,
+    "graph_order_bottom_up": 0x40 | 0x4,
 }
 
 
@@ -100,9 +106,24 @@ def test_bfs_multisource(olv):
     csr = olv.gen_rmat(13, 8, seed=4)
     g = olv.Graph(csr)
     srcs = [3, 100, 5000, 3]
-    out, st = g.bfs_multisource(srcs)
     ref = np.min(np.stack([oracle.bfs(csr.row_ptr, csr.col, s).astype(np.int64) for s in srcs]), axis=0)
-    assert np.array_equal(out.numpy(), ref.astype(np.int32))
+    for fl in (0, 0x40):
+        out, st = g.bfs_multisource(srcs, flags=fl)
+        assert np.array_equal(out.numpy(), ref.astype(np.int32)), fl
+
+
+def test_bfs_twin_matches_graph_order_stats(olv):
+    """The renumbered twin gives the same levels and the same statistics as
+    the run in the graph's own numbering, also on a graph with isolated
+    vertices and a weighted one."""
+    csr = olv.with_weights_i32(olv.gen_rmat(15, 4, seed=11), seed=11)
+    g = olv.Graph(csr)
+    for s in (0, 77, int(np.argmax(csr.outdeg()))):
+        a, sa = g.bfs(s)
+        b, sb = g.bfs(s, flags=0x40)
+        assert np.array_equal(a.numpy(), b.numpy()), s
+        assert (sa.iterations, sa.vertices_reached, sa.edges_traversed) == \
+               (sb.iterations, sb.vertices_reached, sb.edges_traversed), s
 
 
 def test_bfs_errors(olv):
