@@ -189,7 +189,7 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
     r["modes_ms_mean"] = {"pipeline": r["ms_mean"]}
     for name, fl in ({} if args.no_modes else modes).items():
         ts = []
-        for s in srcs[:4]:
+        for s in srcs:
             g.bfs(int(s), out=out, flags=fl)
             ts.append(g.bfs(int(s), out=out, flags=fl)[1].elapsed_ms)
         r["modes_ms_mean"][name] = float(np.mean(ts))

@@ -174,7 +174,8 @@ __global__ void bfs_seed_kernel(const uint32_t *__restrict__ seeds, uint32_t k, 
     const uint32_t old = atomicOr(&a.snap[s >> 5], 1u << (s & 31));
     if (old & (1u << (s & 31))) return 0xffffffffu;
     a.visited[s] = 1;
-    a.set_level(s, 0);
+    if (a.omap) a.set_level<true>(s, 0);
+    else a.set_level<false>(s, 0);
     return s;
   };
   unsigned long long deg = 0;
@@ -196,6 +197,7 @@ struct BfsDir {
 // Level L = ctl->level: claim the vertices of level L + 1, top-down from
 // the level's queue and hub items or bottom-up against frontier bitmap
 // L & 1; reset the next level's fill counters.
+template <bool Mapped>
 __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nwp) {
   if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   const uint32_t L = ctl->level;
@@ -213,9 +215,9 @@ __global__ void __launch_bounds__(kBlock) bfs_expand_kernel(BfsArgs a, BfsCtl *c
   uint32_t nf = 0;
   unsigned long long mf = 0, insp = 0;
   if (st.mode == 0)
-    td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
+    td_step<kBlock, Mapped>(a, qsize, nhubs, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
   else
-    bu_step<kBlock>(a, front, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
+    bu_step<kBlock, Mapped>(a, front, (int32_t)(L + 1), blockIdx.x, gridDim.x, nf, mf, insp);
   flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
 }
 
@@ -259,6 +261,7 @@ __global__ void __launch_bounds__(kBlock) bfs_compact_kernel(BfsArgs a, BfsCtl *
 
 // All levels in one cooperative launch with the XCD-aware grid barrier
 // (OLV_PERSISTENT).  Same steps and decisions as the pipeline.
+template <bool Mapped>
 __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCtl *ctl, uint32_t *bms, uint32_t nw,
                                                                 uint32_t nwp, BfsDir dir, GridBar *bar) {
   GridBarLocal gb;
@@ -282,9 +285,9 @@ __global__ void __launch_bounds__(kBlock) bfs_persistent_kernel(BfsArgs a, BfsCt
     uint32_t nf = 0;
     unsigned long long mf = 0, insp = 0;
     if (st.mode == 0)
-      td_step<kBlock>(a, qsize, nhubs, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
+      td_step<kBlock, Mapped>(a, qsize, nhubs, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
     else
-      bu_step<kBlock>(a, front, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
+      bu_step<kBlock, Mapped>(a, front, (int32_t)(L + 1), vb, nvb, nf, mf, insp);
     flush_counters<kBlock>(&ctl->cnt[L % 3], nf, mf, insp);
     if (!grid_barrier(bar, gb, kSpinLimit)) break;
     const CtrSum c = ctr_total(&ctl->cnt[L % 3]);
@@ -397,7 +400,10 @@ int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can
   const unsigned ge = (unsigned)cu_count() * OLV_BFS_EXPAND_PER_CU;
   const unsigned gc = grid_for(w->nw, kBlock, OLV_BFS_COMPACT_PER_CU);
   for (uint32_t i = 0; i < count; ++i) {
-    hipLaunchKernelGGL(bfs_expand_kernel, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);
+    if (a.omap)
+      hipLaunchKernelGGL(bfs_expand_kernel<true>, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);
+    else
+      hipLaunchKernelGGL(bfs_expand_kernel<false>, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);
     hipLaunchKernelGGL(bfs_compact_kernel, dim3(gc), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nw, w->nwp,
                        (unsigned long long)g->m, dir);
   }
@@ -406,19 +412,20 @@ int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can
 }
 
 static int enqueue_persistent(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu) {
+  BfsArgs a = make_args(g, w, levels);
+  const void *kern = a.omap ? reinterpret_cast<const void *>(bfs_persistent_kernel<true>)
+                            : reinterpret_cast<const void *>(bfs_persistent_kernel<false>);
   int per_cu = 0;
-  OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, bfs_persistent_kernel, kBlock, 0));
+  OLV_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, kBlock, 0));
   if (per_cu <= 0) return fail(OLV_EUNSUPPORTED, "persistent BFS kernel does not fit on a CU");
   const int grid = cu_count();  // one workgroup per CU: cheapest barrier (measured)
-  BfsArgs a = make_args(g, w, levels);
   BfsCtl *ctl = w->ctl;
   uint32_t *bms = w->bms;
   uint32_t nw = w->nw, nwp = w->nwp;
   BfsDir dir = make_dir(g, w, can_bu);
   GridBar *bar = w->bar;
   void *args[] = {&a, &ctl, &bms, &nw, &nwp, &dir, &bar};
-  OLV_HIP(hipLaunchCooperativeKernel(reinterpret_cast<const void *>(bfs_persistent_kernel), dim3(grid), dim3(kBlock),
-                                     args, 0, stream()));
+  OLV_HIP(hipLaunchCooperativeKernel(kern, dim3(grid), dim3(kBlock), args, 0, stream()));
   return OLV_OK;
 }
 
