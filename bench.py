@@ -383,6 +383,13 @@ def bench_adjoint(args, olv, scale=24):
                      "fd_full_gradient_ms": t_fd_dir * (csr.m + csr.n),
                      "speedup_full_gradient_vs_fd": t_fd_dir * (csr.m + csr.n) / (t_f + t_a),
                      "fd_directions_for_5x": k5}
+    if not args.no_modes:
+        # the same solves in the graph's own numbering (no renumbered CSRs)
+        go = olv.OLV_GRAPH_ORDER
+        g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=go)
+        g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=go)
+        res["detail"]["graph_order_forward_ms"] = g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=go)[1].elapsed_ms
+        res["detail"]["graph_order_vjp_ms"] = g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=go)[3].elapsed_ms
     return res
 
 

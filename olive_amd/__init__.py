@@ -518,30 +518,30 @@ class Graph:
         return out, st
 
     def pagerank_weighted(self, w: DeviceBuffer, t: DeviceBuffer, damping=0.85, tol=1e-6, max_iters=1000,
-                          out: DeviceBuffer | None = None):
+                          out: DeviceBuffer | None = None, flags: int = 0):
         out = out or DeviceBuffer(self.n, np.float32)
         st = Stats()
-        check(lib().olv_pagerank_weighted_f32(self.h, w.c, t.c, damping, tol, max_iters, out.c, 0,
+        check(lib().olv_pagerank_weighted_f32(self.h, w.c, t.c, damping, tol, max_iters, out.c, flags,
                                               ctypes.byref(st)), "olv_pagerank_weighted_f32")
         return out, st
 
     def pagerank_vjp(self, w: DeviceBuffer, r: DeviceBuffer, gbar: DeviceBuffer, damping=0.85, tol=1e-6,
-                     max_iters=1000):
+                     max_iters=1000, flags: int = 0):
         gw = DeviceBuffer(self.m, np.float32)
         gt = DeviceBuffer(self.n, np.float32)
         lam = DeviceBuffer(self.n, np.float32)
         st = Stats()
-        check(lib().olv_pagerank_vjp_f32(self.h, w.c, r.c, gbar.c, damping, tol, max_iters, gw.c, gt.c, lam.c, 0,
+        check(lib().olv_pagerank_vjp_f32(self.h, w.c, r.c, gbar.c, damping, tol, max_iters, gw.c, gt.c, lam.c, flags,
                                          ctypes.byref(st)), "olv_pagerank_vjp_f32")
         return gw, gt, lam, st
 
     def pagerank_jvp(self, w: DeviceBuffer, r: DeviceBuffer, dw: DeviceBuffer, dt: DeviceBuffer, damping=0.85,
-                     tol=1e-6, max_iters=1000):
+                     tol=1e-6, max_iters=1000, flags: int = 0):
         """Tangent dr of the weighted fixed point for (dw, dt) at r
         (olv_pagerank_jvp_f32)."""
         dr = DeviceBuffer(self.n, np.float32)
         st = Stats()
-        check(lib().olv_pagerank_jvp_f32(self.h, w.c, r.c, dw.c, dt.c, damping, tol, max_iters, dr.c, 0,
+        check(lib().olv_pagerank_jvp_f32(self.h, w.c, r.c, dw.c, dt.c, damping, tol, max_iters, dr.c, flags,
                                          ctypes.byref(st)), "olv_pagerank_jvp_f32")
         return dr, st
 

@@ -23,29 +23,40 @@
 
 namespace olv {
 
-// The plain pull iterates over a relabelled copy of the in-CSR: vertices
-// numbered by descending in-degree (ties by id), each row's entries mapped to
-// the new numbers in their original slot order.  On a skewed graph the
-// vertices most gathered from (large out-degree, which follows in-degree
-// there) then sit together in a few cache lines of the contribution vector,
-// and the rows of one degree bucket are consecutive.  Every vertex keeps its
-// bucket and its summation order, so the ranks are bit-identical to an
-// iteration in the graph's own order.
-// (The numbering itself, g->rl_pi / g->rl_order, belongs to the graph.)
-struct PrRelabel {
-  uint32_t *rp = nullptr;     // n + 1: row offsets of the relabelled in-CSR
-  uint32_t *col = nullptr;    // m: new numbers of the in-neighbours
-  float *inv = nullptr;       // n: 1/outdeg by new number
+// The iterations run over renumbered copies of the CSRs: vertices numbered
+// by descending in-degree (ties by id), each row's entries mapped to the new
+// numbers in their original slot order (the in-CSR for the pulls, the
+// out-CSR for the adjoint).  On a skewed graph the vertices most gathered
+// from (large out-degree, which follows in-degree there) then sit together
+// in a few cache lines of the gathered vector, and the rows of one degree
+// bucket are consecutive.  Every vertex keeps its bucket and its summation
+// order, so every iterate is bit-identical to the iteration in the graph's
+// own numbering (OLV_GRAPH_ORDER).  The numbering itself, g->rl_pi /
+// g->rl_order, belongs to the graph.
+struct RelabelSide {
+  uint32_t *rp = nullptr;   // n + 1: row offsets, rows in new-number order
+  uint32_t *col = nullptr;  // m: new numbers of the row's entries
+  uint32_t *eid = nullptr;  // m: edge id (index into w) of each slot
   SegPlan plan;
   bool ready = false;
   void release() {
-    void *ps[] = {rp, col, inv};
+    void *ps[] = {rp, col, eid};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    rp = col = nullptr;
-    inv = nullptr;
+    rp = col = eid = nullptr;
     plan.release();
     ready = false;
+  }
+};
+
+struct PrRelabel {
+  RelabelSide in, out;
+  float *inv = nullptr;  // n: 1/outdeg by new number
+  void release() {
+    in.release();
+    out.release();
+    if (inv) (void)hipFree(inv);
+    inv = nullptr;
   }
 };
 
@@ -106,23 +117,30 @@ static int ensure_plan(olv_graph *g, bool in_edges) {
   return OLV_OK;
 }
 
-// Row i of the relabelled in-CSR is row order[i] of the in-CSR with every
-// entry u replaced by pi[u]; one wave per row.
+// Row i of a renumbered CSR is row order[i] of the source CSR with every
+// entry u replaced by pi[u]; eid (if not null) receives each slot's edge id,
+// src_eid[slot] or, with src_eid null, the source slot itself.  One wave per
+// row.
 __global__ void relabel_rows_kernel(const uint32_t *__restrict__ order, const uint32_t *__restrict__ pi,
-                                    const uint32_t *__restrict__ in_rp, const uint32_t *__restrict__ in_col,
-                                    const uint32_t *__restrict__ rp, uint32_t n, uint32_t *__restrict__ col) {
+                                    const uint32_t *__restrict__ src_rp, const uint32_t *__restrict__ src_col,
+                                    const uint32_t *__restrict__ src_eid, const uint32_t *__restrict__ rp, uint32_t n,
+                                    uint32_t *__restrict__ col, uint32_t *__restrict__ eid) {
   const uint32_t waves = gridDim.x * (blockDim.x / kWave);
   const uint32_t lane = threadIdx.x % kWave;
   for (uint32_t i = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave; i < n; i += waves) {
     const uint32_t v = order[i];
-    const uint32_t b = in_rp[v], len = in_rp[v + 1] - b, o = rp[i];
+    const uint32_t b = src_rp[v], len = src_rp[v + 1] - b, o = rp[i];
     for (uint32_t j0 = lane; j0 < len; j0 += 4 * kWave) {  // four loads in flight per lane
       uint32_t c[4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) c[q] = j0 + q * kWave < len ? in_col[b + j0 + q * kWave] : 0u;
+      for (int q = 0; q < 4; ++q) c[q] = j0 + q * kWave < len ? src_col[b + j0 + q * kWave] : 0u;
 #pragma unroll
       for (int q = 0; q < 4; ++q)
-        if (j0 + q * kWave < len) col[o + j0 + q * kWave] = pi[c[q]];
+        if (j0 + q * kWave < len) {
+          const uint32_t k = b + j0 + q * kWave;
+          col[o + j0 + q * kWave] = pi[c[q]];
+          if (eid) eid[o + j0 + q * kWave] = src_eid ? src_eid[k] : k;
+        }
     }
   }
 }
@@ -136,31 +154,39 @@ __global__ void gather_kernel(const T *__restrict__ in, const uint32_t *__restri
 
 static unsigned ew_grid(size_t n);
 
-static int ensure_relabel(olv_graph *g, PrWork *w) {
-  PrRelabel &rl = w->rl;
-  if (rl.ready) return OLV_OK;
-  rl.release();
+// Renumbered copy of the in-CSR (in_edges) or the out-CSR, with the slot
+// edge ids when with_eid (weighted solves).
+static int ensure_side(olv_graph *g, PrWork *w, bool in_edges, bool with_eid) {
+  RelabelSide &sd = in_edges ? w->rl.in : w->rl.out;
+  if (sd.ready && (sd.eid || !with_eid)) return OLV_OK;
+  sd.release();
   const uint32_t n = g->n, m = g->m;
+  if (in_edges && !g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
   std::vector<uint32_t> order;
   OLV_TRY(indeg_order(g, &order));
-  std::vector<uint32_t> in_rp((size_t)n + 1), rp((size_t)n + 1);
-  OLV_HIP(hipMemcpy(in_rp.data(), g->in_row_ptr, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  const uint32_t *d_src_rp = in_edges ? g->in_row_ptr : g->row_ptr;
+  std::vector<uint32_t> src_rp((size_t)n + 1), rp((size_t)n + 1);
+  OLV_HIP(hipMemcpy(src_rp.data(), d_src_rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
   rp[0] = 0;
-  for (uint32_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (in_rp[order[i] + 1] - in_rp[order[i]]);
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.rp), ((size_t)n + 1) * 4));
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.col), std::max<size_t>(m, 1) * 4));
-  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&rl.inv), std::max<size_t>(n, 1) * 4));
-  OLV_HIP(hipMemcpy(rl.rp, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
-  if (n) {
-    hipLaunchKernelGGL(relabel_rows_kernel, dim3(ew_grid((size_t)n * kWave)), dim3(256), 0, stream(), g->rl_order,
-                       g->rl_pi, g->in_row_ptr, g->in_col, rl.rp, n, rl.col);
-    hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(n)), dim3(256), 0, stream(),
-                       (const float *)g->inv_outdeg, g->rl_order, n, rl.inv);
-    OLV_HIP(hipGetLastError());
+  for (uint32_t i = 0; i < n; ++i) rp[i + 1] = rp[i] + (src_rp[order[i] + 1] - src_rp[order[i]]);
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&sd.rp), ((size_t)n + 1) * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&sd.col), std::max<size_t>(m, 1) * 4));
+  if (with_eid) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&sd.eid), std::max<size_t>(m, 1) * 4));
+  OLV_HIP(hipMemcpy(sd.rp, rp.data(), ((size_t)n + 1) * 4, hipMemcpyHostToDevice));
+  if (!w->rl.inv) {
+    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->rl.inv), std::max<size_t>(n, 1) * 4));
+    if (n)
+      hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(n)), dim3(256), 0, stream(),
+                         (const float *)g->inv_outdeg, g->rl_order, n, w->rl.inv);
   }
-  OLV_TRY(build_seg_plan_rp(rp, rl.plan));
+  if (n)
+    hipLaunchKernelGGL(relabel_rows_kernel, dim3(ew_grid((size_t)n * kWave)), dim3(256), 0, stream(), g->rl_order,
+                       g->rl_pi, d_src_rp, in_edges ? g->in_col : g->col, in_edges ? g->in_eid : nullptr, sd.rp, n,
+                       sd.col, sd.eid);
+  OLV_HIP(hipGetLastError());
+  OLV_TRY(build_seg_plan_rp(rp, sd.plan));
   OLV_HIP(hipStreamSynchronize(stream()));
-  rl.ready = true;
+  sd.ready = true;
   return OLV_OK;
 }
 
@@ -219,16 +245,19 @@ struct AdjOp {
   }
 };
 
-// grad_w[e=(u,v)] = d * r_u * lambda_v, written while walking the out-CSR.
+// grad_w[e=(u,v)] = d * r_u * lambda_v, written while walking the out-CSR
+// (Mapped: a renumbered out-CSR, whose slot k holds edge eid[k]).
+template <bool Mapped>
 struct GradOp {
   const uint32_t *ci;
   const float *r;
   const float *lam;
   float *grad_w;
+  const uint32_t *eid;
   float d;
   __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
   __device__ __forceinline__ float val(uint32_t u, uint32_t k, uint32_t v) const {
-    __builtin_nontemporal_store(d * r[u] * lam[v], &grad_w[k]);
+    __builtin_nontemporal_store(d * r[u] * lam[v], &grad_w[Mapped ? ld_stream(&eid[k]) : k]);
     return 0.f;
   }
   __device__ __forceinline__ void finish(uint32_t, float, float &) const {}
@@ -279,6 +308,12 @@ __global__ void __launch_bounds__(256) ctl_init_l1_kernel(IterCtl *ctl, const do
 __global__ void permute_kernel(const float *__restrict__ w, const uint32_t *__restrict__ eid, uint32_t m,
                                float *__restrict__ out) {
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) out[k] = w[eid[k]];
+}
+
+// y[i] = a * x[idx[i]]
+__global__ void gather_scale_kernel(const float *__restrict__ x, const uint32_t *__restrict__ idx, float a, uint32_t n,
+                                    float *__restrict__ y) {
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) y[i] = a * x[idx[i]];
 }
 
 __global__ void scale_kernel(const float *__restrict__ x, float a, uint32_t n, float *__restrict__ y) {
@@ -380,8 +415,9 @@ struct Stage {
 // Fixed point r = d * P^T r + (1-d) t over the in-slot weights w->w_perm,
 // iterated from r0 = t until the L1 change drops below tol; the result is
 // left in w->buf[res->iters & 1].  t must not be w->buf[0] or w->buf[1].
-static int weighted_solve(const olv_graph *g, const SegDev &d, PrWork *w, const float *t, float damping, float tol,
-                          uint32_t max_iters, Timer &timer, IterResult *res) {
+static int weighted_solve(const olv_graph *g, const SegDev &d, const uint32_t *rp, const uint32_t *ci, PrWork *w,
+                          const float *t, float damping, float tol, uint32_t max_iters, Timer &timer,
+                          IterResult *res) {
   OLV_HIP(hipMemcpyAsync(w->buf[0], t, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
   const uint32_t group = 8;
@@ -390,8 +426,8 @@ static int weighted_solve(const olv_graph *g, const SegDev &d, PrWork *w, const 
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
-      WprOp op{g->in_col, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
-      OLV_TRY(enqueue_iteration(d, g->in_row_ptr, op, w, it));
+      WprOp op{ci, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
+      OLV_TRY(enqueue_iteration(d, rp, op, w, it));
     }
     timer.stop();
     OLV_TRY(read_ctl(w));
@@ -399,6 +435,37 @@ static int weighted_solve(const olv_graph *g, const SegDev &d, PrWork *w, const 
     if (res->converged) break;
   }
   return OLV_OK;
+}
+
+// The in-CSR arrays a weighted pull runs over: the renumbered copy, or the
+// graph's own with OLV_GRAPH_ORDER.  eid: each slot's edge id.
+struct PullView {
+  SegDev d;
+  const uint32_t *rp, *ci, *eid;
+  bool relabel;
+};
+
+static int pull_view(olv_graph *g, PrWork *w, unsigned flags, PullView *v) {
+  v->relabel = (flags & OLV_GRAPH_ORDER) == 0;
+  if (v->relabel) {
+    OLV_TRY(ensure_side(g, w, true, true));
+    v->d = make_seg_dev(w->rl.in.plan);
+    v->rp = w->rl.in.rp;
+    v->ci = w->rl.in.col;
+    v->eid = w->rl.in.eid;
+  } else {
+    OLV_TRY(ensure_plan(g, true));
+    v->d = make_seg_dev(g->plan_in);
+    v->rp = g->in_row_ptr;
+    v->ci = g->in_col;
+    v->eid = g->in_eid;
+  }
+  return OLV_OK;
+}
+
+// out[i] = in[idx[i]] for i < n on the library stream.
+static void gather_f32(const float *in, const uint32_t *idx, uint32_t n, float *out) {
+  if (n) hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(n)), dim3(256), 0, stream(), in, idx, n, out);
 }
 
 }  // namespace olv
@@ -416,10 +483,10 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (!relabel) OLV_TRY(ensure_plan(g, true));
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, false, &w));
-  if (relabel) OLV_TRY(ensure_relabel(g, w));
-  const SegDev dr = make_seg_dev(relabel ? w->rl.plan : g->plan_in);
-  const uint32_t *rp = relabel ? w->rl.rp : g->in_row_ptr;
-  const uint32_t *ci = relabel ? w->rl.col : g->in_col;
+  if (relabel) OLV_TRY(ensure_side(g, w, true, false));
+  const SegDev dr = make_seg_dev(relabel ? w->rl.in.plan : g->plan_in);
+  const uint32_t *rp = relabel ? w->rl.in.rp : g->in_row_ptr;
+  const uint32_t *ci = relabel ? w->rl.in.col : g->in_col;
   const float *inv = relabel ? w->rl.inv : g->inv_outdeg;
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
@@ -468,10 +535,10 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
                               uint32_t max_iters, float *ranks, unsigned flags, olv_stats *st) {
   if (!cg || !wts || !tele || !ranks) return fail(OLV_EINVAL, "olv_pagerank_weighted_f32: NULL argument");
   olv_graph *g = const_cast<olv_graph *>(cg);
-  OLV_TRY(ensure_plan(g, true));
-  const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, true, &w));
+  PullView pv;
+  OLV_TRY(pull_view(g, w, flags, &pv));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, st_t, so;
@@ -481,14 +548,23 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   OLV_TRY(s);
   float *r = so.out(ranks, g->n, host, &s);
   OLV_TRY(s);
+  ScratchBuf tb;  // the teleport vector in the new numbering
+  if (pv.relabel) OLV_TRY(tb.alloc(std::max<size_t>(g->n, 1) * 4));
   Timer t;
   t.start();
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
+  const float *tt = dt;
+  if (pv.relabel) {
+    gather_f32(dt, g->rl_order, g->n, tb.as<float>());
+    tt = tb.as<float>();
+  }
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, pv.eid, g->m, w->w_perm);
   IterResult res;
-  OLV_TRY(weighted_solve(g, d, w, dt, damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, tt, damping, tol, max_iters, t, &res));
   const double ms = t.ms();
   const uint32_t iters = res.iters;
-  OLV_HIP(hipMemcpyAsync(r, w->buf[iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  if (pv.relabel) gather_f32(w->buf[iters & 1], g->rl_pi, g->n, r);
+  else OLV_HIP(hipMemcpyAsync(r, w->buf[iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  OLV_HIP(hipGetLastError());
   if (host) OLV_HIP(hipMemcpy(ranks, r, (size_t)g->n * 4, hipMemcpyDeviceToHost));
   else OLV_HIP(hipStreamSynchronize(stream()));
   if (st) {
@@ -512,10 +588,10 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   if (!cg || !wts || !ranks || !dwts || !dtele || !dranks) return fail(OLV_EINVAL, "olv_pagerank_jvp_f32: NULL argument");
   if (!(damping >= 0.0f && damping < 1.0f)) return fail(OLV_EINVAL, "olv_pagerank_jvp_f32: damping must be in [0, 1)");
   olv_graph *g = const_cast<olv_graph *>(cg);
-  OLV_TRY(ensure_plan(g, true));
-  const SegDev d = make_seg_dev(g->plan_in);
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, true, &w));
+  PullView pv;
+  OLV_TRY(pull_view(g, w, flags, &pv));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, sr, sdw, sdt, so;
@@ -529,22 +605,35 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(s);
   float *out = so.out(dranks, g->n, host, &s);
   OLV_TRY(s);
+  ScratchBuf rb, tb;  // r and dt in the new numbering
+  if (pv.relabel) {
+    OLV_TRY(rb.alloc(std::max<size_t>(g->n, 1) * 4));
+    OLV_TRY(tb.alloc(std::max<size_t>(g->n, 1) * 4));
+  }
   Timer t;
   t.start();
+  if (pv.relabel) {
+    gather_f32(dr_in, g->rl_order, g->n, rb.as<float>());
+    gather_f32(ddt, g->rl_order, g->n, tb.as<float>());
+    dr_in = rb.as<float>();
+    ddt = tb.as<float>();
+  }
   // source term s = d * dP^T r + (1-d) dt: one forward iteration with the
   // weights dw, applied to r, and the teleport dt
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), ddw, g->in_eid, g->m, w->w_perm);
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), ddw, pv.eid, g->m, w->w_perm);
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, 0.0);
-  WprOp src_op{g->in_col, w->w_perm, dr_in, w->buf[2], ddt, damping, 1.0f - damping};
-  OLV_TRY(enqueue_iteration(d, g->in_row_ptr, src_op, w, 0));
+  WprOp src_op{pv.ci, w->w_perm, dr_in, w->buf[2], ddt, damping, 1.0f - damping};
+  OLV_TRY(enqueue_iteration(pv.d, pv.rp, src_op, w, 0));
   // dr = d * P^T dr + s, i.e. the forward fixed point with teleport s / (1-d)
   hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), w->buf[2], 1.0f / (1.0f - damping),
                      g->n, w->buf[3]);
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, g->in_eid, g->m, w->w_perm);
+  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, pv.eid, g->m, w->w_perm);
   IterResult res;
-  OLV_TRY(weighted_solve(g, d, w, w->buf[3], damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, w->buf[3], damping, tol, max_iters, t, &res));
   const double ms = t.ms();
-  OLV_HIP(hipMemcpyAsync(out, w->buf[res.iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  if (pv.relabel) gather_f32(w->buf[res.iters & 1], g->rl_pi, g->n, out);
+  else OLV_HIP(hipMemcpyAsync(out, w->buf[res.iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  OLV_HIP(hipGetLastError());
   if (host) OLV_HIP(hipMemcpy(dranks, out, (size_t)g->n * 4, hipMemcpyDeviceToHost));
   else OLV_HIP(hipStreamSynchronize(stream()));
   if (st) {
@@ -567,10 +656,15 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
                          unsigned flags, olv_stats *st) {
   if (!cg || !wts || !ranks || !gbar) return fail(OLV_EINVAL, "olv_pagerank_vjp_f32: NULL argument");
   olv_graph *g = const_cast<olv_graph *>(cg);
-  OLV_TRY(ensure_plan(g, false));
-  const SegDev d = make_seg_dev(g->plan_out);
+  const bool relabel = (flags & OLV_GRAPH_ORDER) == 0;
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, &w));
+  OLV_TRY(get_pr_work(g, relabel, &w));
+  // the out-CSR the adjoint runs over: the renumbered copy or the graph's own
+  if (relabel) OLV_TRY(ensure_side(g, w, false, true));
+  else OLV_TRY(ensure_plan(g, false));
+  const SegDev d = make_seg_dev(relabel ? w->rl.out.plan : g->plan_out);
+  const uint32_t *rp = relabel ? w->rl.out.rp : g->row_ptr;
+  const uint32_t *ci = relabel ? w->rl.out.col : g->col;
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, sr, sg, sgw, sgt, sl;
@@ -586,12 +680,30 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(s);
   float *dl = sl.out(lambda_out, g->n, host, &s);
   OLV_TRY(s);
+  ScratchBuf rb, gb;  // r and gbar in the new numbering
+  if (relabel) {
+    OLV_TRY(rb.alloc(std::max<size_t>(g->n, 1) * 4));
+    OLV_TRY(gb.alloc(std::max<size_t>(g->n, 1) * 4));
+  }
   Timer t;
   t.start();
+  const float *wv = dw, *rv = dr, *gv = dg;
+  if (relabel) {
+    if (g->m)
+      hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, w->rl.out.eid, g->m,
+                         w->w_perm);
+    gather_f32(dr, g->rl_order, g->n, rb.as<float>());
+    gather_f32(dg, g->rl_order, g->n, gb.as<float>());
+    wv = w->w_perm;
+    rv = rb.as<float>();
+    gv = gb.as<float>();
+  }
   // buffers 2/3 hold lambda (0/1 left free for a concurrent forward's state)
   float *lam[2] = {w->buf[2], w->buf[3]};
-  OLV_HIP(hipMemcpyAsync(lam[0], dg, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  OLV_HIP(hipMemcpyAsync(lam[0], gv, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
   {
+    // the stop threshold from gbar in the graph's numbering (the same sum in
+    // both numberings)
     const uint32_t k = (uint32_t)std::min<size_t>(kPartials, ew_grid(g->n));
     hipLaunchKernelGGL(l1_partial_kernel, dim3(k), dim3(256), 0, stream(), dg, g->n, w->part);
     hipLaunchKernelGGL(ctl_init_l1_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, w->part, k, (double)tol);
@@ -602,8 +714,8 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
-      AdjOp op{g->col, dw, lam[it & 1], lam[(it + 1) & 1], dg, damping};
-      OLV_TRY(enqueue_iteration(d, g->row_ptr, op, w, it));
+      AdjOp op{ci, wv, lam[it & 1], lam[(it + 1) & 1], gv, damping};
+      OLV_TRY(enqueue_iteration(d, rp, op, w, it));
     }
     OLV_TRY(read_ctl(w));
     res = iter_result(*w->h_ctl, it);
@@ -612,13 +724,29 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   const uint32_t iters = res.iters;
   const float *lf = lam[iters & 1];
   if (dgw && g->m) {
-    GradOp gop{g->col, dr, lf, dgw, damping};
-    hipLaunchKernelGGL(seg_iter_kernel<GradOp>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, g->row_ptr,
-                       gop, (IterCtl *)nullptr, 0u);
-    // chunked hub rows wrote their edges inside the chunk blocks already
+    // chunked hub rows write their edges inside the chunk blocks
+    if (relabel) {
+      GradOp<true> gop{ci, rv, lf, dgw, w->rl.out.eid, damping};
+      hipLaunchKernelGGL(seg_iter_kernel<GradOp<true>>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, gop,
+                         (IterCtl *)nullptr, 0u);
+    } else {
+      GradOp<false> gop{ci, rv, lf, dgw, nullptr, damping};
+      hipLaunchKernelGGL(seg_iter_kernel<GradOp<false>>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, gop,
+                         (IterCtl *)nullptr, 0u);
+    }
   }
-  if (dgt) hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf, 1.0f - damping, g->n, dgt);
-  if (dl) OLV_HIP(hipMemcpyAsync(dl, lf, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  if (dgt) {
+    if (relabel) {
+      hipLaunchKernelGGL(gather_scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf,
+                         (const uint32_t *)g->rl_pi, 1.0f - damping, g->n, dgt);
+    } else {
+      hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), lf, 1.0f - damping, g->n, dgt);
+    }
+  }
+  if (dl) {
+    if (relabel) gather_f32(lf, g->rl_pi, g->n, dl);
+    else OLV_HIP(hipMemcpyAsync(dl, lf, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
+  }
   OLV_HIP(hipGetLastError());
   t.stop();
   const double ms = t.ms();

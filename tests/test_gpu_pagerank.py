@@ -112,6 +112,41 @@ def weighted_case(olv, scale, seed):
     return csr, t
 
 
+def bits(buf):
+    return buf.numpy().view(np.uint32)
+
+
+@pytest.mark.parametrize("case", ["rmat14", "star"])
+def test_weighted_vjp_jvp_relabel_bit_identical(olv, case):
+    """Forward, VJP (grad_w, grad_t, lambda) and JVP over the renumbered
+    CSRs equal the runs in the graph's own numbering bit for bit."""
+    if case == "rmat14":
+        csr, t = weighted_case(olv, 14, 21)
+    else:
+        base = star_in_hub(olv, 20011)
+        csr = olv.with_weights_f32_normalized(base, seed=4)
+        t = olv.gen_dirichlet(csr.n, seed=4)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights.astype(np.float32))
+    T = olv.DeviceBuffer.from_numpy(t.astype(np.float32))
+    rng = np.random.default_rng(5)
+    gb = olv.DeviceBuffer.from_numpy(rng.standard_normal(csr.n).astype(np.float32))
+    # small tangents (the tangent solve stops on an absolute L1 change, so
+    # its fp32 rounding floor scales with the tangent's size)
+    dW = olv.DeviceBuffer.from_numpy((1e-3 * rng.standard_normal(csr.m)).astype(np.float32))
+    dT = olv.DeviceBuffer.from_numpy((1e-3 * rng.standard_normal(csr.n)).astype(np.float32))
+    res = {}
+    for fl in (0, olv.OLV_GRAPH_ORDER):
+        r, st = g.pagerank_weighted(W, T, tol=1e-6, flags=fl)
+        gw, gt, lam, sv = g.pagerank_vjp(W, r, gb, tol=1e-6, flags=fl)
+        dr, sj = g.pagerank_jvp(W, r, dW, dT, tol=1e-6, flags=fl)
+        res[fl] = ([bits(x) for x in (r, gw, gt, lam, dr)], (st.iterations, sv.iterations, sj.iterations))
+    (a, ia), (b, ib) = res[0], res[olv.OLV_GRAPH_ORDER]
+    assert ia == ib
+    for name, x, y in zip(("r", "grad_w", "grad_t", "lambda", "jvp"), a, b):
+        assert np.array_equal(x, y), (case, name)
+
+
 def test_weighted_pagerank_forward(olv):
     csr, t = weighted_case(olv, 12, 5)
     g = olv.Graph(csr)
