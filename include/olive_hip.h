@@ -51,11 +51,11 @@ enum {
                                         (slower on MI355X; kept to compare)      */
 #define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
                                         with hipEvents (olv_stats.kernel_ms)     */
-#define OLV_GRAPH_ORDER      0x0040u /* olv_pagerank_f32, olv_bfs,
-                                        olv_bfs_multisource: run in the graph's
-                                        own vertex numbering instead of the
-                                        in-degree renumbering (same results,
-                                        bit for bit)                             */
+#define OLV_GRAPH_ORDER      0x0040u /* olv_bfs, olv_bfs_multisource and the
+                                        PageRank solves (plain, weighted, VJP,
+                                        JVP): run in the graph's own vertex
+                                        numbering instead of the in-degree
+                                        renumbering (same results, bit for bit) */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
 
 /* weight dtypes for olv_graph_create */
