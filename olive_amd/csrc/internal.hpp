@@ -79,7 +79,8 @@ struct olv_graph {
   // (built on first use; shared by the PageRank pull and the BFS twin).
   uint32_t *rl_pi = nullptr;       // vertex -> new number
   uint32_t *rl_order = nullptr;    // new number -> vertex
-  std::vector<uint32_t> h_rl_pi;   // host copy of rl_pi (seed mapping)
+  std::vector<uint32_t> h_rl_pi;   // host copies of rl_pi (seed mapping) and rl_order
+  std::vector<uint32_t> h_rl_order;
   olv_graph *twin = nullptr;       // this graph renumbered by rl_pi (BFS runs on it)
   const uint32_t *out_map = nullptr;  // a twin: its vertex -> the original graph's (not owned)
   uint32_t *union_map = nullptr;   // multigraph: unused here

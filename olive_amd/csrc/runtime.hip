@@ -361,7 +361,10 @@ static void graph_free(olv_graph *g) {
 }
 
 int indeg_order(olv_graph *g, std::vector<uint32_t> *order_out) {
-  if (g->rl_pi && !order_out) return OLV_OK;
+  if (g->rl_pi) {
+    if (order_out) *order_out = g->h_rl_order;
+    return OLV_OK;
+  }
   if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
   const uint32_t n = g->n;
   std::vector<uint32_t> in_rp((size_t)n + 1);
@@ -378,12 +381,19 @@ int indeg_order(olv_graph *g, std::vector<uint32_t> *order_out) {
     order[i] = v;
     pi[v] = i;
   }
-  if (!g->rl_pi) {
-    OLV_TRY(upload(&g->rl_pi, pi.data(), n));
-    OLV_TRY(upload(&g->rl_order, order.data(), n));
-    g->h_rl_pi = std::move(pi);
+  uint32_t *d_pi = nullptr, *d_order = nullptr;
+  int st = upload(&d_pi, pi.data(), n);
+  if (st == OLV_OK) st = upload(&d_order, order.data(), n);
+  if (st != OLV_OK) {  // leave the graph without a half-built numbering
+    if (d_pi) (void)hipFree(d_pi);
+    if (d_order) (void)hipFree(d_order);
+    return st;
   }
-  if (order_out) *order_out = std::move(order);
+  g->rl_pi = d_pi;
+  g->rl_order = d_order;
+  g->h_rl_pi = std::move(pi);
+  g->h_rl_order = std::move(order);
+  if (order_out) *order_out = g->h_rl_order;
   return OLV_OK;
 }
 
