@@ -31,6 +31,10 @@ struct SegPlan {
   uint32_t nchunks = 0;
   uint32_t chunk = 0;
   uint32_t nempty = 0;              // rows without entries, at the end of bucket 0
+  // every bucket's list is a run of consecutive ids (a renumbered CSR):
+  // list[i] = i + shift[bucket]
+  bool contig = false;
+  uint32_t shift[kBuckets] = {0, 0, 0, 0};
   uint32_t *hub_vertex = nullptr;   // nhub
   uint32_t *hub_first = nullptr;    // nhub + 1 (chunk slot ranges)
   uint32_t *chunk_hub = nullptr;    // nchunks: owning hub index

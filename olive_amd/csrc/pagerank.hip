@@ -359,7 +359,12 @@ static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, 
                              KernelProfile *prof = nullptr) {
   hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
   if (a) OLV_HIP(hipEventRecord(a, stream()));
-  hipLaunchKernelGGL(seg_iter_kernel<Op>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl, k);
+  if (d.contig)
+    hipLaunchKernelGGL((seg_iter_kernel<Op, true>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl,
+                       k);
+  else
+    hipLaunchKernelGGL((seg_iter_kernel<Op, false>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op,
+                       w->ctl, k);
   if (b) OLV_HIP(hipEventRecord(b, stream()));
   OLV_HIP(hipGetLastError());
   return OLV_OK;
@@ -727,8 +732,12 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
     // chunked hub rows write their edges inside the chunk blocks
     if (relabel) {
       GradOp<true> gop{ci, rv, lf, dgw, w->rl.out.eid, damping};
-      hipLaunchKernelGGL(seg_iter_kernel<GradOp<true>>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, gop,
-                         (IterCtl *)nullptr, 0u);
+      if (d.contig)
+        hipLaunchKernelGGL((seg_iter_kernel<GradOp<true>, true>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d,
+                           rp, gop, (IterCtl *)nullptr, 0u);
+      else
+        hipLaunchKernelGGL((seg_iter_kernel<GradOp<true>, false>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d,
+                           rp, gop, (IterCtl *)nullptr, 0u);
     } else {
       GradOp<false> gop{ci, rv, lf, dgw, nullptr, damping};
       hipLaunchKernelGGL(seg_iter_kernel<GradOp<false>>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, gop,

@@ -327,6 +327,16 @@ int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan) {
     all.insert(all.end(), lists[b].begin(), lists[b].end());
     plan.offsets[b + 1] = (uint32_t)all.size();
   }
+  plan.contig = true;
+  for (uint32_t b = 0; b < SegPlan::kBuckets && plan.contig; ++b) {
+    const uint32_t o = plan.offsets[b];
+    plan.shift[b] = o < plan.offsets[b + 1] ? all[o] - o : 0u;  // mod 2^32
+    for (uint32_t i = o; i < plan.offsets[b + 1]; ++i)
+      if (all[i] != i + plan.shift[b]) {
+        plan.contig = false;
+        break;
+      }
+  }
   OLV_TRY(upload(&plan.list, all.data(), all.size()));
   std::vector<uint32_t> first(hubs.size() + 1, 0), chub;
   for (size_t h = 0; h < hubs.size(); ++h) {

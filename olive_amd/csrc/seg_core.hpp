@@ -36,6 +36,13 @@ struct SegDev {
   // end of the virtual blocks that hold a row with entries: the trailing
   // bucket-0 blocks hold only empty rows
   uint32_t live_end;
+  bool contig;                          // list[i] = i + shift[bucket]
+  uint32_t shift[SegPlan::kBuckets];
+  // Vertex at list position idx of bucket b (Contig: without the load).
+  template <bool Contig>
+  __device__ __forceinline__ uint32_t vertex(uint32_t idx, int b) const {
+    return Contig ? idx + shift[b] : list[idx];
+  }
 };
 
 constexpr int kSegBlock = 256;
@@ -66,6 +73,8 @@ inline SegDev make_seg_dev(const SegPlan &p) {
   d.role_start[5] = d.role_start[4] + (c0 + kSegBlock - 1) / kSegBlock;
   const uint32_t p0 = c0 - p.nempty;  // first empty row within bucket 0
   d.live_end = d.role_start[4] + (p0 + kSegBlock - 1) / kSegBlock;
+  d.contig = p.contig;
+  for (uint32_t i = 0; i < SegPlan::kBuckets; ++i) d.shift[i] = p.shift[i];
   return d;
 }
 
@@ -167,7 +176,7 @@ __device__ __forceinline__ double err_total(const IterCtl *ctl, uint32_t slot) {
 #define OLV_SEG_WAVES_PER_EU 8
 #endif
 
-template <class Op>
+template <class Op, bool Contig = false>
 __global__ void __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(OLV_SEG_WAVES_PER_EU)))
 seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, uint32_t k) {
   if (ctl) {
@@ -228,7 +237,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
         }
       }
     } else if (b < d.role_start[2]) {
-      const uint32_t v = d.list[d.off[3] + (b - d.role_start[1])];
+      const uint32_t v = d.template vertex<Contig>(d.off[3] + (b - d.role_start[1]), 3);
       float s = lane_sum<kSegBlock>(op, v, rp[v], rp[v + 1], threadIdx.x);
       s = block_sum<kSegBlock, float>(s, s_red);
       if (threadIdx.x == 0) op.finish(v, s, err);
@@ -236,7 +245,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       const uint32_t idx = d.off[2] + (b - d.role_start[2]) * (kSegBlock / 64) + threadIdx.x / 64;
       const uint32_t j = threadIdx.x % 64;
       if (idx < d.off[3]) {
-        const uint32_t v = d.list[idx];
+        const uint32_t v = d.template vertex<Contig>(idx, 2);
         float s = lane_sum<64>(op, v, rp[v], rp[v + 1], j);
         s = group_sum<64>(s);
         if (j == 0) op.finish(v, s, err);
@@ -248,7 +257,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       float s = 0.f;
       uint32_t v = 0;
       if (ok) {
-        v = d.list[idx];
+        v = d.template vertex<Contig>(idx, 1);
         s = lane_sum<8>(op, v, rp[v], rp[v + 1], j);
       }
       s = group_sum<8>(s);
@@ -256,7 +265,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
     } else {
       const uint32_t idx = d.off[0] + (b - d.role_start[4]) * kSegBlock + threadIdx.x;
       if (idx < d.off[1]) {
-        const uint32_t v = d.list[idx];
+        const uint32_t v = d.template vertex<Contig>(idx, 0);
         const float s = lane_sum<1>(op, v, rp[v], rp[v + 1], 0u);
         op.finish(v, s, err);
       }
