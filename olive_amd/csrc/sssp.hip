@@ -363,6 +363,11 @@ static int weight_stats(const int32_t *w, uint32_t m, int32_t *wmin, double *wme
   return OLV_OK;
 }
 
+// Workgroups per CU of the relax kernel (uniform 2^22 x 2^25 graph: 6 gives
+// 1.99 ms; 4: 2.00, 8: 2.03, 12: 2.01, 16: 2.05).
+#ifndef OLV_SSSP_RELAX_PER_CU
+#define OLV_SSSP_RELAX_PER_CU 6
+#endif
 #ifndef OLV_SSSP_DELTA_NUM
 #define OLV_SSSP_DELTA_NUM 32
 #endif
@@ -453,7 +458,7 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
     OLV_HIP(hipMemcpy(&hb, w.bar, sizeof(hb), hipMemcpyDeviceToHost));
     if (hb.err || !w.h_ctl->done) return fail(OLV_EINTERNAL, "olv_sssp: grid barrier timed out (co-residency lost)");
   } else {
-    const unsigned gr = (unsigned)cu_count() * 8;
+    const unsigned gr = (unsigned)cu_count() * OLV_SSSP_RELAX_PER_CU;
     const unsigned gc = sgrid(w.nw, kSBlock);
     // first chunk: the longest of the last few runs on this graph (8 at
     // first); longer runs continue in doubling chunks after a host check
