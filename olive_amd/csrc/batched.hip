@@ -24,6 +24,13 @@
 #include "internal.hpp"
 #include "seg_core.hpp"
 
+// Workgroups per CU of the batched pull kernel, at most one per virtual
+// block of its bucket plan (256 sources on RMAT-22: 48 gives 7.21 ms; 64:
+// 7.21, 96: 7.25, 32: 7.30, 16: 7.44, 8: 8.17).
+#ifndef OLV_MBFS_PULL_PER_CU
+#define OLV_MBFS_PULL_PER_CU 48
+#endif
+
 namespace olv {
 
 int bfs_multisource(const olv_graph *g, const uint32_t *seeds, uint32_t k, bool seeds_host, int32_t *levels,
@@ -658,7 +665,8 @@ static int mbfs_enqueue_levels(const MbfsParams &p, const SegDev &d, uint32_t co
   hipStream_t s = stream();
   const unsigned ge = (unsigned)cu_count() * 8;
   const unsigned gw = mgrid(p.nw, kMB);
-  const unsigned gp = seg_grid(d);
+  const unsigned gp = std::max<unsigned>(
+      1u, std::min<unsigned>(d.role_start[5], (unsigned)cu_count() * OLV_MBFS_PULL_PER_CU));
   for (uint32_t i = 0; i < count; ++i) {
     hipLaunchKernelGGL(mbfs_expand_kernel, dim3(ge), dim3(kMB), 0, s, p);
     hipLaunchKernelGGL(mbfs_pull_kernel, dim3(gp), dim3(kMB), 0, s, p, d);
