@@ -44,9 +44,18 @@ constexpr int kIWaves = kIB / kWave;
 constexpr uint32_t kTextChunk = 64;          // text bytes per lane
 constexpr uint32_t kTextTile = kIB * kTextChunk;
 
-unsigned grid_for(size_t items, size_t per) {
+// Workgroups per CU of the grid-stride ingest kernels, and of the line
+// parser (more lines in flight for its dependent byte reads: RMAT-22 text
+// 2.21 ms with 64, 2.27 with 32, 2.39 with 16).
+#ifndef OLV_INGEST_GRID_PER_CU
+#define OLV_INGEST_GRID_PER_CU 16
+#endif
+#ifndef OLV_PARSE_GRID_PER_CU
+#define OLV_PARSE_GRID_PER_CU 64
+#endif
+unsigned grid_for(size_t items, size_t per, size_t per_cu = OLV_INGEST_GRID_PER_CU) {
   const size_t b = (items + per - 1) / per;
-  return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * 16));
+  return (unsigned)std::max<size_t>(1, std::min(b, (size_t)cu_count() * per_cu));
 }
 
 template <class T>
@@ -994,7 +1003,7 @@ int olv_graph_from_edge_list_text(olv_graph **out, const char *text, size_t len,
   a.err = err.as<unsigned long long>();
   Timer tk;
   tk.start();
-  hipLaunchKernelGGL(edge_parse_kernel, dim3(grid_for(L.nl, kIB)), dim3(kIB), 0, s, a);
+  hipLaunchKernelGGL(edge_parse_kernel, dim3(grid_for(L.nl, kIB, OLV_PARSE_GRID_PER_CU)), dim3(kIB), 0, s, a);
   tk.stop();
   OLV_HIP(hipGetLastError());
   OLV_HIP(hipStreamSynchronize(s));
