@@ -51,7 +51,8 @@ enum {
                                         (slower on MI355X; kept to compare)      */
 #define OLV_PROFILE          0x0010u /* time each launch of the dominant kernel
                                         with hipEvents (olv_stats.kernel_ms)     */
-#define OLV_GRAPH_ORDER      0x0040u /* olv_bfs, olv_bfs_multisource and the
+#define OLV_GRAPH_ORDER      0x0040u /* olv_bfs, olv_bfs_multisource, the
+                                        batched BFS calls and plans and the
                                         PageRank solves (plain, weighted, VJP,
                                         JVP): run in the graph's own vertex
                                         numbering instead of the in-degree

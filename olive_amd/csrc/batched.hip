@@ -75,6 +75,9 @@ struct MbfsParams {
   uint2 *hubs;     // hub items
   uint8_t *S;      // n * sstride  level byte per (vertex, source)
   int32_t *levels; // k * n
+  // on a renumbered twin: its vertex -> levels index (omap) and back (imap);
+  // both nullptr on the graph itself
+  const uint32_t *omap, *imap;
   uint32_t sstride;
   MbfsCtl *ctl;
   unsigned long long m;
@@ -168,7 +171,8 @@ __device__ __forceinline__ void mbfs_record(const MbfsParams &p, uint32_t v, con
   if (nlevel >= 255)
 #pragma unroll
     for (int j = 0; j < kMW; ++j)
-      for (uint32_t x = nwv[j]; x; x &= x - 1) p.levels[(size_t)(j * 32 + __ffs(x) - 1) * p.n + v] = nlevel;
+      for (uint32_t x = nwv[j]; x; x &= x - 1)
+        p.levels[(size_t)(j * 32 + __ffs(x) - 1) * p.n + (p.omap ? p.omap[v] : v)] = nlevel;
 }
 
 __device__ __forceinline__ void mbfs_buffers(const MbfsParams &p, uint32_t L, const uint32_t *&Ff, uint32_t *&Fn,
@@ -550,17 +554,21 @@ __global__ void __launch_bounds__(kMB) mbfs_levels_kernel(MbfsParams p) {
   const uint32_t q4 = p.sstride / 16;  // uint4 per row
   for (uint32_t v0 = blockIdx.x * kLevTile; v0 < p.n; v0 += gridDim.x * kLevTile) {
     const uint32_t rows = min((uint32_t)kLevTile, p.n - v0);
-    const uint4 *src = reinterpret_cast<const uint4 *>(p.S + (size_t)v0 * p.sstride);
+    // output vertex v0 + r reads the row of vertex imap[v0 + r] (a whole
+    // sstride-byte row, so a renumbered twin still reads full lines)
     for (uint32_t q = threadIdx.x; q < rows * q4; q += kMB) {
-      const uint4 x = src[q];
+      const uint32_t ov = v0 + q / q4, tv = p.imap ? p.imap[ov] : ov;
+      const uint4 x = reinterpret_cast<const uint4 *>(p.S + (size_t)tv * p.sstride)[q % q4];
       uint32_t *d = &s_row[q / q4][(q % q4) * 4];
       d[0] = x.x;
       d[1] = x.y;
       d[2] = x.z;
       d[3] = x.w;
     }
-    for (uint32_t q = threadIdx.x; q < rows * kw; q += kMB)
-      s_vis[q / kw][q % kw] = p.V[(size_t)(v0 + q / kw) * kMW + q % kw];
+    for (uint32_t q = threadIdx.x; q < rows * kw; q += kMB) {
+      const uint32_t ov = v0 + q / kw, tv = p.imap ? p.imap[ov] : ov;
+      s_vis[q / kw][q % kw] = p.V[(size_t)tv * kMW + q % kw];
+    }
     __syncthreads();
     const uint32_t t = threadIdx.x % kLevTile;
     if (t < rows) {
@@ -692,6 +700,24 @@ static int mbfs_read(MbfsWork &w) {
 }
 
 // In-degree buckets for pull levels (empty schedule when pull is off).
+// The graph batched BFS runs on: g's renumbered twin unless OLV_GRAPH_ORDER
+// (seeds mapped to the twin's numbering in place), else g itself.
+static int mbfs_target(olv_graph *g, unsigned flags, std::vector<uint32_t> &seeds, olv_graph **run) {
+  *run = g;
+  if (flags & OLV_GRAPH_ORDER) return OLV_OK;
+  OLV_TRY(ensure_twin(g));
+  g->twin->tune = g->tune;
+  for (uint32_t &x : seeds) x = g->h_rl_pi[x];
+  *run = g->twin;
+  return OLV_OK;
+}
+
+static void mbfs_map(const olv_graph *g, const olv_graph *run, MbfsParams &p) {
+  if (run == g) return;
+  p.omap = run->out_map;
+  p.imap = g->rl_pi;
+}
+
 static int mbfs_pull_plan(olv_graph *g, unsigned flags, SegDev *d) {
   std::memset(d, 0, sizeof(*d));
   if (flags & OLV_NO_DIRECTION_OPT) return OLV_OK;
@@ -794,14 +820,17 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_bfs_batched: source %u out of range (n=%u)", srcs[i], g->n);
   if ((uint64_t)k * g->n > (1ull << 40)) return fail(OLV_ERANGE, "olv_bfs_batched: output too large");
+  std::vector<uint32_t> seeds(srcs, srcs + k);
+  olv_graph *rg = nullptr;
+  OLV_TRY(mbfs_target(g, flags, seeds, &rg));
   SegDev d;
-  OLV_TRY(mbfs_pull_plan(g, flags, &d));
+  OLV_TRY(mbfs_pull_plan(rg, flags, &d));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   MbfsWork w;
   OLV_TRY(w.alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m, std::min(k, 256u)));
   DevBuf d_srcs, d_out;
   OLV_TRY(d_srcs.alloc((size_t)k * 4));
-  OLV_HIP(hipMemcpy(d_srcs.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice));
+  OLV_HIP(hipMemcpy(d_srcs.p, seeds.data(), (size_t)k * 4, hipMemcpyHostToDevice));
   int32_t *out = levels;
   if (host) {
     OLV_TRY(d_out.alloc((size_t)k * g->n * 4));
@@ -812,7 +841,8 @@ int olv_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, int32
   uint32_t max_levels = 0, bu_levels = 0;
   for (uint32_t g0 = 0; g0 < k; g0 += 256) {
     const uint32_t kg = std::min(256u, k - g0);
-    MbfsParams p = mbfs_params(g, w, kg, out + (size_t)g0 * g->n);
+    MbfsParams p = mbfs_params(rg, w, kg, out + (size_t)g0 * g->n);
+    mbfs_map(g, rg, p);
     if (flags & OLV_NO_DIRECTION_OPT) p.can_bu = 0;
     OLV_TRY(mbfs_run_group(p, d, w, d_srcs.as<uint32_t>() + g0));
     max_levels = std::max(max_levels, w.h_ctl->levels);
@@ -1020,8 +1050,11 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
   olv_graph *g = const_cast<olv_graph *>(cg);
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_plan_bfs_batched: source %u out of range", srcs[i]);
+  std::vector<uint32_t> seeds(srcs, srcs + k);
+  olv_graph *rg = nullptr;
+  OLV_TRY(mbfs_target(g, flags, seeds, &rg));
   SegDev d;
-  OLV_TRY(mbfs_pull_plan(g, flags, &d));
+  OLV_TRY(mbfs_pull_plan(rg, flags, &d));
   olv_plan *p = new olv_plan();
   p->kind = 2;
   p->seg = d;
@@ -1031,12 +1064,13 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
   p->mw = new MbfsWork();
   int s = p->mw->alloc(g->n, ((g->n + 31) / 32 + 63) / 64 * 64, g->m, std::min(k, 256u));
   if (s == OLV_OK) s = p->seeds.alloc((size_t)k * 4);
-  if (s == OLV_OK && hipMemcpy(p->seeds.p, srcs, (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
+  if (s == OLV_OK && hipMemcpy(p->seeds.p, seeds.data(), (size_t)k * 4, hipMemcpyHostToDevice) != hipSuccess)
     s = fail(OLV_EHIP, "olv_plan_bfs_batched: seed copy failed");
   std::vector<uint32_t> depth;
   for (uint32_t g0 = 0; s == OLV_OK && g0 < k; g0 += 256) {
     const uint32_t kg = std::min(256u, k - g0);
-    MbfsParams prm = mbfs_params(g, *p->mw, kg, levels + (size_t)g0 * g->n);
+    MbfsParams prm = mbfs_params(rg, *p->mw, kg, levels + (size_t)g0 * g->n);
+    mbfs_map(g, rg, prm);
     if (flags & OLV_NO_DIRECTION_OPT) prm.can_bu = 0;
     p->groups.push_back(prm);
     s = mbfs_run_group(prm, p->seg, *p->mw, p->seeds.as<uint32_t>() + g0);  // learns the depth

@@ -36,7 +36,7 @@ def test_bfs_batched_rmat(olv, k):
         assert np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(s))), (k, i, s)
 
 
-@pytest.mark.parametrize("flags", [0, 0x2])
+@pytest.mark.parametrize("flags", [0, 0x2, 0x40])  # 0x40: the graph's own numbering
 def test_bfs_batched_duplicates_and_grid(olv, flags):
     csr = olv.gen_grid(40, 50)
     g = olv.Graph(csr)
@@ -47,7 +47,7 @@ def test_bfs_batched_duplicates_and_grid(olv, flags):
         assert np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(s)))
 
 
-@pytest.mark.parametrize("flags", [0, 0x2])
+@pytest.mark.parametrize("flags", [0, 0x2, 0x40])
 def test_bfs_batched_deep(olv, flags):
     """Levels of 255 and more (a 3 x 300 grid is 301 levels deep from a
     corner) next to shallow ones in the same rows."""
@@ -71,6 +71,9 @@ def test_bfs_batched_rmat18_256(olv):
     got = out.numpy().reshape(256, csr.n)
     for i in range(0, 256, 17):
         assert np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(srcs[i]))), i
+    # on the renumbered twin (default) and in the graph's own numbering
+    own, _ = g.bfs_batched(srcs, flags=0x40)
+    assert np.array_equal(own.numpy().reshape(256, csr.n), got)
 
 
 def test_sssp_batched(olv):
