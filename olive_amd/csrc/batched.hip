@@ -422,13 +422,14 @@ __global__ void __launch_bounds__(kMB) mbfs_contract_kernel(MbfsParams p) {
   const int32_t nlevel = (int32_t)(L + 1);
   uint32_t nf = 0;
   unsigned long long mf = 0;
-  for (uint32_t w = blockIdx.x * kMB + threadIdx.x; w < p.nw; w += gridDim.x * kMB) {
-    uint32_t tw = Tc[w];
-    uint32_t anext = 0, dnew = 0;
-    while (tw) {
-      const uint32_t b = __ffs(tw) - 1;
-      tw &= tw - 1;
-      const uint32_t v = w * 32u + b;
+  // One lane per vertex (touched vertices cluster in few words on a
+  // renumbered graph, so a lane per word left most lanes idle); a wave
+  // covers two bitmap words, whose new bits lanes 0 and 32 write.
+  const uint32_t lane = lane_id();
+  for (uint32_t v = blockIdx.x * kMB + threadIdx.x; v < p.nw * 32u; v += gridDim.x * kMB) {
+    const uint32_t w = v >> 5, b = v & 31u;
+    bool act = false, dn = false;
+    if ((Tc[w] >> b) & 1u) {
       uint32_t nwv[kMW], vv[kMW];
       bool any = false;
 #pragma unroll
@@ -439,16 +440,21 @@ __global__ void __launch_bounds__(kMB) mbfs_contract_kernel(MbfsParams p) {
         nwv[j] = nj & ~vv[j];
         any = any || nwv[j];
       }
-      if (!any) continue;
-      if (mbfs_apply(p, v, nwv, vv, Fn)) dnew |= 1u << b;
-      anext |= 1u << b;
-      nf += 1;
-      mf += p.row_ptr[v + 1] - p.row_ptr[v];
-      mbfs_record(p, v, nwv, nlevel);
+      if (any) {
+        dn = mbfs_apply(p, v, nwv, vv, Fn);
+        act = true;
+        nf += 1;
+        mf += p.row_ptr[v + 1] - p.row_ptr[v];
+        mbfs_record(p, v, nwv, nlevel);
+      }
     }
-    // |=: on a pull level the pull kernel has already set bits of this word
-    if (anext) An[w] |= anext;
-    if (dnew) p.D[w] |= dnew;
+    const uint64_t am = __ballot(act), dm = __ballot(dn);
+    if ((lane & 31u) == 0) {
+      const uint32_t anext = (uint32_t)(am >> (lane & 32u)), dnew = (uint32_t)(dm >> (lane & 32u));
+      // |=: on a pull level the pull kernel has already set bits of this word
+      if (anext) An[w] |= anext;
+      if (dnew) p.D[w] |= dnew;
+    }
   }
   flush_counters<kMB>(&ctl->cnt[L % 3], nf, mf, 0ull);
 }
@@ -673,12 +679,13 @@ static int mbfs_enqueue_levels(const MbfsParams &p, const SegDev &d, uint32_t co
   hipStream_t s = stream();
   const unsigned ge = (unsigned)cu_count() * 8;
   const unsigned gw = mgrid(p.nw, kMB);
+  const unsigned gv = mgrid((size_t)p.nw * 32, kMB);  // one lane per vertex
   const unsigned gp = std::max<unsigned>(
       1u, std::min<unsigned>(d.role_start[5], (unsigned)cu_count() * OLV_MBFS_PULL_PER_CU));
   for (uint32_t i = 0; i < count; ++i) {
     hipLaunchKernelGGL(mbfs_expand_kernel, dim3(ge), dim3(kMB), 0, s, p);
     hipLaunchKernelGGL(mbfs_pull_kernel, dim3(gp), dim3(kMB), 0, s, p, d);
-    hipLaunchKernelGGL(mbfs_contract_kernel, dim3(gw), dim3(kMB), 0, s, p);
+    hipLaunchKernelGGL(mbfs_contract_kernel, dim3(gv), dim3(kMB), 0, s, p);
     hipLaunchKernelGGL(mbfs_compact_kernel, dim3(gw), dim3(kMB), 0, s, p);
   }
   OLV_HIP(hipGetLastError());
