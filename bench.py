@@ -5,7 +5,17 @@ Default workload (`--workload pagerank`): PageRank on RMAT scale 20
 dropped, seed 1), damping 0.85, exactly 20 iterations, fp32 ranks with fp32
 accumulation -- the configuration the headline metric is quoted on.  One
 step = one full 20-iteration PageRank over the HBM-resident graph.  The line
-also carries a `secondary` BFS measurement on RMAT scale 22.
+also carries `secondary` measurements of the other targets (skip them with
+--no-secondary):
+  bfs_rmat22      BFS on RMAT-22 (8 seeded sources), with the cost of the
+                  first call (renumbered twin built) and the graph-order time
+  grid128         BFS on the 128 x 128 grid from vertex 0: the oracle on one
+                  host core (the configuration's CPU path) and the GPU
+  batched_256     256-source BFS on RMAT-22 by plan replay vs 256 plain calls
+  multigraph_64   64 RMAT graphs (scales 12..18) in one captured plan vs 64
+                  plain calls
+  adjoint_rmat24  weighted PageRank forward + VJP on RMAT-24 vs forward-only
+                  with central finite differences
 
 Prints one JSON line.  `value` is edges/s = E * 20 * steps / time.
 `roofline` covers the dominant kernel (the fused pull iteration):
@@ -72,14 +82,63 @@ def roofline(bytes_per_launch, avg_launch_s, key, kernel):
             "bytes_per_launch": bytes_per_launch, "avg_launch_us": avg_launch_s * 1e6}
 
 
+_CPU = {"build": "prebuilt liboracle.so (-O3, baseline x86-64)"}
+
+
+def cpu_setup():
+    """The CPU baseline's oracle: oracle.c built here with -O3 -march=native
+    (the host that runs this), falling back to the prebuilt portable build."""
+    import tempfile
+    import oracle
+    if _CPU.get("native_tried"):
+        return
+    _CPU["native_tried"] = True
+    try:
+        path = oracle.build_native(tempfile.mkdtemp(prefix="olive_oracle_"))
+        oracle.use_library(path)
+        _CPU["build"] = "gcc -O3 -march=native on this host"
+    except Exception as e:  # no compiler: keep the portable build
+        log(f"native oracle build failed ({e}); using the prebuilt one")
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model
+
+
 def timed_oracle(fn, budget_s, max_runs=5):
+    """Median wall time of up to max_runs calls within budget_s, with this
+    process pinned to one core (the oracle is serial)."""
+    cpu_setup()
+    allowed = os.sched_getaffinity(0)
+    core = min(allowed)
+    os.sched_setaffinity(0, {core})
     times = []
-    t_start = time.perf_counter()
-    while len(times) < max_runs and (time.perf_counter() - t_start) < budget_s:
-        t0 = time.perf_counter()
-        fn()
-        times.append(time.perf_counter() - t0)
+    try:
+        t_start = time.perf_counter()
+        while len(times) < max_runs and (time.perf_counter() - t_start) < budget_s:
+            t0 = time.perf_counter()
+            fn()
+            times.append(time.perf_counter() - t0)
+    finally:
+        os.sched_setaffinity(0, allowed)
+    _CPU["core"] = core
     return float(np.median(times)), len(times)
+
+
+def cpu_line(value, unit, sample):
+    """cpu_baseline object: one pinned core, the host described."""
+    return {"value": value, "unit": unit, "cores": 1, "kind": "port", "sample": sample,
+            "build": _CPU["build"], "pinned_core": _CPU.get("core"), "host_cpu": host_cpu(),
+            "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))}
 
 
 def sources_of(csr, k, seed):
@@ -146,12 +205,19 @@ def bench_pagerank(args, olv):
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         t, runs = timed_oracle(lambda: oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0), 20.0)
-        res["cpu_baseline"] = {"value": m * iters / t, "unit": "edges/s", "cores": 1, "kind": "port",
-                               "sample": f"full workload (RMAT-20, {iters} iterations), serial fp64 restatement of "
-                                         f"testPageRank.cu:57-104, median of {runs} runs"}
+        res["cpu_baseline"] = cpu_line(m * iters / t, "edges/s",
+                                       f"full workload (RMAT-20, {iters} iterations), serial fp64 restatement of "
+                                       f"testPageRank.cu:57-104, median of {runs} runs")
     if not args.no_secondary:
-        del g
-        res["secondary"] = {"bfs_rmat22": bfs_rmat22(olv, args, cpu=False)[0]}
+        del g, out
+        sec = {}
+        r22, csr22, g22, _ = bfs_rmat22(olv, args, cpu=not args.no_cpu_baseline)
+        sec["bfs_rmat22"] = r22
+        sec["grid128"] = grid128(olv, args)
+        sec.update(batched_legs(olv, args, csr22, g22, sssp=False))
+        del g22, csr22
+        sec["adjoint_rmat24"] = adjoint_leg(olv, args)
+        res["secondary"] = sec
     return res
 
 
@@ -163,6 +229,18 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
     g = olv.Graph(csr)
     srcs = sources_of(csr, nsrc, 22)
     out = olv.DeviceBuffer(csr.n, np.int32)
+    L = olv._lib.lib()
+    # first call on the graph: work buffers and the in-degree renumbered twin
+    # are built inside it (wall clock, then one steady call for comparison)
+    L.olv_synchronize()
+    t0 = time.perf_counter()
+    olv._lib.check(L.olv_bfs(g.h, int(srcs[0]), out.c, 0, None))
+    L.olv_synchronize()
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    olv._lib.check(L.olv_bfs(g.h, int(srcs[0]), out.c, 0, None))
+    L.olv_synchronize()
+    t_steady = time.perf_counter() - t0
     for s in srcs[:2]:
         for _ in range(max(1, args.warmup)):
             g.bfs(int(s), out=out)
@@ -182,7 +260,9 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
     ok = bool(np.array_equal(out.numpy(), check))
     r = {"gteps_harmonic": gteps_h, "hbm_frac_mean": frac, "ms_mean": float(t.mean() * 1e3),
          "edges": csr.m, "sources": len(srcs), "levels": [p[3] for p in per], "bottom_up_levels": [p[4] for p in per],
-         "bit_exact_last_source": ok}
+         "bit_exact_last_source": ok, "first_call_ms": t_first * 1e3, "steady_call_ms": t_steady * 1e3,
+         "twin_build_ms": (t_first - t_steady) * 1e3,
+         "twin_build_in_traversals": (t_first - t_steady) / max(float(t.mean()), 1e-9)}
     # the same sources through the other level drivers (same results)
     modes = {"host_loop": olv.OLV_HOST_LOOP, "persistent": olv.OLV_PERSISTENT, "push_only": olv.OLV_NO_DIRECTION_OPT,
              "graph_order": olv.OLV_GRAPH_ORDER}
@@ -193,9 +273,13 @@ def bfs_rmat22(olv, args, cpu=True, nsrc=8):
             g.bfs(int(s), out=out, flags=fl)
             ts.append(g.bfs(int(s), out=out, flags=fl)[1].elapsed_ms)
         r["modes_ms_mean"][name] = float(np.mean(ts))
+    if "graph_order" in r["modes_ms_mean"]:
+        r["graph_order_ms_mean"] = r["modes_ms_mean"]["graph_order"]
     if cpu:
         tc, runs = timed_oracle(lambda: oracle.bfs(csr.row_ptr, csr.col, int(srcs[0])), 20.0)
         r["cpu_gteps"] = float(er[0] / tc / 1e9)
+        r["cpu_baseline"] = cpu_line(r["cpu_gteps"], "GTEPS", f"one source of the same graph, serial restatement "
+                                     f"of testBFS.cu:56-79, median of {runs}")
     return r, csr, g, srcs
 
 
@@ -209,9 +293,8 @@ def bench_bfs(args, olv):
                        "unit": "GB/s", "frac": r["hbm_frac_mean"], "traffic": load_traffic("bfs_rmat22"),
                        "kernel": "bfs_expand_kernel + bfs_compact_kernel (whole traversal)",
                        "bytes_model": "8*E_r + 16*V_r + 4*V"}
-    if "cpu_gteps" in r:
-        res["cpu_baseline"] = {"value": r["cpu_gteps"], "unit": "GTEPS", "cores": 1, "kind": "port",
-                               "sample": "one source of the same graph, serial restatement of testBFS.cu:56-79"}
+    if "cpu_baseline" in r:
+        res["cpu_baseline"] = r.pop("cpu_baseline")
     res["detail"] = r
     return res
 
@@ -243,33 +326,44 @@ def bench_sssp(args, olv):
                        "bytes_model": "12*E_r + 16*V_r + 4*V"}
     if not args.no_cpu_baseline:
         tc, runs = timed_oracle(lambda: oracle.sssp(csr.row_ptr, csr.col, csr.weights, 0), 30.0, 2)
-        res["cpu_baseline"] = {"value": er / tc / 1e9, "unit": "GTEPS", "cores": 1, "kind": "port",
-                               "sample": f"full workload, serial frontier relaxation (SSSP.cu:146-163), "
-                                         f"median of {runs}"}
+        td, druns = timed_oracle(lambda: oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 0), 30.0, 2)
+        res["cpu_baseline"] = cpu_line(er / tc / 1e9, "GTEPS", f"full workload, serial frontier relaxation "
+                                       f"(SSSP.cu:146-163), median of {runs}")
+        res["cpu_baseline"]["dijkstra_gteps"] = er / td / 1e9
     return res
 
 
 # ------------------------------------------------------------------- batched
-def bench_batched(args, olv):
+def batched_legs(olv, args, csr, g, sssp=True):
+    """256-source BFS on RMAT-22 by plan replay vs the plain engine, the
+    64-graph batch in one captured plan vs 64 plain calls, and (sssp) 16-source
+    SSSP on the uniform graph batched vs plain.  Returns a dict of legs."""
     import oracle
     L = olv._lib.lib()
-    log("generating RMAT-22 ...")
-    csr = olv.gen_rmat(22, 16, seed=1)
-    g = olv.Graph(csr)
+    reps = max(3, args.steps // 4)
     srcs = sources_of(csr, 256, 4)
     out = olv.DeviceBuffer(csr.n * 256, np.int32)
     plan = ctypes.c_void_p()
+    t0 = time.perf_counter()
     olv._lib.check(L.olv_plan_bfs_batched(g.h, srcs.ctypes.data, 256, out.c, 0, ctypes.byref(plan)))
+    L.olv_synchronize()
+    t_capture = time.perf_counter() - t0
     st = olv.Stats()
     for _ in range(args.warmup):
         olv._lib.check(L.olv_plan_launch(plan, ctypes.byref(st)))
-    tb = []
-    for _ in range(max(3, args.steps // 4)):
+    tb, tw = [], []
+    for _ in range(reps):
+        L.olv_synchronize()
+        t0 = time.perf_counter()
         olv._lib.check(L.olv_plan_launch(plan, ctypes.byref(st)))
+        L.olv_synchronize()
+        tw.append(time.perf_counter() - t0)
         tb.append(st.elapsed_ms / 1e3)
-    t_batched = float(np.median(tb))
+    t_batched, t_batched_wall = float(np.median(tb)), float(np.median(tw))
     got = out.numpy().reshape(256, csr.n)
-    exact = all(np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(srcs[i]))) for i in (0, 97, 255))
+    rows = (0, 97, 255)
+    exact = all(np.array_equal(got[i], oracle.bfs(csr.row_ptr, csr.col, int(srcs[i]))) for i in rows)
+    del got
     # plain engine, one source at a time (same sources): wall clock over 256
     # C-ABI calls without statistics, and the sum of their device times
     # (hipEvents from reset to last level: the plain engine's best case)
@@ -285,8 +379,17 @@ def bench_batched(args, olv):
         olv._lib.check(L.olv_bfs(g.h, int(s), one.c, 0, None))
     L.olv_synchronize()
     t_plain = time.perf_counter() - t0
+    wt, bt, al, be = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float(), ctypes.c_float()
+    olv._lib.check(L.olv_get_tuning(g.h, ctypes.byref(wt), ctypes.byref(bt), ctypes.byref(al), ctypes.byref(be)))
+    tuning = {"wave_thr": wt.value, "block_thr": bt.value, "alpha": al.value, "beta": be.value}
     L.olv_plan_destroy(plan)
-    del out
+    del out, one
+    legs = {"batched_256": {
+        "graph": "rmat22", "sources": 256, "batched_ms": t_batched * 1e3, "batched_wall_ms": t_batched_wall * 1e3,
+        "capture_ms": t_capture * 1e3, "plain_256_wall_ms": t_plain * 1e3, "plain_256_device_ms": t_plain_dev * 1e3,
+        "speedup_wall": t_plain / t_batched_wall, "speedup_device": t_plain_dev / t_batched,
+        "gteps": er_tot / t_batched / 1e9, "edges_traversed": int(er_tot), "bit_exact_sampled_rows": list(rows),
+        "bit_exact": bool(exact), "tuning": tuning}}
     # 64 RMAT graphs, scales uniform in [12, 18], one captured launch
     log("generating the 64-graph batch ...")
     csrs = [olv.gen_rmat(olv.batch_scale(1, i, 12, 18), 16, seed=olv.batch_seed(1, i)) for i in range(64)]
@@ -302,14 +405,18 @@ def bench_batched(args, olv):
     olv._lib.check(L.olv_plan_bfs(uni, seeds.ctypes.data, 64, uout.c, 0, ctypes.byref(uplan)))
     for _ in range(args.warmup):
         olv._lib.check(L.olv_plan_launch(uplan, ctypes.byref(st)))
-    tu = []
-    for _ in range(max(3, args.steps // 4)):
+    tu, tuw = [], []
+    for _ in range(reps):
+        L.olv_synchronize()
+        t0 = time.perf_counter()
         olv._lib.check(L.olv_plan_launch(uplan, ctypes.byref(st)))
+        L.olv_synchronize()
+        tuw.append(time.perf_counter() - t0)
         tu.append(st.elapsed_ms / 1e3)
-    t_union = float(np.median(tu))
+    t_union, t_union_wall = float(np.median(tu)), float(np.median(tuw))
     uo = uout.numpy()
     uexact = all(np.array_equal(uo[offs[i]:offs[i + 1]], oracle.bfs(csrs[i].row_ptr, csrs[i].col, int(bsrc[i])))
-                 for i in (0, 31, 63))
+                 for i in range(64))
     # plain engine, one graph at a time: outputs allocated and every graph
     # run once beforehand (work buffers, renumbered twin), then timed
     outs = [olv.DeviceBuffer(c.n, np.int32) for c in csrs]
@@ -323,6 +430,13 @@ def bench_batched(args, olv):
     t_plain_multi = time.perf_counter() - t0
     L.olv_plan_destroy(uplan)
     L.olv_graph_destroy(uni)
+    legs["multigraph_64"] = {
+        "graphs": 64, "scales": [12, 18], "vertices": int(offs[64]), "edges": int(sum(c.m for c in csrs)),
+        "plan_ms": t_union * 1e3, "plan_wall_ms": t_union_wall * 1e3, "plain_64_wall_ms": t_plain_multi * 1e3,
+        "speedup_wall": t_plain_multi / t_union_wall, "bit_exact_all_graphs": bool(uexact)}
+    del gs, outs, uout
+    if not sssp:
+        return legs
     # 16-source SSSP on the uniform 2^22 x 2^25 graph, batched vs plain
     log("generating uniform 2^22 x 2^25 ...")
     ucsr = olv.with_weights_i32(olv.gen_uniform(1 << 22, 1 << 25, seed=1), 1, 64, seed=1)
@@ -330,7 +444,7 @@ def bench_batched(args, olv):
     ssrc = sources_of(ucsr, 16, 5)
     sout, sst = ug.sssp_batched(ssrc)
     ts = []
-    for _ in range(max(3, args.steps // 4)):
+    for _ in range(reps):
         sout, sst = ug.sssp_batched(ssrc, out=sout)
         ts.append(sst.elapsed_ms / 1e3)
     t_sssp_b = float(np.median(ts))
@@ -344,22 +458,62 @@ def bench_batched(args, olv):
         t_sssp_p += s1.elapsed_ms / 1e3
         if i in (0, 15):
             sexact = sexact and bool(np.array_equal(sb[i], one.numpy()))
-    del sout, one
+    legs["sssp_batched_16"] = {"graph": "uniform_4M_33M", "sources": 16, "batched_ms": t_sssp_b * 1e3,
+                               "plain_16_device_ms": t_sssp_p * 1e3, "speedup": t_sssp_p / t_sssp_b,
+                               "batched_equals_plain": sexact}
+    return legs
+
+
+def bench_batched(args, olv):
+    log("generating RMAT-22 ...")
+    csr = olv.gen_rmat(22, 16, seed=1)
+    g = olv.Graph(csr)
+    legs = batched_legs(olv, args, csr, g, sssp=True)
+    b = legs["batched_256"]
     cfg = {"workload": "batched_bfs_rmat22_256src_plus_64_graphs", "vertices": csr.n, "edges": csr.m,
-           "sources": 256, "bit_exact_sampled": bool(exact and uexact),
-           "multigraph_vertices": int(offs[64]), "multigraph_edges": int(sum(c.m for c in csrs))}
-    res = base_line(args, cfg, er_tot / t_batched / 1e9, "GTEPS", t_batched * 1e3, "int32")
-    res["detail"] = {"batched_256_ms": t_batched * 1e3, "plain_256_ms": t_plain * 1e3,
-                     "speedup_256": t_plain / t_batched, "plain_256_device_ms": t_plain_dev * 1e3,
-                     "speedup_256_device": t_plain_dev / t_batched, "multigraph_ms": t_union * 1e3,
-                     "plain_64_graphs_ms": t_plain_multi * 1e3, "speedup_64_graphs": t_plain_multi / t_union,
-                     "sssp_batched_16_uniform_ms": t_sssp_b * 1e3, "sssp_plain_16_uniform_ms": t_sssp_p * 1e3,
-                     "speedup_sssp_16": t_sssp_p / t_sssp_b, "sssp_batched_equals_plain": sexact}
+           "sources": 256, "bit_exact_sampled": bool(b["bit_exact"] and legs["multigraph_64"]["bit_exact_all_graphs"]),
+           "multigraph_vertices": legs["multigraph_64"]["vertices"],
+           "multigraph_edges": legs["multigraph_64"]["edges"]}
+    res = base_line(args, cfg, b["gteps"], "GTEPS", b["batched_ms"], "int32")
+    res["detail"] = legs
     return res
 
 
+# --------------------------------------------------------------------- grid
+def grid128(olv, args):
+    """BFS on the 128 x 128 directed grid from vertex 0 (255 levels): the
+    oracle on one host core -- this configuration's CPU reference path --
+    and the GPU engine, both over the same graph."""
+    import oracle
+    csr = olv.gen_grid(128, 128)
+    g = olv.Graph(csr)
+    out = olv.DeviceBuffer(csr.n, np.int32)
+    for _ in range(max(1, args.warmup)):
+        g.bfs(0, out=out)
+    ts = []
+    for _ in range(max(5, args.steps // 2)):
+        _, st = g.bfs(0, out=out)
+        ts.append(st.elapsed_ms / 1e3)
+    t_gpu = float(np.median(ts))
+    ref = oracle.bfs(csr.row_ptr, csr.col, 0)
+    lv = out.numpy()
+    closed = np.add.outer(np.arange(128), np.arange(128)).ravel()
+    r = {"vertices": csr.n, "edges": csr.m, "levels": int(lv.max()) + 1, "gpu_ms": t_gpu * 1e3,
+         "gpu_gteps": csr.m / t_gpu / 1e9, "bit_exact": bool(np.array_equal(lv, ref) and np.array_equal(lv, closed))}
+    if not args.no_cpu_baseline:
+        tc, runs = timed_oracle(lambda: oracle.bfs(csr.row_ptr, csr.col, 0), 5.0, max_runs=5)
+        r["cpu_ms"] = tc * 1e3
+        r["cpu_baseline"] = cpu_line(csr.m / tc / 1e9, "GTEPS", f"the whole workload, serial restatement of "
+                                     f"testBFS.cu:56-79, median of {runs}")
+    return r
+
+
 # ------------------------------------------------------------------- adjoint
-def bench_adjoint(args, olv, scale=24):
+def adjoint_leg(olv, args, scale=24):
+    """Weighted PageRank forward + VJP on RMAT-`scale` (tol 1e-6) against
+    forward-only with central finite differences (2 forward solves per
+    direction).  Repeated calls pass OLV_W_UNCHANGED (the weights are the
+    same), so the weights are put in slot order once per solver."""
     log(f"generating weighted RMAT-{scale} ...")
     csr = olv.with_weights_f32_normalized(olv.gen_rmat(scale, 16, seed=1), seed=1)
     t = olv.gen_dirichlet(csr.n, seed=1)
@@ -368,34 +522,51 @@ def bench_adjoint(args, olv, scale=24):
     T = olv.DeviceBuffer.from_numpy(t)
     rng = np.random.default_rng(1)
     gbar = olv.DeviceBuffer.from_numpy(rng.standard_normal(csr.n).astype(np.float32))
+    U = olv._lib.OLV_W_UNCHANGED
+    t0 = time.perf_counter()
     r, st = g.pagerank_weighted(W, T, tol=1e-6)
-    fwd = []
-    adj = []
+    _, _, _, sa = g.pagerank_vjp(W, r, gbar, tol=1e-6)
+    olv._lib.lib().olv_synchronize()
+    t_first = time.perf_counter() - t0
+    fwd, adj = [], []
     for _ in range(max(2, args.steps // 4)):
-        r, st = g.pagerank_weighted(W, T, tol=1e-6, out=r)
+        r, st = g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=U)
         fwd.append(st.elapsed_ms)
-        _, _, _, sa = g.pagerank_vjp(W, r, gbar, tol=1e-6)
+        _, _, _, sa = g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=U)
         adj.append(sa.elapsed_ms)
     t_f, t_a = float(np.median(fwd)), float(np.median(adj))
+    # the same calls re-permuting the weights every time (no reuse flag)
+    r, st0 = g.pagerank_weighted(W, T, tol=1e-6, out=r)
+    _, _, _, sa0 = g.pagerank_vjp(W, r, gbar, tol=1e-6)
     # one directional derivative by central differences = 2 forward solves
     t_fd_dir = 2 * t_f
-    # full gradient by FD needs 2 solves per parameter (E + V parameters)
-    k5 = (t_f + t_a) * 5 / t_fd_dir
-    cfg = {"workload": f"pagerank_fwd_adjoint_rmat{scale}", "vertices": csr.n, "edges": csr.m, "tol": 1e-6,
-           "fwd_iterations": st.iterations, "adj_iterations": sa.iterations}
-    res = base_line(args, cfg, csr.m * (st.iterations + sa.iterations + 1) / ((t_f + t_a) / 1e3), "edges/s",
-                    t_f + t_a, "f32")
-    res["detail"] = {"forward_ms": t_f, "vjp_ms": t_a, "fd_per_direction_ms": t_fd_dir,
-                     "fd_full_gradient_ms": t_fd_dir * (csr.m + csr.n),
-                     "speedup_full_gradient_vs_fd": t_fd_dir * (csr.m + csr.n) / (t_f + t_a),
-                     "fd_directions_for_5x": k5}
+    it_f = st.iterations
+    d = {"vertices": csr.n, "edges": csr.m, "tol": 1e-6, "forward_ms": t_f, "vjp_ms": t_a,
+         "forward_iterations": it_f, "vjp_iterations": sa.iterations,
+         "forward_iteration_ms": t_f / max(it_f, 1),
+         "forward_iteration_hbm_frac": (12 * csr.m + 12 * csr.n) / (t_f / max(it_f, 1) / 1e3) / 1e9 / HBM_PEAK_GBS,
+         "forward_ms_repermuting_weights": st0.elapsed_ms, "vjp_ms_repermuting_weights": sa0.elapsed_ms,
+         "first_calls_ms": t_first * 1e3, "fd_per_direction_ms": t_fd_dir,
+         "fd_full_gradient_ms": t_fd_dir * (csr.m + csr.n),
+         "speedup_full_gradient_vs_fd": t_fd_dir * (csr.m + csr.n) / (t_f + t_a),
+         "fd_directions_for_5x": (t_f + t_a) * 5 / t_fd_dir}
     if not args.no_modes:
-        # the same solves in the graph's own numbering (no renumbered CSRs)
         go = olv.OLV_GRAPH_ORDER
         g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=go)
         g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=go)
-        res["detail"]["graph_order_forward_ms"] = g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=go)[1].elapsed_ms
-        res["detail"]["graph_order_vjp_ms"] = g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=go)[3].elapsed_ms
+        d["graph_order_forward_ms"] = g.pagerank_weighted(W, T, tol=1e-6, out=r, flags=go | U)[1].elapsed_ms
+        d["graph_order_vjp_ms"] = g.pagerank_vjp(W, r, gbar, tol=1e-6, flags=go)[3].elapsed_ms
+    return d
+
+
+def bench_adjoint(args, olv, scale=24):
+    d = adjoint_leg(olv, args, scale)
+    cfg = {"workload": f"pagerank_fwd_adjoint_rmat{scale}", "vertices": d["vertices"], "edges": d["edges"],
+           "tol": 1e-6, "fwd_iterations": d["forward_iterations"], "adj_iterations": d["vjp_iterations"]}
+    t_f, t_a = d["forward_ms"], d["vjp_ms"]
+    res = base_line(args, cfg, d["edges"] * (d["forward_iterations"] + d["vjp_iterations"] + 1) / ((t_f + t_a) / 1e3),
+                    "edges/s", t_f + t_a, "f32")
+    res["detail"] = d
     return res
 
 
@@ -460,8 +631,8 @@ def bench_ingest(args, olv):
     if not args.no_cpu_baseline:
         # the oracle's C loader (csrGraph.h:106-171 restated + stable counting sort), same text, one core
         t_cpu, runs = timed_oracle(lambda: oracle.load_edge_list_text(text), 30.0, max_runs=2)
-        res["cpu_baseline"] = {"value": csr.m / t_cpu, "unit": "edges/s", "cores": 1, "kind": "port",
-                               "sample": f"the whole RMAT-22 text, oracle.c orc_edge_list_load, median of {runs}"}
+        res["cpu_baseline"] = cpu_line(csr.m / t_cpu, "edges/s",
+                                       f"the whole RMAT-22 text, oracle.c orc_edge_list_load, median of {runs}")
     return res
 
 
@@ -508,9 +679,8 @@ def bench_egress(args, olv):
         with open(cpath, "rb") as a, open(path, "rb") as b:
             res["config"]["bit_exact"] = bool(exact and a.read() == b.read())
         os.unlink(cpath)
-        res["cpu_baseline"] = {"value": csr.n / t_cpu, "unit": "lines/s", "cores": 1, "kind": "port",
-                               "sample": f"the same levels, oracle.c orc_print_values (fprintf per vertex), "
-                                         f"median of {runs}"}
+        res["cpu_baseline"] = cpu_line(csr.n / t_cpu, "lines/s", f"the same levels, oracle.c orc_print_values "
+                                       f"(fprintf per vertex), median of {runs}")
     os.unlink(path)
     os.rmdir(tmpd)
     return res

@@ -58,6 +58,13 @@ enum {
                                         numbering instead of the in-degree
                                         renumbering (same results, bit for bit) */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
+#define OLV_W_UNCHANGED      0x0200u /* weighted PageRank, VJP, JVP: the weights
+                                        at this address are the ones an earlier
+                                        call on this graph received, unchanged;
+                                        reuse the copy of them that call put in
+                                        the solver's slot order (ignored for
+                                        host pointers or when no such copy is
+                                        kept)                                    */
 
 /* weight dtypes for olv_graph_create */
 enum { OLV_W_NONE = 0, OLV_W_I32 = 1, OLV_W_F32 = 2 };

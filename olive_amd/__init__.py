@@ -12,7 +12,10 @@ distances use INF = 0x7fffffff (BFS.cu:101, SSSP.cu:123).
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
+import sys
+import weakref
 
 import numpy as np
 
@@ -31,6 +34,36 @@ __all__ = [
 ]
 
 _init_done = False
+# Live device objects, released by _release_all at interpreter exit while the
+# HIP runtime is still up; after that (and during interpreter finalisation)
+# no native free is attempted.
+_live: "weakref.WeakSet" = weakref.WeakSet()
+_released = False
+
+
+def _release_all():
+    """atexit hook: wait for queued work, then free every live graph and
+    buffer in this order (graphs first), before Python's own teardown and the
+    runtime's exit handlers run."""
+    global _released
+    if _released or not _init_done:
+        _released = True
+        return
+    try:
+        lib().olv_synchronize()
+        objs = list(_live)
+        for o in objs:
+            if isinstance(o, Graph):
+                o.close()
+        for o in objs:
+            if isinstance(o, DeviceBuffer):
+                o.free()
+    finally:
+        _released = True
+
+
+def _native_free_allowed():
+    return not _released and not sys.is_finalizing()
 
 
 def init(device: int = 0):
@@ -39,6 +72,7 @@ def init(device: int = 0):
     if not _init_done:
         check(lib().olv_init(device), "olv_init")
         _init_done = True
+        atexit.register(_release_all)
 
 
 def _np_ptr(a):
@@ -57,6 +91,7 @@ class DeviceBuffer:
         p = ctypes.c_void_p()
         check(lib().olv_malloc(ctypes.byref(p), max(self.nbytes, 4)), "olv_malloc")
         self.ptr = p.value
+        _live.add(self)
 
     @classmethod
     def from_numpy(cls, a):
@@ -81,7 +116,8 @@ class DeviceBuffer:
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib().olv_free(self.ptr)
+            if _native_free_allowed():
+                lib().olv_free(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -420,6 +456,7 @@ class Graph:
         check(lib().olv_graph_create(ctypes.byref(h), csr.n, csr.m, csr.row_ptr.ctypes.data,
                                      _np_ptr(csr.col), _np_ptr(csr.weights), wd, flags), "olv_graph_create")
         self.h = h
+        _live.add(self)
 
     @classmethod
     def _adopt(cls, h, csr: CsrGraph | None = None):
@@ -427,6 +464,7 @@ class Graph:
         downloaded unless given."""
         g = cls.__new__(cls)
         g.h = h
+        _live.add(g)
         n, m = ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().olv_graph_counts(h, ctypes.byref(n), ctypes.byref(m)), "olv_graph_counts")
         g.n, g.m = n.value, m.value
@@ -459,7 +497,8 @@ class Graph:
 
     def close(self):
         if getattr(self, "h", None) and self.h.value:
-            lib().olv_graph_destroy(self.h)
+            if _native_free_allowed():
+                lib().olv_graph_destroy(self.h)
             self.h = ctypes.c_void_p()
 
     def __del__(self):

@@ -22,8 +22,10 @@ from __future__ import annotations
 
 import contextlib
 import ctypes
+import weakref
 
 from . import Graph, check, lib
+from ._lib import OLV_W_UNCHANGED
 
 __all__ = ["PageRankFunction", "pagerank"]
 
@@ -52,6 +54,19 @@ def _check_vec(x, size, name):
     return x.contiguous()
 
 
+def _weights_flag(graph, w):
+    """OLV_W_UNCHANGED when `w` is the tensor object the previous weighted
+    call on this graph received, at the same address, and torch's version
+    counter shows no in-place change since: the library may then reuse the
+    copy of the weights it already put in its solvers' slot order.  Writes
+    that bypass torch (raw pointers) are not seen, as with any autograd
+    version check."""
+    last = getattr(graph, "_w_seen", None)
+    same = last is not None and last[0]() is w and last[1] == w._version and last[2] == w.data_ptr()
+    graph._w_seen = (weakref.ref(w), w._version, w.data_ptr())
+    return OLV_W_UNCHANGED if same else 0
+
+
 class PageRankFunction:
     """`PageRankFunction.apply(graph, w, t, damping, tol, max_iters)` -> r.
 
@@ -78,7 +93,8 @@ def _make_function():
             r = torch.empty(graph.n, dtype=torch.float32, device=t.device)
             with _torch_stream(t.device):
                 check(lib().olv_pagerank_weighted_f32(graph.h, w.data_ptr(), t.data_ptr(), damping, tol, max_iters,
-                                                      r.data_ptr(), 0, None), "olv_pagerank_weighted_f32")
+                                                      r.data_ptr(), _weights_flag(graph, w), None),
+                      "olv_pagerank_weighted_f32")
             ctx.graph = graph
             ctx.hyper = (damping, tol, max_iters)
             ctx.save_for_backward(w, r)
@@ -96,7 +112,8 @@ def _make_function():
             lam = torch.empty_like(r)
             with _torch_stream(r.device):
                 check(lib().olv_pagerank_vjp_f32(graph.h, w.data_ptr(), r.data_ptr(), gbar.data_ptr(), damping, tol,
-                                                 max_iters, gw.data_ptr(), gt.data_ptr(), lam.data_ptr(), 0, None),
+                                                 max_iters, gw.data_ptr(), gt.data_ptr(), lam.data_ptr(),
+                                                 _weights_flag(graph, w), None),
                       "olv_pagerank_vjp_f32")
             return None, gw, gt, None, None, None
 
@@ -110,7 +127,8 @@ def _make_function():
             dr = torch.empty_like(r)
             with _torch_stream(r.device):
                 check(lib().olv_pagerank_jvp_f32(graph.h, w.data_ptr(), r.data_ptr(), dw.data_ptr(), dt.data_ptr(),
-                                                 damping, tol, max_iters, dr.data_ptr(), 0, None),
+                                                 damping, tol, max_iters, dr.data_ptr(), _weights_flag(graph, w),
+                                                 None),
                       "olv_pagerank_jvp_f32")
             return dr
 

@@ -81,34 +81,48 @@ struct BfsWork {
   bool force_bu = false;            // OLV_FORCE_BOTTOM_UP of the current run
 };
 
-void free_bfs_work(olv_graph *g) {
-  BfsWork *w = g->bfs;
-  if (!w) return;
+static void release_bfs_work(BfsWork *w) {
   void *ps[] = {w->visited, w->snap, w->bms, w->queue, w->hubs, w->ctl, w->bar, w->stat};
   for (void *p : ps)
     if (p) (void)hipFree(p);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
   delete w;
+}
+
+void free_bfs_work(olv_graph *g) {
+  if (!g->bfs) return;
+  release_bfs_work(g->bfs);
   g->bfs = nullptr;
 }
 
+static int alloc_bfs_work(BfsWork *w, const olv_graph *g) {
+  w->n = g->n;
+  w->nw = (g->n + 31) / 32;
+  w->nwp = (w->nw + 63) / 64 * 64;
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 32));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->snap), (size_t)w->nwp * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 2));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->queue), (size_t)g->n * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->hubs), hub_capacity(g->m) * sizeof(uint2)));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(BfsCtl)));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bar), sizeof(GridBar)));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->stat), 2 * sizeof(unsigned long long)));
+  OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(BfsCtl), hipHostMallocDefault));
+  return OLV_OK;
+}
+
+// The graph's work buffers, allocated on first use; published to the graph
+// only once every allocation has succeeded.
 static int get_work(const olv_graph *cg, BfsWork **out) {
   olv_graph *g = const_cast<olv_graph *>(cg);
   if (!g->bfs) {
     BfsWork *w = new BfsWork();
+    const int rc = alloc_bfs_work(w, g);
+    if (rc != OLV_OK) {
+      release_bfs_work(w);
+      return rc;
+    }
     g->bfs = w;
-    w->n = g->n;
-    w->nw = (g->n + 31) / 32;
-    w->nwp = (w->nw + 63) / 64 * 64;
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->visited), (size_t)w->nwp * 32));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->snap), (size_t)w->nwp * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bms), (size_t)w->nwp * 4 * 2));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->queue), (size_t)g->n * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->hubs), hub_capacity(g->m) * sizeof(uint2)));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(BfsCtl)));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->bar), sizeof(GridBar)));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->stat), 2 * sizeof(unsigned long long)));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(BfsCtl), hipHostMallocDefault));
   }
   *out = g->bfs;
   return OLV_OK;

@@ -148,8 +148,11 @@ __device__ __forceinline__ uint32_t xcc_id() { return __builtin_amdgcn_s_getreg(
 // The state must be zeroed before launch.  Spins are bounded: on a timeout
 // the error flag is raised and every caller returns false, so every wave
 // still exits.
+// Arrival counters only ever grow (barrier k completes at the k-th multiple
+// of the arrivals it expects), so no reset store can race with an early
+// arrival at the next barrier.
 struct GridBar {
-  uint32_t count, gen, err, nxcc;
+  uint32_t count, gen, err, icount;
   uint32_t local[16 * 16];  // per-XCD arrivals, 64 bytes apart
   uint32_t xcnt[16 * 16];   // workgroups per XCD
 };
@@ -179,12 +182,12 @@ __device__ __forceinline__ bool gbar_init(GridBar *b, GridBarLocal &l, uint32_t 
     uint32_t ok = 1;
     const uint32_t x = xcc_id();
     __hip_atomic_fetch_add(&b->xcnt[x * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint32_t a = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (a == nblocks - 1) {
-      __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
+    // release: this workgroup's xcnt arrival is ordered before its count
+    // arrival; the last arriver's acquire sees every xcnt before bumping gen
+    const uint32_t a = __hip_atomic_fetch_add(&b->icount, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (a == nblocks - 1) __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     ok = gbar_wait(b, 0u, spin_limit) ? 1u : 0u;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     uint32_t nx = 0;
     for (uint32_t i = 0; i < 16; ++i)
       nx += __hip_atomic_load(&b->xcnt[i * 16], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 1u : 0u;
@@ -208,15 +211,13 @@ __device__ __forceinline__ bool grid_barrier(GridBar *b, GridBarLocal &l, uint32
   __syncthreads();
   if (threadIdx.x == 0) {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    // barrier l.gen (1, 2, ...): its arrivals are the l.gen-th block of
+    // xsize arrivals on this XCD and of nxcc arrivals on count
     const uint32_t p = __hip_atomic_fetch_add(&b->local[l.xcc * 16], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (p == l.xsize - 1) {
-      __hip_atomic_store(&b->local[l.xcc * 16], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (p == l.gen * l.xsize - 1) {
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
       const uint32_t a = __hip_atomic_fetch_add(&b->count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      if (a == l.nxcc - 1) {
-        __hip_atomic_store(&b->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      if (a == l.gen * l.nxcc - 1) __hip_atomic_fetch_add(&b->gen, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     s_ok = gbar_wait(b, l.gen, spin_limit) ? 1u : 0u;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");

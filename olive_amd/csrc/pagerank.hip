@@ -6,11 +6,12 @@
 // (PageRank.cu:63-75) and a host-side vertexReduce (full D2H copy,
 // oliver.h:176-183) per iteration.  Here one launch per iteration computes
 // every vertex's in-sum with a fixed summation order, applies the update,
-// writes the next contribution vector and per-workgroup residual partials;
-// a one-workgroup launch totals the residual and raises a device flag that
-// turns the remaining pre-enqueued iterations into no-ops.  Index and
-// weight arrays are read through ld_stream (plain loads: non-temporal loads
-// measured slower on MI355X).
+// writes the next contribution vector and adds its workgroup's residual to
+// one of 32 counter slots; the next iteration totals those slots at its start
+// and, once the residual is below epsilon, raises a device flag that turns
+// the remaining pre-enqueued iterations into no-ops.  Index and weight
+// arrays are read with plain loads (non-temporal loads measured slower on
+// MI355X).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -64,7 +65,13 @@ struct PrWork {
   uint32_t n = 0, m = 0;
   PrRelabel rl;
   float *buf[4] = {nullptr, nullptr, nullptr, nullptr};  // n floats each
-  float *w_perm = nullptr;                                // m floats
+  // Edge weights in a solver's slot order (w[eid[k]] at slot k): [0] for the
+  // pulls over the in-CSR, [1] for the adjoint over the out-CSR; each with
+  // the weights and slot map it was made from, reused by calls that pass
+  // OLV_W_UNCHANGED with the same weights.
+  float *wslot[2] = {nullptr, nullptr};  // m floats each
+  const float *wslot_src[2] = {nullptr, nullptr};
+  const uint32_t *wslot_eid[2] = {nullptr, nullptr};
   IterCtl *ctl = nullptr;
   IterCtl *h_ctl = nullptr;
   double *part = nullptr;  // per-workgroup partial sums (kPartials)
@@ -77,7 +84,8 @@ void free_pr_work(olv_graph *g) {
   if (!w) return;
   for (float *p : w->buf)
     if (p) (void)hipFree(p);
-  if (w->w_perm) (void)hipFree(w->w_perm);
+  for (float *p : w->wslot)
+    if (p) (void)hipFree(p);
   if (w->ctl) (void)hipFree(w->ctl);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
   if (w->part) (void)hipFree(w->part);
@@ -86,20 +94,29 @@ void free_pr_work(olv_graph *g) {
   g->pr = nullptr;
 }
 
-static int get_pr_work(olv_graph *g, bool need_wperm, PrWork **out) {
+static int alloc_pr_work(PrWork *w, const olv_graph *g) {
+  w->n = g->n;
+  w->m = g->m;
+  for (auto &b : w->buf) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&b), (size_t)g->n * 4));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(IterCtl)));
+  OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(IterCtl), hipHostMallocDefault));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->part), kPartials * sizeof(double)));
+  return OLV_OK;
+}
+
+// The graph's PageRank buffers, allocated on first use and published to the
+// graph only once every allocation has succeeded.
+static int get_pr_work(olv_graph *g, PrWork **out) {
   if (!g->pr) {
     PrWork *w = new PrWork();
     g->pr = w;
-    w->n = g->n;
-    w->m = g->m;
-    for (auto &b : w->buf) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&b), (size_t)g->n * 4));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->ctl), sizeof(IterCtl)));
-    OLV_HIP(hipHostMalloc(reinterpret_cast<void **>(&w->h_ctl), sizeof(IterCtl), hipHostMallocDefault));
-    OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->part), kPartials * sizeof(double)));
+    const int rc = alloc_pr_work(w, g);
+    if (rc != OLV_OK) {
+      free_pr_work(g);
+      return rc;
+    }
   }
-  PrWork *w = g->pr;
-  if (need_wperm && !w->w_perm) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&w->w_perm), std::max<size_t>(g->m, 1) * 4));
-  *out = w;
+  *out = g->pr;
   return OLV_OK;
 }
 
@@ -310,6 +327,28 @@ __global__ void permute_kernel(const float *__restrict__ w, const uint32_t *__re
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < m; k += gridDim.x * blockDim.x) out[k] = w[eid[k]];
 }
 
+// The weights w in slot order by eid for side 0 (in-CSR pulls) or 1 (the
+// adjoint over the out-CSR), in the side's cached buffer.  With `reuse` the
+// permutation is skipped when that buffer was last made from the same
+// weights address and slot map (the caller vouches that the weights did not
+// change since the call that made it); host-staged weights are never reused.
+static int slot_weights(const olv_graph *g, PrWork *pw, int side, const float *w, const uint32_t *eid, bool reuse,
+                        bool staged, const float **out) {
+  if (!pw->wslot[side]) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&pw->wslot[side]), std::max<size_t>(g->m, 1) * 4));
+  if (!(reuse && !staged && pw->wslot_src[side] == w && pw->wslot_eid[side] == eid)) {
+    if (g->m)
+      hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), w, eid, g->m, pw->wslot[side]);
+    OLV_HIP(hipGetLastError());
+    pw->wslot_src[side] = staged ? nullptr : w;
+    pw->wslot_eid[side] = eid;
+    // a call that did not vouch for these weights invalidates the other
+    // side's copy of them too
+    if (!reuse && pw->wslot_src[side ^ 1] == w) pw->wslot_src[side ^ 1] = nullptr;
+  }
+  *out = pw->wslot[side];
+  return OLV_OK;
+}
+
 // y[i] = a * x[idx[i]]
 __global__ void gather_scale_kernel(const float *__restrict__ x, const uint32_t *__restrict__ idx, float a, uint32_t n,
                                     float *__restrict__ y) {
@@ -417,11 +456,11 @@ struct Stage {
   }
 };
 
-// Fixed point r = d * P^T r + (1-d) t over the in-slot weights w->w_perm,
+// Fixed point r = d * P^T r + (1-d) t over the in-slot weights ws,
 // iterated from r0 = t until the L1 change drops below tol; the result is
 // left in w->buf[res->iters & 1].  t must not be w->buf[0] or w->buf[1].
 static int weighted_solve(const olv_graph *g, const SegDev &d, const uint32_t *rp, const uint32_t *ci, PrWork *w,
-                          const float *t, float damping, float tol, uint32_t max_iters, Timer &timer,
+                          const float *ws, const float *t, float damping, float tol, uint32_t max_iters, Timer &timer,
                           IterResult *res) {
   OLV_HIP(hipMemcpyAsync(w->buf[0], t, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
@@ -431,7 +470,7 @@ static int weighted_solve(const olv_graph *g, const SegDev &d, const uint32_t *r
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
-      WprOp op{ci, w->w_perm, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
+      WprOp op{ci, ws, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
       OLV_TRY(enqueue_iteration(d, rp, op, w, it));
     }
     timer.stop();
@@ -487,7 +526,7 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
   if (!relabel) OLV_TRY(ensure_plan(g, true));
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, false, &w));
+  OLV_TRY(get_pr_work(g, &w));
   if (relabel) OLV_TRY(ensure_side(g, w, true, false));
   const SegDev dr = make_seg_dev(relabel ? w->rl.in.plan : g->plan_in);
   const uint32_t *rp = relabel ? w->rl.in.rp : g->in_row_ptr;
@@ -541,7 +580,7 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   if (!cg || !wts || !tele || !ranks) return fail(OLV_EINVAL, "olv_pagerank_weighted_f32: NULL argument");
   olv_graph *g = const_cast<olv_graph *>(cg);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, true, &w));
+  OLV_TRY(get_pr_work(g, &w));
   PullView pv;
   OLV_TRY(pull_view(g, w, flags, &pv));
   const bool host = (flags & OLV_PTR_HOST) != 0;
@@ -562,9 +601,10 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
     gather_f32(dt, g->rl_order, g->n, tb.as<float>());
     tt = tb.as<float>();
   }
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, pv.eid, g->m, w->w_perm);
+  const float *ws = nullptr;
+  OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
-  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, tt, damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, tt, damping, tol, max_iters, t, &res));
   const double ms = t.ms();
   const uint32_t iters = res.iters;
   if (pv.relabel) gather_f32(w->buf[iters & 1], g->rl_pi, g->n, r);
@@ -594,7 +634,7 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   if (!(damping >= 0.0f && damping < 1.0f)) return fail(OLV_EINVAL, "olv_pagerank_jvp_f32: damping must be in [0, 1)");
   olv_graph *g = const_cast<olv_graph *>(cg);
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, true, &w));
+  OLV_TRY(get_pr_work(g, &w));
   PullView pv;
   OLV_TRY(pull_view(g, w, flags, &pv));
   const bool host = (flags & OLV_PTR_HOST) != 0;
@@ -610,11 +650,12 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(s);
   float *out = so.out(dranks, g->n, host, &s);
   OLV_TRY(s);
-  ScratchBuf rb, tb;  // r and dt in the new numbering
+  ScratchBuf rb, tb, dwb;  // r and dt in the new numbering; dw in slot order
   if (pv.relabel) {
     OLV_TRY(rb.alloc(std::max<size_t>(g->n, 1) * 4));
     OLV_TRY(tb.alloc(std::max<size_t>(g->n, 1) * 4));
   }
+  OLV_TRY(dwb.alloc(std::max<size_t>(g->m, 1) * 4));
   Timer t;
   t.start();
   if (pv.relabel) {
@@ -625,16 +666,18 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   }
   // source term s = d * dP^T r + (1-d) dt: one forward iteration with the
   // weights dw, applied to r, and the teleport dt
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), ddw, pv.eid, g->m, w->w_perm);
+  if (g->m)
+    hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), ddw, pv.eid, g->m, dwb.as<float>());
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, 0.0);
-  WprOp src_op{pv.ci, w->w_perm, dr_in, w->buf[2], ddt, damping, 1.0f - damping};
+  WprOp src_op{pv.ci, dwb.as<float>(), dr_in, w->buf[2], ddt, damping, 1.0f - damping};
   OLV_TRY(enqueue_iteration(pv.d, pv.rp, src_op, w, 0));
   // dr = d * P^T dr + s, i.e. the forward fixed point with teleport s / (1-d)
   hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), w->buf[2], 1.0f / (1.0f - damping),
                      g->n, w->buf[3]);
-  if (g->m) hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, pv.eid, g->m, w->w_perm);
+  const float *ws = nullptr;
+  OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
-  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, w->buf[3], damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, w->buf[3], damping, tol, max_iters, t, &res));
   const double ms = t.ms();
   if (pv.relabel) gather_f32(w->buf[res.iters & 1], g->rl_pi, g->n, out);
   else OLV_HIP(hipMemcpyAsync(out, w->buf[res.iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
@@ -663,7 +706,7 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   olv_graph *g = const_cast<olv_graph *>(cg);
   const bool relabel = (flags & OLV_GRAPH_ORDER) == 0;
   PrWork *w = nullptr;
-  OLV_TRY(get_pr_work(g, relabel, &w));
+  OLV_TRY(get_pr_work(g, &w));
   // the out-CSR the adjoint runs over: the renumbered copy or the graph's own
   if (relabel) OLV_TRY(ensure_side(g, w, false, true));
   else OLV_TRY(ensure_plan(g, false));
@@ -694,12 +737,9 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   t.start();
   const float *wv = dw, *rv = dr, *gv = dg;
   if (relabel) {
-    if (g->m)
-      hipLaunchKernelGGL(permute_kernel, dim3(ew_grid(g->m)), dim3(256), 0, stream(), dw, w->rl.out.eid, g->m,
-                         w->w_perm);
+    OLV_TRY(slot_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wv));
     gather_f32(dr, g->rl_order, g->n, rb.as<float>());
     gather_f32(dg, g->rl_order, g->n, gb.as<float>());
-    wv = w->w_perm;
     rv = rb.as<float>();
     gv = gb.as<float>();
   }

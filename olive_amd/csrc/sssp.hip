@@ -480,7 +480,8 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
       enq += chunk;
       OLV_TRY(sssp_read(w));
       if (w.h_ctl->done) break;
-      if (enq > (uint64_t)g->n + 2) return fail(OLV_EINTERNAL, "olv_sssp: round loop did not terminate");
+      // the device stops by itself after p.max_rounds rounds (capped)
+      if (enq > (uint64_t)p.max_rounds + 2) return fail(OLV_EINTERNAL, "olv_sssp: round loop did not terminate");
       if (!(flags & OLV_HOST_LOOP)) chunk = std::min(chunk * 2, 256u);
     }
   }

@@ -39,12 +39,28 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
-def lib():
+def build_native(out_dir: str) -> str:
+    """Compile oracle.c with -O3 -march=native into out_dir (the CPU
+    baseline: built on the host that times it).  Returns the library path."""
+    subprocess.run(["make", "-s", "-B", "native", f"NATIVE_DIR={os.path.abspath(out_dir)}"], cwd=HERE, check=True)
+    return os.path.join(os.path.abspath(out_dir), "liboracle.so")
+
+
+def use_library(path: str) -> None:
+    """Bind the oracle functions to the build at `path` (e.g. the native
+    build from build_native) instead of the default liboracle.so."""
+    global _lib
+    _lib = None
+    lib(path)
+
+
+def lib(path: str | None = None):
     global _lib
     if _lib is None:
-        if not os.path.exists(LIB_PATH):
+        path = path or LIB_PATH
+        if path == LIB_PATH and not os.path.exists(LIB_PATH):
             build()
-        L = ctypes.CDLL(LIB_PATH)
+        L = ctypes.CDLL(path)
         u32p = ctypes.POINTER(ctypes.c_uint32)
         i32p = ctypes.POINTER(ctypes.c_int32)
         f64p = ctypes.POINTER(ctypes.c_double)
@@ -62,6 +78,8 @@ def lib():
         L.orc_edge_list_load.argtypes = [ctypes.c_char_p, ctypes.c_size_t, ctypes.c_uint32, ctypes.c_uint32,
                                          u32p, u32p, i32p, ctypes.POINTER(ctypes.c_int)]
         L.orc_print_values.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int]
+        L.orc_bfs_verify.argtypes = [ctypes.c_uint32, u32p, u32p, ctypes.c_uint32, u32p, i32p, ctypes.c_int]
+        L.orc_bfs_verify.restype = ctypes.c_int64
         for f in ("orc_bfs", "orc_sssp_frontier", "orc_dijkstra", "orc_pagerank",
                   "orc_pagerank_weighted", "orc_pagerank_adjoint", "orc_edge_list_header",
                   "orc_edge_list_load", "orc_print_values"):
@@ -92,6 +110,23 @@ def bfs(row_ptr, col, src):
     if rc < 0:
         raise ValueError(f"orc_bfs failed ({rc})")
     return out[:n]
+
+
+def bfs_verify(row_ptr, col, srcs, levels, threads=None):
+    """Index of the first row of `levels` (rows x n, row r from srcs[r])
+    that is not the BFS level array, or -1 when all are (orc_bfs_verify:
+    level 0 only at the source, no edge skips a level, every reached vertex
+    has a parent one level up)."""
+    rp, c = _csr(row_ptr, col)
+    n = rp.size - 1
+    s = np.ascontiguousarray(srcs, dtype=np.uint32)
+    lv = np.ascontiguousarray(levels, dtype=np.int32).reshape(s.size, n)
+    th = threads or min(16, len(os.sched_getaffinity(0)))
+    rc = lib().orc_bfs_verify(n, _p(rp, ctypes.c_uint32), _p(c, ctypes.c_uint32), s.size,
+                              _p(s, ctypes.c_uint32), _p(lv, ctypes.c_int32), th)
+    if rc < -1:
+        raise MemoryError("orc_bfs_verify: allocation failed")
+    return int(rc)
 
 
 def sssp(row_ptr, col, w, src):

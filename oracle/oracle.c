@@ -13,6 +13,7 @@
  * header comment of each function and exercised by tests/test_oracle.py.
  */
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdint.h>
 #include <stdlib.h>
@@ -423,4 +424,95 @@ int orc_print_values(const char *path, const void *values, uint64_t count, int i
     else fprintf(f, "%d\n", ((const int32_t *)values)[i]);
   }
   return fclose(f) == 0 ? 0 : -1;
+}
+
+/*
+ * Checks rows of BFS levels against the two properties that fix them: the
+ * levels testBFS.cu:56-79 computes are the unique array with
+ *   (a) lvl[src] = 0, every other finite level >= 1, no negative level;
+ *   (b) every edge (u, v) with lvl[u] finite has lvl[v] <= lvl[u] + 1;
+ *   (c) every reached v != src has an in-edge (u, v) with lvl[u] = lvl[v] - 1.
+ * (c) gives a path from src of length lvl[v], so lvl[v] >= dist(v); (b)
+ * gives lvl[v] <= dist(v) along a shortest path and makes every vertex with
+ * a reached in-neighbour reached; so lvl = dist, INF exactly where no path
+ * exists.  Row r of `levels` (n entries) belongs to srcs[r].  Rows are split
+ * over `threads` POSIX threads.  Returns -1 when every row passes, otherwise
+ * the lowest failing row index; -2 on allocation failure.
+ * Pinned by tests/test_oracle.py: rows from orc_bfs pass, and planted
+ * mistakes (a level one too high, one too low, a reached vertex set to INF,
+ * a second level-0 vertex) each fail.
+ */
+typedef struct {
+  uint32_t n;
+  const uint32_t *row_ptr, *col, *srcs;
+  const int32_t *levels;
+  uint32_t r0, r1;
+  int64_t bad;  /* lowest failing row of this slice, or -1 */
+  int err;
+} orc_verify_job;
+
+static int orc_verify_row(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, uint32_t src,
+                          const int32_t *lv, uint8_t *has_parent) {
+  if (src >= n || lv[src] != 0) return 0;
+  memset(has_parent, 0, n);
+  for (uint32_t u = 0; u < n; u++) {
+    const int32_t l = lv[u];
+    if (l == ORC_INF) continue;
+    if (l < 0 || (l == 0 && u != src)) return 0;
+    for (uint32_t e = row_ptr[u]; e < row_ptr[u + 1]; e++) {
+      const uint32_t v = col[e];
+      const int32_t lvv = lv[v];
+      if (lvv == ORC_INF || (int64_t)lvv > (int64_t)l + 1) return 0;
+      if (lvv == l + 1) has_parent[v] = 1;
+    }
+  }
+  for (uint32_t v = 0; v < n; v++)
+    if (v != src && lv[v] != ORC_INF && !has_parent[v]) return 0;
+  return 1;
+}
+
+static void *orc_verify_worker(void *arg) {
+  orc_verify_job *j = (orc_verify_job *)arg;
+  uint8_t *hp = (uint8_t *)malloc(j->n ? j->n : 1);
+  j->bad = -1;
+  j->err = hp == NULL;
+  if (!hp) return NULL;
+  for (uint32_t r = j->r0; r < j->r1; r++)
+    if (!orc_verify_row(j->n, j->row_ptr, j->col, j->srcs[r], j->levels + (size_t)r * j->n, hp)) {
+      j->bad = r;
+      break;
+    }
+  free(hp);
+  return NULL;
+}
+
+int64_t orc_bfs_verify(uint32_t n, const uint32_t *row_ptr, const uint32_t *col, uint32_t rows,
+                       const uint32_t *srcs, const int32_t *levels, int threads) {
+  if (threads < 1) threads = 1;
+  if ((uint32_t)threads > rows) threads = rows ? (int)rows : 1;
+  orc_verify_job *jobs = (orc_verify_job *)calloc((size_t)threads, sizeof(orc_verify_job));
+  pthread_t *tid = (pthread_t *)calloc((size_t)threads, sizeof(pthread_t));
+  if (!jobs || !tid) {
+    free(jobs);
+    free(tid);
+    return -2;
+  }
+  for (int t = 0; t < threads; t++) {
+    jobs[t] = (orc_verify_job){n, row_ptr, col, srcs, levels, (uint32_t)((uint64_t)rows * t / threads),
+                               (uint32_t)((uint64_t)rows * (t + 1) / threads), -1, 0};
+    if (pthread_create(&tid[t], NULL, orc_verify_worker, &jobs[t]) != 0) {
+      orc_verify_worker(&jobs[t]);
+      tid[t] = 0;
+    }
+  }
+  int64_t bad = -1;
+  int err = 0;
+  for (int t = 0; t < threads; t++) {
+    if (tid[t]) pthread_join(tid[t], NULL);
+    err |= jobs[t].err;
+    if (jobs[t].bad >= 0 && (bad < 0 || jobs[t].bad < bad)) bad = jobs[t].bad;
+  }
+  free(jobs);
+  free(tid);
+  return err ? -2 : bad;
 }

@@ -277,3 +277,40 @@ def test_jvp_against_central_differences(olv):
     rel = np.linalg.norm(dr.numpy().astype(np.float64) - fd) / np.linalg.norm(fd)
     say(f"  jvp vs fd rel {rel:.2e}")
     assert rel <= 1e-4
+
+
+def test_weighted_slot_weights_reuse(olv):
+    """OLV_W_UNCHANGED reuses the weights already in slot order: results are
+    bit-identical to a fresh permutation; weights rewritten in place and
+    passed without the flag are re-permuted, and the other solver's copy of
+    them is dropped, so a later flagged call does not see the old weights."""
+    from olive_amd import _lib
+    U = _lib.OLV_W_UNCHANGED
+    csr, t = weighted_case(olv, 12, 24)
+    g = olv.Graph(csr)
+    W = olv.DeviceBuffer.from_numpy(csr.weights)
+    T = olv.DeviceBuffer.from_numpy(t)
+    G = olv.DeviceBuffer.from_numpy(np.random.default_rng(3).standard_normal(csr.n).astype(np.float32))
+    D = olv.DeviceBuffer.from_numpy(np.random.default_rng(4).standard_normal(csr.m).astype(np.float32))
+    r0, _ = g.pagerank_weighted(W, T, tol=1e-8)
+    r1, _ = g.pagerank_weighted(W, T, tol=1e-8, flags=U)
+    assert np.array_equal(bits(r0), bits(r1))
+    a = g.pagerank_vjp(W, r0, G, tol=1e-7)
+    b = g.pagerank_vjp(W, r0, G, tol=1e-7, flags=U)
+    for x, y in zip(a[:3], b[:3]):
+        assert np.array_equal(bits(x), bits(y))
+    j0, _ = g.pagerank_jvp(W, r0, D, T, tol=1e-7)
+    j1, _ = g.pagerank_jvp(W, r0, D, T, tol=1e-7, flags=U)
+    assert np.array_equal(bits(j0), bits(j1))
+    # new weights in the same buffer
+    w2 = olv.with_weights_f32_normalized(olv.gen_rmat(12, 16, seed=24), seed=99).weights
+    _lib.check(_lib.lib().olv_memcpy_h2d(W.ptr, w2.ctypes.data, W.nbytes))
+    W2 = olv.DeviceBuffer.from_numpy(w2)
+    r2, _ = g.pagerank_weighted(W, T, tol=1e-8)
+    r2ref, _ = g.pagerank_weighted(W2, T, tol=1e-8)
+    assert np.array_equal(bits(r2), bits(r2ref))
+    assert not np.array_equal(bits(r2), bits(r0))
+    c = g.pagerank_vjp(W, r2, G, tol=1e-7, flags=U)
+    cref = g.pagerank_vjp(W2, r2, G, tol=1e-7)
+    for x, y in zip(c[:3], cref[:3]):
+        assert np.array_equal(bits(x), bits(y))

@@ -209,6 +209,26 @@ def test_sssp_zero_and_large_weights(olv):
         assert np.array_equal(g.sssp(0)[0].numpy(), ref), (lo, hi)
 
 
+@pytest.mark.parametrize("flags", [0, 0x8])
+def test_sssp_chain_with_one_heavy_edge(olv, flags):
+    """A 1000-vertex chain with unit weights and one weight-1e6 edge: the
+    near-far step (~32k) needs ~31 promotions with nothing to relax, so the
+    run takes more rounds than the chain has vertices."""
+    n = 1000
+    rp = np.arange(n + 1, dtype=np.uint32)
+    rp[n] = n - 1
+    col = np.arange(1, n, dtype=np.uint32)
+    w = np.ones(n - 1, np.int32)
+    w[500] = 1_000_000
+    csr = olv.CsrGraph(rp, col, w)
+    g = olv.Graph(csr, transpose=False)
+    out, st = g.sssp(0, flags=flags)
+    ref = oracle.dijkstra(rp, col, w, 0)
+    assert ref[-1] == n - 2 + 1_000_000
+    assert np.array_equal(out.numpy(), ref)
+    say(f"  chain: {st.iterations} rounds")
+
+
 def test_sssp_negative_cycle_reports_enoconv(olv):
     csr = olv.CsrGraph(np.array([0, 1, 2], np.uint32), np.array([1, 0], np.uint32), np.array([-1, -1], np.int32))
     g = olv.Graph(csr)

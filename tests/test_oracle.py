@@ -124,6 +124,44 @@ def test_bfs_plants_mistake():
         bfs_invariants(rp, col, 0, lv)
 
 
+def test_bfs_verify_rows():
+    """The multi-row C verifier accepts the oracle's rows (and the numpy
+    invariants agree) and flags each planted mistake at its row."""
+    _, rp, col = oracle.gen_rmat(11, 16, seed=3)
+    outdeg = np.diff(rp.astype(np.int64))
+    srcs = np.flatnonzero(outdeg)[[0, 5, 50, 500, 900]].astype(np.uint32)
+    rows = np.stack([oracle.bfs(rp, col, int(s)) for s in srcs])
+    for s, r in zip(srcs, rows):
+        bfs_invariants(rp, col, int(s), r)
+    for th in (1, 3, 8):
+        assert oracle.bfs_verify(rp, col, srcs, rows, threads=th) == -1
+    n = rp.size - 1
+    # grid closed form: row + col levels pass; the verifier agrees with them
+    g_rp, g_col = oracle.gen_grid(16, 16)[1:]
+    g_lv = np.array([(i // 16) + (i % 16) for i in range(256)], np.int32)
+    assert oracle.bfs_verify(g_rp, g_col, np.array([0], np.uint32), g_lv[None]) == -1
+    reached = np.flatnonzero((rows[2] != INF) & (rows[2] > 1))
+    leaf = reached[0]
+    plants = {
+        "one too high": lambda r: r.__setitem__(leaf, r[leaf] + 1),
+        "one too low": lambda r: r.__setitem__(leaf, r[leaf] - 1),
+        "reached set to INF": lambda r: r.__setitem__(leaf, INF),
+        "second level 0": lambda r: r.__setitem__(leaf, 0),
+        "source not 0": lambda r: r.__setitem__(int(srcs[2]), 1),
+    }
+    for name, plant in plants.items():
+        bad = rows.copy()
+        plant(bad[2])
+        assert oracle.bfs_verify(rp, col, srcs, bad, threads=2) == 2, name
+    # an unreachable vertex given a finite level
+    unreached = np.flatnonzero(rows[3] == INF)
+    if unreached.size:
+        bad = rows.copy()
+        bad[3][unreached[0]] = 3
+        assert oracle.bfs_verify(rp, col, srcs, bad) == 3
+    assert n == 2048
+
+
 # -------------------------------------------------------------------- SSSP
 def test_sssp_matches_networkx_golden():
     keys = [k for k in G.files if "/sssp/" in k]
