@@ -232,6 +232,22 @@ int olv_read_binary(const char *path, void *values, size_t capacity_bytes,
 int olv_bfs(const olv_graph *g, uint32_t src, int32_t *levels, unsigned flags,
             olv_stats *st);
 
+/* olv_bfs without any allocation in the call: no reference counterpart; for
+ * callers that cannot allocate device memory (tight serving loops, code
+ * that must not call hipMalloc).  Same levels as olv_bfs, computed in the
+ * graph's own numbering (as OLV_GRAPH_ORDER).  Every work buffer lives in one
+ * static 32 MiB device array per GPU (held until the process ends), which
+ * fits graphs up to about 5.5 M vertices; larger graphs fail with
+ * OLV_ERANGE.  `levels` must be a device pointer (OLV_PTR_HOST and
+ * OLV_PERSISTENT are rejected).  Direction-optimising runs need the graph's
+ * in-edge CSR with its in-list heads (OLV_BUILD_TRANSPOSE at creation, or one
+ * olv_bfs call); otherwise pass OLV_NO_DIRECTION_OPT.  st->elapsed_ms is
+ * host wall time.  All calls on one GPU share the array: calls that overlap
+ * on one GPU (two threads, two streams) race; each call initialises what it
+ * reads, so sequential calls need no clearing in between. */
+int olv_bfs_noalloc(const olv_graph *g, uint32_t src, int32_t *levels,
+                    unsigned flags, olv_stats *st);
+
 /* SSSP distances over int32 weights (graph weights, or `weights` if not
  * NULL), INF = 0x7fffffff.  Frontier relaxation as SSSP.cu:146-163 with
  * cond(v, acc[v]) (SSSP.cu:72-75). */

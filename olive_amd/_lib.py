@@ -119,6 +119,7 @@ SIGNATURES = {
     "olv_write_binary": (_int, [ctypes.c_char_p, _vp, _sz, _sz, _int, _uint]),
     "olv_read_binary": (_int, [ctypes.c_char_p, _vp, _sz, ctypes.POINTER(_u64), _u32p, ctypes.POINTER(_int), _uint]),
     "olv_bfs": (_int, [_vp, _u32, _vp, _uint, _statp]),
+    "olv_bfs_noalloc": (_int, [_vp, _u32, _vp, _uint, _statp]),
     "olv_bfs_multisource": (_int, [_vp, _vp, _u32, _vp, _uint, _statp]),
     "olv_sssp": (_int, [_vp, _u32, _vp, _vp, _uint, _statp]),
     "olv_pagerank_f32": (_int, [_vp, _f32, _u32, _f64, _vp, _uint, _statp]),

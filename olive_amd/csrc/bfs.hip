@@ -20,7 +20,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "bfs_core.hpp"
@@ -468,12 +470,12 @@ GridBar *bfs_bar(BfsWork *w) { return w->bar; }
 // stop event of `t` follows the last enqueued level, ahead of the control
 // block read-back, so host latency after the last kernel is not timed (the
 // gaps between chunks are).
-static int run_pipeline(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t chunk0, Timer &t) {
+static int run_pipeline(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t chunk0, Timer *t) {
   uint32_t chunk = std::max(1u, chunk0);
   uint64_t enq = 0;
   for (;;) {
     OLV_TRY(bfs_enqueue_levels(g, w, levels, can_bu, chunk));
-    t.stop();
+    if (t) t->stop();
     enq += chunk;
     OLV_TRY(bfs_result(w, nullptr, nullptr));
     if (w->h_ctl->done) break;
@@ -549,7 +551,7 @@ int bfs_multisource(const olv_graph *g0, const uint32_t *seeds, uint32_t k, bool
     // doubling chunks after one host check
     const uint32_t recent_max = std::max(std::max(w->recent[0], w->recent[1]), std::max(w->recent[2], w->recent[3]));
     const uint32_t chunk0 = (flags & OLV_HOST_LOOP) ? 1u : std::max(4u, recent_max);
-    OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0, t));
+    OLV_TRY(run_pipeline(g, w, out, can_bu, chunk0, &t));
   }
   const double ms = t.ms();
   uint32_t levels_run = 0, bu_levels = 0;
@@ -585,4 +587,133 @@ extern "C" int olv_bfs(const olv_graph *g, uint32_t src, int32_t *levels, unsign
   if (!g) return fail(OLV_EINVAL, "olv_bfs: NULL graph");
   if (src >= g->n) return fail(OLV_EINVAL, "olv_bfs: source %u out of range (n=%u)", src, g->n);
   return bfs_multisource(g, &src, 1, true, levels, flags, st);
+}
+
+// ------------------------------------------------ allocation-free entry point
+// olv_bfs_noalloc keeps every work buffer of a traversal (visited bytes,
+// snapshot and frontier bitmaps, queue, hub items, control block, barrier,
+// statistics, the seed) in one static device array per GPU instead of
+// buffers allocated with the graph.  The host code only ever uses the array's
+// device address, obtained once per device with hipGetSymbolAddress.
+namespace olv {
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+}  // namespace olv
+
+static __device__ __align__(256) unsigned char scratch[olv::kNoAllocScratchBytes];
+
+namespace olv {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_ptr[kMaxDevices] = {};
+
+struct NoAllocLayout {
+  size_t visited, snap, bms, queue, hubs, ctl, bar, stat, seed, total;
+  uint32_t nw, nwp;
+};
+
+NoAllocLayout noalloc_layout(uint32_t n, uint64_t m) {
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  NoAllocLayout L;
+  L.nw = (n + 31) / 32;
+  L.nwp = (L.nw + 63) / 64 * 64;
+  size_t o = 0;
+  L.visited = o;
+  o += al((size_t)L.nwp * 32);
+  L.snap = o;
+  o += al((size_t)L.nwp * 4);
+  L.bms = o;
+  o += al((size_t)L.nwp * 8);
+  L.queue = o;
+  o += al(std::max<size_t>(n, 1) * 4);
+  L.hubs = o;
+  o += al(hub_capacity(m) * sizeof(uint2));
+  L.ctl = o;
+  o += al(sizeof(BfsCtl));
+  L.bar = o;
+  o += al(sizeof(GridBar));
+  L.stat = o;
+  o += al(2 * sizeof(unsigned long long));
+  L.seed = o;
+  o += al(sizeof(uint32_t));
+  L.total = o;
+  return L;
+}
+
+__global__ void store_word_kernel(uint32_t *__restrict__ p, uint32_t v) { *p = v; }
+}  // namespace
+
+// The current device's copy of `scratch` (its address looked up once per
+// device, with that device current).
+int noalloc_scratch(unsigned char **out) {
+  const int dev = current_device();
+  if (dev < 0 || dev >= kMaxDevices) return fail(OLV_EUNSUPPORTED, "device %d beyond the %d supported", dev, kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return fail(OLV_EHIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return OLV_OK;
+}
+
+}  // namespace olv
+
+extern "C" int olv_bfs_noalloc(const olv_graph *g, uint32_t src, int32_t *levels, unsigned flags, olv_stats *st) {
+  if (!g || !levels) return fail(OLV_EINVAL, "olv_bfs_noalloc: NULL argument");
+  if (src >= g->n) return fail(OLV_EINVAL, "olv_bfs_noalloc: source %u out of range (n=%u)", src, g->n);
+  if (flags & OLV_PTR_HOST) return fail(OLV_EINVAL, "olv_bfs_noalloc: levels must be a device pointer");
+  if (flags & OLV_PERSISTENT) return fail(OLV_EINVAL, "olv_bfs_noalloc: OLV_PERSISTENT is not supported");
+  const bool can_bu = !(flags & OLV_NO_DIRECTION_OPT);
+  if (!can_bu && (flags & OLV_FORCE_BOTTOM_UP))
+    return fail(OLV_EINVAL, "olv_bfs_noalloc: OLV_FORCE_BOTTOM_UP and OLV_NO_DIRECTION_OPT together");
+  if (can_bu && (!g->in_row_ptr || !g->in_head))
+    return fail(OLV_EINVAL,
+                "olv_bfs_noalloc: the graph has no in-edge CSR with in-list heads (create it with "
+                "OLV_BUILD_TRANSPOSE and run olv_bfs on it once, or pass OLV_NO_DIRECTION_OPT)");
+  const NoAllocLayout L = noalloc_layout(g->n, g->m);
+  if (L.total > kNoAllocScratchBytes)
+    return fail(OLV_ERANGE, "olv_bfs_noalloc: the graph needs %zu bytes of work memory, the static scratch holds %zu "
+                            "(n=%u, m=%u)",
+                L.total, kNoAllocScratchBytes, g->n, g->m);
+  unsigned char *base = nullptr;
+  OLV_TRY(noalloc_scratch(&base));
+  BfsWork w;  // views into the scratch array; nothing here is freed
+  BfsCtl hctl;
+  w.n = g->n;
+  w.nw = L.nw;
+  w.nwp = L.nwp;
+  w.visited = base + L.visited;
+  w.snap = reinterpret_cast<uint32_t *>(base + L.snap);
+  w.bms = reinterpret_cast<uint32_t *>(base + L.bms);
+  w.queue = reinterpret_cast<uint32_t *>(base + L.queue);
+  w.hubs = reinterpret_cast<uint2 *>(base + L.hubs);
+  w.ctl = reinterpret_cast<BfsCtl *>(base + L.ctl);
+  w.bar = reinterpret_cast<GridBar *>(base + L.bar);
+  w.stat = reinterpret_cast<unsigned long long *>(base + L.stat);
+  w.h_ctl = &hctl;
+  w.force_bu = (flags & OLV_FORCE_BOTTOM_UP) != 0;
+  uint32_t *seed = reinterpret_cast<uint32_t *>(base + L.seed);
+  const auto t0 = std::chrono::steady_clock::now();
+  hipLaunchKernelGGL(store_word_kernel, dim3(1), dim3(1), 0, stream(), seed, src);
+  OLV_TRY(bfs_enqueue_init(g, &w, seed, 1, levels, can_bu));
+  OLV_TRY(run_pipeline(g, &w, levels, can_bu, (flags & OLV_HOST_LOOP) ? 1u : 8u, nullptr));
+  uint32_t levels_run = 0, bu_levels = 0;
+  OLV_TRY(bfs_result(&w, &levels_run, &bu_levels));
+  const double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (hctl.overflow) return fail(OLV_EINTERNAL, "olv_bfs_noalloc: more than n levels");
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = ms;
+    st->iterations = levels_run;
+    st->bottom_up_levels = bu_levels;
+    st->edges_inspected = hctl.inspected;
+    OLV_TRY(collect_stats(g, &w, levels, st));
+  }
+  return OLV_OK;
 }

@@ -106,6 +106,9 @@ int compute_inv_outdeg(const uint32_t *rp, uint32_t n, float *inv);  // 1/outdeg
 // Device exclusive prefix sum (ingest.hip): out[0..n), out[n] = total < 2^32.
 int exclusive_scan(const uint32_t *in, uint32_t *out, size_t n);
 void free_bfs_work(olv_graph *g);
+// This device's address of the allocation-free BFS's static work array
+// (looked up once per device).
+int noalloc_scratch(unsigned char **out);
 void free_pr_work(olv_graph *g);
 
 // RAII device buffer for temporaries inside an entry point.

@@ -532,6 +532,8 @@ int olv_init(int device) {
   g_device = device;
   g_cus = 0;
   (void)cu_count();
+  unsigned char *scratch_base = nullptr;  // set-up of the allocation-free entry point on this device
+  OLV_TRY(noalloc_scratch(&scratch_base));
   return OLV_OK;
 }
 
