@@ -27,6 +27,7 @@ OLV_PERSISTENT = 0x20
 OLV_GRAPH_ORDER = 0x40
 OLV_BUILD_TRANSPOSE = 0x100
 OLV_W_UNCHANGED = 0x200
+OLV_PR_ROW_PULL = 0x400
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
 OLV_FMT_EDGE_LIST, OLV_FMT_DIMACS = 0, 1

@@ -1,0 +1,198 @@
+// Host construction of the source-segmented pull plan (blocked.hpp).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "blocked.hpp"
+
+namespace olv {
+
+namespace {
+
+template <class T>
+int upload_vec(T **dst, const std::vector<T> &src) {
+  const size_t bytes = std::max<size_t>(src.size(), 1) * sizeof(T);
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(dst), bytes));
+  if (!src.empty()) OLV_HIP(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return OLV_OK;
+}
+
+inline uint32_t piece_bucket(uint32_t len) { return len <= 8 ? 0u : len <= 64 ? 1u : 2u; }
+
+}  // namespace
+
+int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
+                       BlockedHost &h) {
+  const uint32_t n = (uint32_t)(rp.size() - 1);
+  const uint32_t m = rp[n];
+  h.n = n;
+  h.m = m;
+  uint32_t n_live = n;
+  while (n_live > 0 && rp[n_live] == rp[n_live - 1]) --n_live;
+  h.n_live = n_live;
+  const uint32_t nseg = n ? ((n - 1) >> kSegBits) + 1 : 0;
+  h.nseg = nseg;
+
+  // 1. entries stably sorted by the segment of their source: within a
+  //    segment, rows ascending and each row's entries in slot order
+  std::vector<uint32_t> soff((size_t)nseg + 1, 0);
+  for (uint32_t k = 0; k < m; ++k) {
+    if (col[k] >= n) return fail(OLV_EINTERNAL, "blocked plan: entry %u out of range", col[k]);
+    soff[(col[k] >> kSegBits) + 1]++;
+  }
+  for (uint32_t s = 0; s < nseg; ++s) soff[s + 1] += soff[s];
+  std::vector<uint32_t> perm(m), prow(m);
+  {
+    std::vector<uint32_t> cur(soff.begin(), soff.end() - (nseg ? 1 : 0));
+    for (uint32_t v = 0; v < n; ++v)
+      for (uint32_t k = rp[v]; k < rp[v + 1]; ++k) {
+        const uint32_t pos = cur[col[k] >> kSegBits]++;
+        perm[pos] = k;
+        prow[pos] = v;
+      }
+  }
+
+  // 2. pieces: runs of one row within a segment, cut at kPieceMax entries;
+  //    slots grouped by row in creation order (segment, then chunk)
+  struct Piece {
+    uint32_t start, len, slot;
+  };
+  std::vector<uint32_t> npc(n, 0);
+  size_t npieces = 0;
+  for (uint32_t s = 0; s < nseg; ++s)
+    for (uint32_t p = soff[s]; p < soff[s + 1];) {
+      uint32_t e = p;
+      while (e < soff[s + 1] && prow[e] == prow[p]) ++e;
+      const uint32_t chunks = (e - p + kPieceMax - 1) / kPieceMax;
+      npc[prow[p]] += chunks;
+      npieces += chunks;
+      p = e;
+    }
+  if (npieces >= 0xffffffffull) return fail(OLV_ERANGE, "blocked plan: %zu pieces", npieces);
+  std::vector<uint32_t> &cmb = h.cmb_rp;
+  cmb.assign((size_t)n + 1, 0);
+  for (uint32_t v = 0; v < n; ++v) cmb[v + 1] = cmb[v] + npc[v];
+  // rows with many partials are combined by a wave each: the prefix of rows
+  // up to the last one with more than kCombineLane partials
+  h.n_heavy = 0;
+  for (uint32_t v = 0; v < n; ++v)
+    if (npc[v] > kCombineLane) h.n_heavy = v + 1;
+  std::vector<uint32_t> slot_cur(cmb.begin(), cmb.end() - 1);
+  std::vector<Piece> order;  // final piece order: segment, then bucket (longest first), then row
+  order.reserve(npieces);
+  std::vector<uint4> &units = h.units;
+  units.clear();
+  std::vector<uint32_t> unit_cost;
+  std::vector<Piece> byb[3];
+  uint32_t edge_cursor = 0;  // first entry of the next unit
+  for (uint32_t s = 0; s < nseg; ++s) {
+    for (auto &b : byb) b.clear();
+    for (uint32_t p = soff[s]; p < soff[s + 1];) {
+      uint32_t e = p;
+      while (e < soff[s + 1] && prow[e] == prow[p]) ++e;
+      const uint32_t v = prow[p];
+      for (uint32_t c = p; c < e; c += kPieceMax) {
+        const uint32_t len = std::min(kPieceMax, e - c);
+        byb[piece_bucket(len)].push_back(Piece{c, len, slot_cur[v]++});
+      }
+      p = e;
+    }
+    for (int b = 2; b >= 0; --b) {
+      // units of at most kUnitEdges entries and kUnitPieces pieces over this
+      // (segment, bucket) group
+      size_t i = 0;
+      while (i < byb[b].size()) {
+        const uint32_t first = (uint32_t)order.size();
+        uint32_t edges = 0, pieces = 0;
+        while (i < byb[b].size() &&
+               (pieces == 0 || (pieces < kUnitPieces && edges + byb[b][i].len <= kUnitEdges))) {
+          edges += byb[b][i].len;
+          ++pieces;
+          order.push_back(byb[b][i++]);
+        }
+        units.push_back(make_uint4(s, (uint32_t)b, first, edge_cursor));
+        for (uint32_t q = first; q < (uint32_t)order.size(); ++q) edge_cursor += order[q].len;
+        // cost in steps of the whole workgroup: the staging (measured at
+        // about three steps), then one step per 1024 / 128 / 16 pieces (1, 8
+        // or 64 lanes per piece)
+        const uint32_t per_step = b == 0 ? 1024u : b == 1 ? 128u : 16u;
+        unit_cost.push_back(3 + (pieces + per_step - 1) / per_step);
+      }
+    }
+  }
+
+  // 3. entry arrays in piece order
+  std::vector<uint32_t> &pbeg = h.pbeg, &pslot = h.pslot, &lslot = h.lslot;
+  std::vector<uint16_t> &lidx = h.lidx;
+  pbeg.assign(npieces + 1, 0);
+  pslot.assign(npieces, 0);
+  lidx.assign(m, 0);
+  lslot.assign(with_slots ? m : 0, 0);
+  uint32_t o = 0;
+  for (size_t q = 0; q < npieces; ++q) {
+    const Piece &pc = order[q];
+    pbeg[q] = o;
+    pslot[q] = pc.slot;
+    for (uint32_t t = 0; t < pc.len; ++t) {
+      const uint32_t k = perm[pc.start + t];
+      lidx[o + t] = (uint16_t)(col[k] & (kSegSize - 1));
+      if (with_slots) lslot[o + t] = k;
+    }
+    o += pc.len;
+  }
+  pbeg[npieces] = o;
+
+  // 4. contiguous unit ranges of about equal cost per workgroup; a sentinel
+  //    unit closes the list (its first piece and entry are the totals)
+  const uint32_t nunits = (uint32_t)units.size();
+  units.push_back(make_uint4(nseg, 0, (uint32_t)npieces, o));
+  const uint32_t ng = std::max<uint32_t>(1, std::min<uint32_t>(groups ? groups : 1, std::max<uint32_t>(nunits, 1)));
+  std::vector<uint32_t> &wg = h.wg_unit;
+  wg.assign(ng + 1, nunits);
+  wg[0] = 0;
+  {
+    double total = 0;
+    for (uint32_t c : unit_cost) total += c;
+    double acc = 0;
+    uint32_t w = 1;
+    for (uint32_t u = 0; u < nunits && w < ng; ++u) {
+      acc += unit_cost[u];
+      while (w < ng && acc >= total * w / ng) wg[w++] = u + 1;
+    }
+    for (; w < ng; ++w) wg[w] = nunits;
+  }
+
+  h.npieces = (uint32_t)npieces;
+  h.nunits = nunits;
+  h.ngroups = ng;
+  return OLV_OK;
+}
+
+int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
+                       BlockedPlan &bp) {
+  bp.release();
+  BlockedHost h;
+  OLV_TRY(build_blocked_host(rp, col, with_slots, groups, h));
+  bp.n = h.n;
+  bp.m = h.m;
+  bp.n_live = h.n_live;
+  bp.n_heavy = h.n_heavy;
+  bp.nseg = h.nseg;
+  bp.npieces = h.npieces;
+  bp.nunits = h.nunits;
+  bp.ngroups = h.ngroups;
+  OLV_TRY(upload_vec(&bp.units, h.units));
+  OLV_TRY(upload_vec(&bp.wg_unit, h.wg_unit));
+  OLV_TRY(upload_vec(&bp.pbeg, h.pbeg));
+  OLV_TRY(upload_vec(&bp.pslot, h.pslot));
+  OLV_TRY(upload_vec(&bp.lidx, h.lidx));
+  if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));
+  OLV_TRY(upload_vec(&bp.cmb_rp, h.cmb_rp));
+  OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bp.partial), std::max<size_t>(h.npieces, 1) * sizeof(float)));
+  bp.ready = true;
+  return OLV_OK;
+}
+
+}  // namespace olv

@@ -1,0 +1,307 @@
+// Source-segmented pull for the PageRank iteration (CSR segmenting: Zhang,
+// Kiriansky, Mendis, Amarasinghe, Zaharia, Matei, "Making Caches Work for
+// Graph Analytics", IEEE BigData 2017, restated for a GPU with a 160 KB LDS
+// per CU).
+//
+// The iterate's source vector is cut into segments of kSegSize consecutive
+// vertices (64 KB of fp32, one LDS slice).  The in-edges of every
+// destination are grouped by the segment of their source into "pieces"
+// (destination, segment), each piece's entries kept in the row's slot order
+// and stored as 16-bit offsets into its segment; pieces longer than
+// kPieceMax entries are cut into chunks.  One iteration is two launches:
+//
+//   segment pass  each workgroup owns a run of work units (a unit: at most
+//                 kUnitPieces pieces and kUnitEdges entries of one segment
+//                 and one length bucket).  Per unit it stages, with
+//                 coalesced loads, the unit's entries, piece offsets and
+//                 piece slots in LDS (and the segment of the source vector,
+//                 when the segment changes), then sums every piece from LDS
+//                 alone -- 1, 8 or 64 lanes per piece by length -- and
+//                 stores one partial per piece at the piece's slot;
+//   combine       per destination, its partials (slots grouped by
+//                 destination, in segment order) are added in a fixed order
+//                 -- by one lane, or by a wave for the rows with more than
+//                 kCombineLane partials -- and the vertex update runs.
+//
+// A piece's terms are summed in a fixed order and the partials in segment
+// order, so an iteration is bit-reproducible from run to run.  With the
+// vertices numbered by descending in-degree (the renumbered CSRs of the
+// PageRank solver) a skewed graph's in-edges fall into few pieces: RMAT-20
+// has 16.8 M in-edges in 2.3 M pieces, most of them in segment 0.  Per
+// iteration the pass reads 2 B per edge and 12 B per piece from HBM, and
+// every gather is an LDS read.
+#pragma once
+#include <vector>
+
+#include "device.hpp"
+#include "internal.hpp"
+#include "seg_core.hpp"
+
+namespace olv {
+
+constexpr uint32_t kSegBits = 14;
+constexpr uint32_t kSegSize = 1u << kSegBits;  // sources per LDS slice (64 KB of fp32)
+constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 145 KB of LDS)
+constexpr uint32_t kPieceMax = 512;            // longest piece (one wave step); longer ones are cut into chunks
+constexpr uint32_t kUnitEdges = 24576;         // entries per work unit, at most (48 KB staged)
+constexpr uint32_t kUnitPieces = 4096;         // pieces per work unit, at most (2 x 16 KB staged)
+constexpr uint32_t kCombineLane = 32;          // rows with more partials are combined by a wave
+
+// Device arrays of a segmented pull over one CSR.
+struct BlockedPlan {
+  uint32_t n = 0, m = 0, n_live = 0;  // vertices, entries, rows with entries (a prefix of the rows)
+  uint32_t n_heavy = 0;               // rows [0, n_heavy) hold every row with > kCombineLane partials
+  uint32_t nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
+  uint4 *units = nullptr;        // nunits + 1: (segment, bucket 0/1/2 = 1/8/64 lanes per piece, first piece,
+                                 // first entry); the last is a sentinel (-, -, npieces, m)
+  uint32_t *wg_unit = nullptr;   // ngroups + 1: unit range of each segment-pass workgroup
+  uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
+  uint32_t *pslot = nullptr;     // npieces: partial slot of each piece
+  uint16_t *lidx = nullptr;      // m: source offset within the segment, in piece order
+  uint32_t *lslot = nullptr;     // m (weighted solves only): CSR slot each entry came from
+  uint32_t *cmb_rp = nullptr;    // n + 1: partial slots of each row
+  float *partial = nullptr;      // npieces
+  bool ready = false;
+  void release() {
+    void *ps[] = {units, wg_unit, pbeg, pslot, lidx, lslot, cmb_rp, partial};
+    for (void *p : ps)
+      if (p) (void)hipFree(p);
+    units = nullptr;
+    wg_unit = pbeg = pslot = lslot = cmb_rp = nullptr;
+    lidx = nullptr;
+    partial = nullptr;
+    ready = false;
+  }
+};
+
+// Host form of the plan (the arrays before upload).
+struct BlockedHost {
+  uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
+  std::vector<uint4> units;
+  std::vector<uint32_t> wg_unit, pbeg, pslot, lslot, cmb_rp;
+  std::vector<uint16_t> lidx;
+};
+int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
+                       BlockedHost &h);
+
+// Builds the plan for a CSR whose rows are given by host offsets rp (n + 1)
+// and host entries col (m, source ids); rows must be ordered so that the rows
+// without entries come last (n_live = first such row).  with_slots keeps the
+// CSR slot of every entry (to gather per-slot data such as edge weights into
+// piece order).  groups: segment-pass workgroups to balance the units over.
+int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
+                       BlockedPlan &bp);
+
+struct BlkDev {
+  const uint4 *units;
+  const uint32_t *wg_unit;
+  const uint32_t *pbeg;
+  const uint32_t *pslot;
+  const uint16_t *lidx;
+  const uint32_t *cmb_rp;
+  float *partial;
+  uint32_t n, n_live, n_heavy;
+};
+
+inline BlkDev make_blk_dev(const BlockedPlan &p) {
+  return BlkDev{p.units, p.wg_unit, p.pbeg, p.pslot, p.lidx, p.cmb_rp, p.partial, p.n, p.n_live, p.n_heavy};
+}
+
+// Sum of sv(lidx[k]) for k = beg + j, beg + j + G, ... < end (lidx and the
+// slice in LDS) with U reads in flight per lane, U interleaved accumulators
+// added pairwise.
+template <int G, int U, class SV>
+__device__ __forceinline__ float piece_sum(const SV &sv, const uint16_t *lidx, uint32_t beg, uint32_t end,
+                                           uint32_t j) {
+  float acc[U];
+#pragma unroll
+  for (int i = 0; i < U; ++i) acc[i] = 0.f;
+  for (uint32_t k = beg + j; k < end; k += (uint32_t)U * G) {
+    uint32_t ix[U];
+#pragma unroll
+    for (int i = 0; i < U; ++i) ix[i] = k + (uint32_t)i * G < end ? (uint32_t)lidx[k + (uint32_t)i * G] : 0u;
+#pragma unroll
+    for (int i = 0; i < U; ++i)
+      if (k + (uint32_t)i * G < end) acc[i] += sv(k + (uint32_t)i * G, ix[i]);
+  }
+#pragma unroll
+  for (int w = U / 2; w > 0; w >>= 1)
+#pragma unroll
+    for (int i = 0; i < w; ++i) acc[i] += acc[i + w];
+  return acc[0];
+}
+
+template <int G>
+__device__ __forceinline__ float blk_group_sum(float v) {
+#pragma unroll
+  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// Segment pass of iteration k.  With a control block (ctl != nullptr) it
+// first applies the stop rule of the iteration loop: returns at once when
+// done; stops (sets done) when iteration k - 1 left an L1 residual below
+// eps; workgroup 0 clears the residual slots of iteration k + 1.
+// SV: void bind(const float *slice, uint32_t entry0) and
+// float operator()(uint32_t k, uint32_t i) const -- the summand of the
+// unit's entry k (entry0 + k in the plan) whose source is slice[i].
+template <class SV>
+__global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV sv, const float *__restrict__ src,
+                                                                   IterCtl *ctl, uint32_t k) {
+  __shared__ float slice[kSegSize];
+  __shared__ uint16_t s_lidx[kUnitEdges];
+  __shared__ uint32_t s_pbeg[kUnitPieces + 1];
+  __shared__ uint32_t s_pslot[kUnitPieces];
+  if (ctl) {
+    if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+    if (k > 0) {
+      const double res = err_total(ctl, (k - 1) % 3);
+      if (res < ctl->eps) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+          ctl->iters = k;
+          ctl->residual = res;
+          __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        return;
+      }
+    }
+    if (blockIdx.x == 0) {
+      double *z = &ctl->errs[(k + 1) % 3][0].v;
+      for (uint32_t i = threadIdx.x; i < kErrSlots * (sizeof(ErrSlot) / sizeof(double)); i += kBlkThreads) z[i] = 0.0;
+    }
+  }
+  const uint32_t tid = threadIdx.x;
+  constexpr uint32_t kLi = kUnitEdges / kBlkThreads, kPi = kUnitPieces / kBlkThreads;
+  // staging data of the next unit, loaded into registers while the current
+  // unit is summed (every load of a thread issued before its LDS stores)
+  uint4 nu = make_uint4(0, 0, 0, 0);
+  uint32_t n_np = 0, n_ne = 0;
+  uint16_t li[kLi];
+  uint32_t pb[kPi + 1], ps[kPi];
+  auto prefetch = [&](uint32_t u) {
+    nu = d.units[u];
+    const uint4 nx = d.units[u + 1];
+    n_np = nx.z - nu.z;
+    n_ne = nx.w - nu.w;
+#pragma unroll
+    for (uint32_t j = 0; j < kLi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      li[j] = i < n_ne ? d.lidx[nu.w + i] : (uint16_t)0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j <= kPi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      pb[j] = i <= n_np ? d.pbeg[nu.z + i] : 0u;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      ps[j] = i < n_np ? d.pslot[nu.z + i] : 0u;
+    }
+  };
+  uint32_t cur = 0xffffffffu;
+  const uint32_t u0 = d.wg_unit[blockIdx.x], u1 = d.wg_unit[blockIdx.x + 1];
+  if (u0 < u1) prefetch(u0);
+  for (uint32_t u = u0; u < u1; ++u) {
+    const uint4 un = nu;
+    const uint32_t np = n_np, ne = n_ne, e0 = un.w;
+    __syncthreads();  // the previous unit is done with the staged arrays
+    if (un.x != cur) {
+      const uint32_t base = un.x << kSegBits;
+      const uint32_t cnt = min(kSegSize, d.n - base);
+      constexpr uint32_t kSl = kSegSize / (4 * kBlkThreads);
+      float4 v[kSl];
+#pragma unroll
+      for (uint32_t j = 0; j < kSl; ++j) {
+        const uint32_t i = (tid + j * kBlkThreads) * 4;
+        if (i + 3 < cnt) {
+          v[j] = *reinterpret_cast<const float4 *>(src + base + i);
+        } else {
+          v[j].x = i < cnt ? src[base + i] : 0.f;
+          v[j].y = i + 1 < cnt ? src[base + i + 1] : 0.f;
+          v[j].z = i + 2 < cnt ? src[base + i + 2] : 0.f;
+          v[j].w = 0.f;
+        }
+      }
+#pragma unroll
+      for (uint32_t j = 0; j < kSl; ++j) *reinterpret_cast<float4 *>(&slice[(tid + j * kBlkThreads) * 4]) = v[j];
+      cur = un.x;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kLi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      if (i < ne) s_lidx[i] = li[j];
+    }
+#pragma unroll
+    for (uint32_t j = 0; j <= kPi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      if (i <= np) s_pbeg[i] = pb[j] - e0;
+    }
+#pragma unroll
+    for (uint32_t j = 0; j < kPi; ++j) {
+      const uint32_t i = tid + j * kBlkThreads;
+      if (i < np) s_pslot[i] = ps[j];
+    }
+    if (u + 1 < u1) prefetch(u + 1);
+    __syncthreads();
+    sv.bind(slice, e0);
+    if (un.y == 0) {  // one lane per piece (up to 8 entries)
+      for (uint32_t q = tid; q < np; q += kBlkThreads)
+        d.partial[s_pslot[q]] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
+    } else if (un.y == 1) {  // 8 lanes per piece (up to 64)
+      const uint32_t j = tid % 8;
+      for (uint32_t q = tid / 8; q < np; q += kBlkThreads / 8) {
+        float s = piece_sum<8, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
+        s = blk_group_sum<8>(s);
+        if (j == 0) d.partial[s_pslot[q]] = s;
+      }
+    } else {  // one wave per piece (up to kPieceMax)
+      const uint32_t j = tid % kWave;
+      for (uint32_t q = tid / kWave; q < np; q += kBlkThreads / kWave) {
+        float s = piece_sum<kWave, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
+        s = wave_sum(s);
+        if (j == 0) d.partial[s_pslot[q]] = s;
+      }
+    }
+  }
+}
+
+// Combine of iteration k: row v's partials added in a fixed order, then
+// fin.finish(v, sum, err); the residual goes to errs[k % 3] (ctl != nullptr).
+// Rows [0, n_heavy) go a wave each (lane j adds partials j, j + 64, ...,
+// then a shuffle tree); the others a lane each, in slot order.  Rows
+// without entries (v >= n_live) get a constant update: iterations 0 and 1
+// write it into both ping-pong buffers, later iterations skip them.
+template <class Fin>
+__global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, IterCtl *ctl, uint32_t k) {
+  if (ctl && __hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
+  __shared__ float s_red[256 / kWave];
+  const uint32_t nv = (!ctl || k >= 2) ? d.n_live : d.n;
+  float err = 0.f;
+  const uint32_t lane = threadIdx.x % kWave;
+  for (uint32_t v = (blockIdx.x * 256 + threadIdx.x) / kWave; v < d.n_heavy; v += gridDim.x * (256 / kWave)) {
+    const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+    float s = 0.f;
+    for (uint32_t q = a + lane; q < b; q += kWave) s += d.partial[q];
+    s = wave_sum(s);
+    if (lane == 0) fin.finish(v, s, err);
+  }
+  for (uint32_t v = d.n_heavy + blockIdx.x * 256 + threadIdx.x; v < nv; v += gridDim.x * 256) {
+    const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+    float s = 0.f;
+    for (uint32_t q = a; q < b; q += 8) {  // eight loads in flight, added in slot order
+      float p[8];
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) p[i] = q + i < b ? d.partial[q + i] : 0.f;
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i)
+        if (q + i < b) s += p[i];
+    }
+    fin.finish(v, s, err);
+  }
+  if (!ctl) return;
+  err = block_sum<256, float>(err, s_red);
+  if (threadIdx.x == 0 && err != 0.f) atomicAdd(&ctl->errs[k % 3][blockIdx.x % kErrSlots].v, (double)err);
+}
+
+}  // namespace olv

@@ -19,6 +19,7 @@
 #include <cstring>
 #include <vector>
 
+#include "blocked.hpp"
 #include "internal.hpp"
 #include "seg_core.hpp"
 
@@ -52,10 +53,12 @@ struct RelabelSide {
 
 struct PrRelabel {
   RelabelSide in, out;
+  BlockedPlan blk_in;    // source-segmented pull over the renumbered in-CSR
   float *inv = nullptr;  // n: 1/outdeg by new number
   void release() {
     in.release();
     out.release();
+    blk_in.release();
     if (inv) (void)hipFree(inv);
     inv = nullptr;
   }
@@ -207,6 +210,18 @@ static int ensure_side(olv_graph *g, PrWork *w, bool in_edges, bool with_eid) {
   return OLV_OK;
 }
 
+// The source-segmented pull plan over the renumbered in-CSR (built once).
+static int ensure_blocked(olv_graph *g, PrWork *w) {
+  BlockedPlan &bp = w->rl.blk_in;
+  if (bp.ready) return OLV_OK;
+  OLV_TRY(ensure_side(g, w, true, false));
+  const uint32_t n = g->n, m = g->m;
+  std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint32_t>(m, 1));
+  OLV_HIP(hipMemcpy(rp.data(), w->rl.in.rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  if (m) OLV_HIP(hipMemcpy(col.data(), w->rl.in.col, (size_t)m * 4, hipMemcpyDeviceToHost));
+  return build_blocked_plan(rp, col.data(), false, (uint32_t)cu_count(), bp);
+}
+
 // ----------------------------------------------------------------- ops
 // testPageRank.cu:79-91: sum_{u->v} r_u/outdeg(u), r' = d*sum + (1-d)/n,
 // err += |r' - r|.  contrib[u] = r_u * (1/outdeg(u)).
@@ -219,6 +234,27 @@ struct PrOp {
   float d, base;
   __device__ __forceinline__ uint32_t idx(uint32_t k) const { return ld_stream(&ci[k]); }
   __device__ __forceinline__ float val(uint32_t, uint32_t, uint32_t u) const { return ld_gather(&contrib_cur[u]); }
+  __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
+    const float rn = d * s + base;
+    err += fabsf(rn - r[v]);
+    r[v] = rn;
+    contrib_next[v] = rn * inv_outdeg[v];
+  }
+};
+
+// Summand of the segmented pull for plain PageRank: the staged contribution.
+struct SliceVal {
+  const float *sl = nullptr;
+  __device__ __forceinline__ void bind(const float *s, uint32_t) { sl = s; }
+  __device__ __forceinline__ float operator()(uint32_t, uint32_t i) const { return sl[i]; }
+};
+
+// The update of PrOp::finish for the combine of the segmented pull.
+struct PrFin {
+  float *contrib_next;
+  float *r;
+  const float *inv_outdeg;
+  float d, base;
   __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
     const float rn = d * s + base;
     err += fabsf(rn - r[v]);
@@ -528,6 +564,8 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, &w));
   if (relabel) OLV_TRY(ensure_side(g, w, true, false));
+  const bool blocked = relabel && !(flags & OLV_PR_ROW_PULL);
+  if (blocked) OLV_TRY(ensure_blocked(g, w));
   const SegDev dr = make_seg_dev(relabel ? w->rl.in.plan : g->plan_in);
   const uint32_t *rp = relabel ? w->rl.in.rp : g->in_row_ptr;
   const uint32_t *ci = relabel ? w->rl.in.col : g->in_col;
@@ -545,9 +583,24 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   hipLaunchKernelGGL(pr_init_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), g->n, 1.0f / (float)g->n, inv, ri,
                      w->buf[0], w->ctl, epsilon);
   const float base = (float)((1.0 - (double)damping) / (double)g->n);
-  for (uint32_t it = 0; it < max_iters; ++it) {
-    PrOp op{ci, w->buf[it & 1], w->buf[(it + 1) & 1], ri, inv, damping, base};
-    OLV_TRY(enqueue_iteration(dr, rp, op, w, it, &prof));
+  if (blocked) {
+    const BlockedPlan &bp = w->rl.blk_in;
+    const BlkDev bd = make_blk_dev(bp);
+    const unsigned gc = ew_grid(g->n);
+    for (uint32_t it = 0; it < max_iters; ++it) {
+      hipEvent_t a = prof.next(), b = prof.next();
+      if (a) OLV_HIP(hipEventRecord(a, stream()));
+      hipLaunchKernelGGL(blk_segment_kernel<SliceVal>, dim3(bp.ngroups), dim3(kBlkThreads), 0, stream(), bd, SliceVal{},
+                         (const float *)w->buf[it & 1], w->ctl, it);
+      hipLaunchKernelGGL(blk_combine_kernel<PrFin>, dim3(gc), dim3(256), 0, stream(), bd,
+                         PrFin{w->buf[(it + 1) & 1], ri, inv, damping, base}, w->ctl, it);
+      if (b) OLV_HIP(hipEventRecord(b, stream()));
+    }
+  } else {
+    for (uint32_t it = 0; it < max_iters; ++it) {
+      PrOp op{ci, w->buf[it & 1], w->buf[(it + 1) & 1], ri, inv, damping, base};
+      OLV_TRY(enqueue_iteration(dr, rp, op, w, it, &prof));
+    }
   }
   if (relabel && g->n)
     hipLaunchKernelGGL(gather_kernel<float>, dim3(ew_grid(g->n)), dim3(256), 0, stream(), (const float *)ri, g->rl_pi,

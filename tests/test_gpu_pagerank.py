@@ -90,19 +90,41 @@ def star_in_hub(olv, n):
     return olv.CsrGraph(rp, dst[o].astype(np.uint32))
 
 
-@pytest.mark.parametrize("case", ["rmat16", "grid", "star"])
+@pytest.mark.parametrize("case", ["rmat16", "rmat18", "grid", "star"])
 def test_pagerank_relabel_bit_identical(olv, case):
-    """The in-degree relabelled iteration gives the same fp32 ranks, bit for
-    bit, as the iteration in the graph's own order, and both match fp64."""
-    csr = {"rmat16": lambda: olv.gen_rmat(16, 16, seed=7), "grid": lambda: olv.gen_grid(64, 64),
-           "star": lambda: star_in_hub(olv, 20011)}[case]()
+    """The per-row pull over the in-degree relabelled CSR (OLV_PR_ROW_PULL)
+    gives the same fp32 ranks, bit for bit, as the iteration in the graph's
+    own order.  The default source-segmented pull adds the same terms in
+    another fixed order (per 32768-source segment, then across segments):
+    like the others it is within 1e-5 relative of fp64, and run to run it is
+    bit-reproducible.  rmat18 spans 8 segments; the star's hub row (20009
+    in-edges from one segment) is cut into pieces of at most 8192."""
+    csr = {"rmat16": lambda: olv.gen_rmat(16, 16, seed=7), "rmat18": lambda: olv.gen_rmat(18, 16, seed=2),
+           "grid": lambda: olv.gen_grid(64, 64), "star": lambda: star_in_hub(olv, 20011)}[case]()
     g = olv.Graph(csr)
+    row = olv._lib.OLV_PR_ROW_PULL
     for iters in (1, 2, 20):
-        a = g.pagerank(iters, epsilon=0.0)[0].numpy()
+        a = g.pagerank(iters, epsilon=0.0, flags=row)[0].numpy()
         b = g.pagerank(iters, epsilon=0.0, flags=olv._lib.OLV_GRAPH_ORDER)[0].numpy()
         assert np.array_equal(a.view(np.uint32), b.view(np.uint32)), (case, iters)
-    ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, 20, epsilon=0.0)
-    assert maxrel(a, ref) <= 1e-5
+        c = g.pagerank(iters, epsilon=0.0)[0].numpy()
+        c2 = g.pagerank(iters, epsilon=0.0)[0].numpy()
+        assert np.array_equal(c.view(np.uint32), c2.view(np.uint32)), (case, iters)
+        ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0)
+        assert maxrel(a, ref) <= 1e-5 and maxrel(c, ref) <= 1e-5, (case, iters)
+    say(f"  {case}: segmented vs row pull max rel diff {maxrel(c, a.astype(np.float64)):.2e}")
+
+
+def test_pagerank_segmented_epsilon_stop(olv):
+    """The stop rule (L1 change below epsilon) in the segmented pull stops
+    at the same iteration as the oracle."""
+    csr = olv.gen_rmat(17, 16, seed=11)
+    g = olv.Graph(csr)
+    for eps in (1e-3, 1e-5):
+        out, st = g.pagerank(1000, epsilon=eps)
+        ref, updates, _ = oracle.pagerank(csr.row_ptr, csr.col, 1000, epsilon=eps)
+        assert st.iterations == updates, eps
+        assert maxrel(out.numpy(), ref) <= 1e-5
 
 
 # ------------------------------------------------ weighted + adjoint
