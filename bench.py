@@ -339,9 +339,16 @@ def batched_legs(olv, args, csr, g, sssp=True):
     64-graph batch in one captured plan vs 64 plain calls, and (sssp) 16-source
     SSSP on the uniform graph batched vs plain.  Returns a dict of legs."""
     import oracle
+    import tempfile
     L = olv._lib.lib()
     reps = max(3, args.steps // 4)
     srcs = sources_of(csr, 256, 4)
+    # launch shapes and thresholds from the autotuner (measured on this graph,
+    # or taken from the tuning cache under its signature)
+    t0 = time.perf_counter()
+    tune = g.autotune_ex(olv._lib.OLV_TUNE_BFS | olv._lib.OLV_TUNE_BATCHED,
+                         os.path.join(tempfile.gettempdir(), "olive_amd_tuning.jsonl"))
+    t_tune = time.perf_counter() - t0
     out = olv.DeviceBuffer(csr.n * 256, np.int32)
     plan = ctypes.c_void_p()
     t0 = time.perf_counter()
@@ -379,9 +386,9 @@ def batched_legs(olv, args, csr, g, sssp=True):
         olv._lib.check(L.olv_bfs(g.h, int(s), one.c, 0, None))
     L.olv_synchronize()
     t_plain = time.perf_counter() - t0
-    wt, bt, al, be = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_float(), ctypes.c_float()
-    olv._lib.check(L.olv_get_tuning(g.h, ctypes.byref(wt), ctypes.byref(bt), ctypes.byref(al), ctypes.byref(be)))
-    tuning = {"wave_thr": wt.value, "block_thr": bt.value, "alpha": al.value, "beta": be.value}
+    tuning = tune.as_dict()
+    tuning["autotune_wall_ms"] = t_tune * 1e3
+    tuning["signature"] = g.signature()
     L.olv_plan_destroy(plan)
     del out, one
     legs = {"batched_256": {
@@ -400,6 +407,12 @@ def batched_legs(olv, args, csr, g, sssp=True):
     offs = np.zeros(65, np.uint32)
     olv._lib.check(L.olv_graph_union(arr, 64, ctypes.byref(uni), offs.ctypes.data))
     seeds = (offs[:64] + bsrc).astype(np.uint32)
+    utune = olv._lib.Tuning()
+    t0 = time.perf_counter()
+    olv._lib.check(L.olv_autotune_ex(uni, olv._lib.OLV_TUNE_BFS,
+                                     os.path.join(tempfile.gettempdir(), "olive_amd_tuning.jsonl").encode(),
+                                     ctypes.byref(utune)))
+    t_utune = time.perf_counter() - t0
     uout = olv.DeviceBuffer(int(offs[64]), np.int32)
     uplan = ctypes.c_void_p()
     olv._lib.check(L.olv_plan_bfs(uni, seeds.ctypes.data, 64, uout.c, 0, ctypes.byref(uplan)))
@@ -433,7 +446,8 @@ def batched_legs(olv, args, csr, g, sssp=True):
     legs["multigraph_64"] = {
         "graphs": 64, "scales": [12, 18], "vertices": int(offs[64]), "edges": int(sum(c.m for c in csrs)),
         "plan_ms": t_union * 1e3, "plan_wall_ms": t_union_wall * 1e3, "plain_64_wall_ms": t_plain_multi * 1e3,
-        "speedup_wall": t_plain_multi / t_union_wall, "bit_exact_all_graphs": bool(uexact)}
+        "speedup_wall": t_plain_multi / t_union_wall, "bit_exact_all_graphs": bool(uexact),
+        "tuning": dict(utune.as_dict(), autotune_wall_ms=t_utune * 1e3)}
     del gs, outs, uout
     if not sssp:
         return legs

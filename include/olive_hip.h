@@ -329,7 +329,10 @@ int olv_bfs_multigraph(olv_graph *const *gs, uint32_t k, const uint32_t *srcs,
 /* Capture a whole traversal (state reset, seeding, persistent kernel) once;
  * olv_plan_launch replays it on the current stream.  `levels` (device) is
  * bound at capture time.  A plan shares the graph's work buffers: do not
- * run two plans of one graph concurrently. */
+ * run two plans of one graph concurrently.  A plan runs with the graph's
+ * tuning at capture time; a graph never tuned first takes its signature's
+ * entry in the tuning cache named by the OLV_TUNE_CACHE environment
+ * variable, when there is one (olv_autotune_ex). */
 typedef struct olv_plan olv_plan;
 int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k,
                  int32_t *levels, unsigned flags, olv_plan **out);
@@ -339,11 +342,59 @@ int olv_plan_bfs_batched(const olv_graph *g, const uint32_t *srcs, uint32_t k,
 int olv_plan_launch(olv_plan *p, olv_stats *st);
 int olv_plan_destroy(olv_plan *p);
 
-/* ---- autotuner --------------------------------------------------------- */
-/* Time the plain BFS over candidate degree-bucket thresholds (thread / wave /
- * workgroup per vertex) and direction-switch parameters on this graph and
- * keep the fastest in the handle.  Results are returned through the
- * out-pointers (any may be NULL). */
+/* ---- autotuner --------------------------------------------------------- *
+ * No reference counterpart: Olive's programs hand-pick GroupSize per run
+ * (BFS.cu:127-135); here the launch shapes and thresholds of every kernel
+ * family are run-time settings of the graph handle, searched on the graph
+ * itself and kept in a cache keyed by the graph's signature. */
+typedef struct olv_tuning {
+  uint32_t bfs_expand_per_cu;       /* BFS expansion workgroups per CU (graph numbering) */
+  uint32_t bfs_expand_per_cu_twin;  /* the same on the renumbered twin                   */
+  uint32_t bfs_compact_per_cu;      /* BFS/SSSP compaction workgroups per CU             */
+  uint32_t td_wave_thr;             /* push: out-degree >= this: a wave per vertex       */
+  uint32_t td_block_thr;            /* push: out-degree >= this: a workgroup per vertex  */
+  uint32_t hub_thr;                 /* push: out-degree >= this: items of hub_thr edges
+                                       (>= 256)                                          */
+  float alpha, beta;                /* BFS direction switch (Beamer)                     */
+  uint32_t mbfs_pull_per_cu;        /* batched BFS bottom-up workgroups per CU           */
+  uint32_t seg_grid_per_cu;         /* row pulls (weighted PageRank, adjoint, per-row
+                                       PageRank): workgroups per CU                      */
+  uint32_t seg_unroll;              /* row pulls: gathers in flight per lane (4, 8, 16)  */
+  uint32_t seg_bounds[3];           /* row pulls: 1 lane per row up to seg_bounds[0]
+                                       entries, 8 lanes up to [1], a wave up to [2],
+                                       then a workgroup                                  */
+  uint32_t sssp_relax_per_cu;       /* SSSP relaxation workgroups per CU                 */
+  uint32_t source;                  /* 0 defaults, 1 measured by olv_autotune_ex, 2 read
+                                       from a tuning cache, 3 set by the caller          */
+  double tune_ms;                   /* host time olv_autotune_ex spent (0 on a hit)      */
+} olv_tuning;
+
+#define OLV_TUNE_BFS     0x1u  /* plain BFS: grids, bucket thresholds, hubs, alpha/beta */
+#define OLV_TUNE_BATCHED 0x2u  /* 256-source BFS: bottom-up grid                         */
+#define OLV_TUNE_PULL    0x4u  /* row pulls: grid, unroll, bucket bounds                 */
+#define OLV_TUNE_SSSP    0x8u  /* SSSP relaxation grid (graphs with int32 weights)       */
+#define OLV_TUNE_ALL     0xFu
+
+int olv_get_tuning_ex(const olv_graph *g, olv_tuning *out);
+/* Validates every field (EINVAL names the first bad one). */
+int olv_set_tuning_ex(olv_graph *g, const olv_tuning *t);
+/* Search the settings of the kernel families in `what` on this graph
+ * (coordinate descent over candidate values, timing runs of the engine
+ * itself) and keep the winner in the handle.  With cache_path (a JSON Lines
+ * file), a graph whose signature is already in the cache takes the cached
+ * settings without measuring, and a measured result is written back. */
+int olv_autotune_ex(olv_graph *g, unsigned what, const char *cache_path, olv_tuning *out);
+/* Signature of a graph: vertex and edge counts and a 64-bit hash of its
+ * log2 out- and in-degree histograms, as "v<n>-e<m>-h<hex>" (key of the
+ * tuning cache).  olv_csr_signature computes it from a host CSR. */
+int olv_graph_signature(const olv_graph *g, char *key, size_t len);
+int olv_csr_signature(uint32_t n, uint32_t m, const uint32_t *row_ptr, const uint32_t *col, char *key, size_t len);
+/* The tuning cache (host only).  load: *found = 1 and *out filled when the
+ * key is present; store: adds or replaces the key's line. */
+int olv_tuning_cache_load(const char *path, const char *key, olv_tuning *out, int *found);
+int olv_tuning_cache_store(const char *path, const char *key, const olv_tuning *t);
+/* Plain-BFS tuning only, without a cache (olv_autotune_ex with OLV_TUNE_BFS);
+ * returns the bucket thresholds in the form olv_set_buckets takes. */
 int olv_autotune(olv_graph *g, uint32_t *thread_max, uint32_t *wave_max,
                  uint32_t *block_threads);
 int olv_set_buckets(olv_graph *g, uint32_t thread_max, uint32_t wave_max,

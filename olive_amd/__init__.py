@@ -21,7 +21,7 @@ import numpy as np
 
 from ._lib import (OLV_BUILD_TRANSPOSE, OLV_FMT_DIMACS, OLV_FMT_EDGE_LIST, OLV_FORCE_BOTTOM_UP, OLV_GRAPH_ORDER,
                    OLV_HOST_LOOP, OLV_NO_DIRECTION_OPT, OLV_PERSISTENT, OLV_PROFILE, OLV_PTR_HOST, OLV_W_F32, OLV_W_I32, OLV_W_NONE,
-                   HostCsr, OliveError, Stats, check, lib)
+                   HostCsr, OliveError, Stats, Tuning, check, lib)
 
 INF = 0x7FFFFFFF
 
@@ -152,6 +152,25 @@ class CsrGraph:
 
     def with_weights(self, w):
         return CsrGraph(self.row_ptr, self.col, w)
+
+
+def csr_signature(g: CsrGraph) -> str:
+    """olv_csr_signature of a host CSR (the tuning cache key)."""
+    buf = ctypes.create_string_buffer(96)
+    check(lib().olv_csr_signature(g.n, g.m, g.row_ptr.ctypes.data, _np_ptr(g.col), buf, 96), "olv_csr_signature")
+    return buf.value.decode()
+
+
+def tuning_cache_load(path, key: str):
+    """The cached olv_tuning of `key`, or None."""
+    t, found = Tuning(), ctypes.c_int()
+    check(lib().olv_tuning_cache_load(str(path).encode(), key.encode(), ctypes.byref(t), ctypes.byref(found)),
+          "olv_tuning_cache_load")
+    return t if found.value else None
+
+
+def tuning_cache_store(path, key: str, t: Tuning):
+    check(lib().olv_tuning_cache_store(str(path).encode(), key.encode(), ctypes.byref(t)), "olv_tuning_cache_store")
 
 
 def _take_host_csr(h: HostCsr) -> CsrGraph:
@@ -596,6 +615,28 @@ class Graph:
         a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
         check(lib().olv_autotune(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "olv_autotune")
         return a.value, b.value, c.value
+
+    def tuning(self) -> Tuning:
+        """The handle's current olv_tuning."""
+        t = Tuning()
+        check(lib().olv_get_tuning_ex(self.h, ctypes.byref(t)), "olv_get_tuning_ex")
+        return t
+
+    def set_tuning(self, t: Tuning):
+        check(lib().olv_set_tuning_ex(self.h, ctypes.byref(t)), "olv_set_tuning_ex")
+
+    def autotune_ex(self, what: int = 0xF, cache_path: str | None = None) -> Tuning:
+        """olv_autotune_ex: search the kernel families in `what` on this graph
+        (or take the cached result for its signature from cache_path)."""
+        t = Tuning()
+        check(lib().olv_autotune_ex(self.h, what, None if cache_path is None else str(cache_path).encode(),
+                                    ctypes.byref(t)), "olv_autotune_ex")
+        return t
+
+    def signature(self) -> str:
+        buf = ctypes.create_string_buffer(96)
+        check(lib().olv_graph_signature(self.h, buf, 96), "olv_graph_signature")
+        return buf.value.decode()
 
     def set_buckets(self, thread_max, wave_max, block_threads=0):
         check(lib().olv_set_buckets(self.h, thread_max, wave_max, block_threads), "olv_set_buckets")

@@ -28,6 +28,7 @@ OLV_GRAPH_ORDER = 0x40
 OLV_BUILD_TRANSPOSE = 0x100
 OLV_W_UNCHANGED = 0x200
 OLV_PR_ROW_PULL = 0x400
+OLV_TUNE_BFS, OLV_TUNE_BATCHED, OLV_TUNE_PULL, OLV_TUNE_SSSP, OLV_TUNE_ALL = 0x1, 0x2, 0x4, 0x8, 0xF
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
 OLV_FMT_EDGE_LIST, OLV_FMT_DIMACS = 0, 1
@@ -56,6 +57,34 @@ class Stats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Tuning(ctypes.Structure):
+    """olv_tuning: launch shapes and thresholds of a graph handle."""
+    _fields_ = [
+        ("bfs_expand_per_cu", ctypes.c_uint32),
+        ("bfs_expand_per_cu_twin", ctypes.c_uint32),
+        ("bfs_compact_per_cu", ctypes.c_uint32),
+        ("td_wave_thr", ctypes.c_uint32),
+        ("td_block_thr", ctypes.c_uint32),
+        ("hub_thr", ctypes.c_uint32),
+        ("alpha", ctypes.c_float),
+        ("beta", ctypes.c_float),
+        ("mbfs_pull_per_cu", ctypes.c_uint32),
+        ("seg_grid_per_cu", ctypes.c_uint32),
+        ("seg_unroll", ctypes.c_uint32),
+        ("seg_bounds", ctypes.c_uint32 * 3),
+        ("sssp_relax_per_cu", ctypes.c_uint32),
+        ("source", ctypes.c_uint32),
+        ("tune_ms", ctypes.c_double),
+    ]
+    SOURCES = {0: "defaults", 1: "measured", 2: "cache", 3: "caller"}
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["seg_bounds"] = list(self.seg_bounds)
+        d["source"] = self.SOURCES.get(self.source, self.source)
+        return d
 
 
 class HostCsr(ctypes.Structure):
@@ -139,6 +168,13 @@ SIGNATURES = {
     "olv_set_buckets": (_int, [_vp, _u32, _u32, _u32]),
     "olv_set_direction": (_int, [_vp, _f32, _f32]),
     "olv_get_tuning": (_int, [_vp, _u32p, _u32p, ctypes.POINTER(_f32), ctypes.POINTER(_f32)]),
+    "olv_get_tuning_ex": (_int, [_vp, ctypes.POINTER(Tuning)]),
+    "olv_set_tuning_ex": (_int, [_vp, ctypes.POINTER(Tuning)]),
+    "olv_autotune_ex": (_int, [_vp, _uint, ctypes.c_char_p, ctypes.POINTER(Tuning)]),
+    "olv_graph_signature": (_int, [_vp, ctypes.c_char_p, _sz]),
+    "olv_csr_signature": (_int, [_u32, _u32, _vp, _vp, ctypes.c_char_p, _sz]),
+    "olv_tuning_cache_load": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Tuning), ctypes.POINTER(_int)]),
+    "olv_tuning_cache_store": (_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(Tuning)]),
     "olv_gen_grid": (_int, [_u32, _u32, _csrp]),
     "olv_gen_rmat": (_int, [_u32, _u32, _f64, _f64, _f64, _u64, _int, _csrp]),
     "olv_gen_uniform": (_int, [_u32, _u32, _u64, _csrp]),

@@ -24,12 +24,6 @@
 #include "internal.hpp"
 #include "seg_core.hpp"
 
-// Workgroups per CU of the batched pull kernel, at most one per virtual
-// block of its bucket plan (256 sources on RMAT-22: 48 gives 7.21 ms; 64:
-// 7.21, 96: 7.25, 32: 7.30, 16: 7.44, 8: 8.17).
-#ifndef OLV_MBFS_PULL_PER_CU
-#define OLV_MBFS_PULL_PER_CU 48
-#endif
 
 namespace olv {
 
@@ -84,6 +78,7 @@ struct MbfsParams {
   uint32_t wave_thr, block_thr, hub_thr;
   float alpha, beta;
   uint32_t can_bu;
+  uint32_t pull_per_cu;  // pull workgroups per CU (host side only)
 };
 
 __device__ __forceinline__ uint32_t wave_or(uint32_t v) {
@@ -216,7 +211,7 @@ __global__ void __launch_bounds__(kMB) mbfs_expand_kernel(MbfsParams p) {
   const int32_t nlevel = (int32_t)(L + 1);
   if (st.mode == 0) {
     MbfsTdVisitor vis(p, Ff, Tc);
-    push_expand<kMB>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, blockIdx.x, gridDim.x, vis,
+    push_expand<kMB>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, p.hub_thr, blockIdx.x, gridDim.x, vis,
                      insp);
   }
   flush_counters<kMB>(&ctl->cnt[L % 3], nf, mf, insp);
@@ -664,6 +659,7 @@ static MbfsParams mbfs_params(const olv_graph *g, MbfsWork &w, uint32_t k, int32
   p.alpha = g->tune.alpha;
   p.beta = g->tune.beta;
   p.can_bu = g->in_row_ptr ? 1u : 0u;
+  p.pull_per_cu = g->tune.mbfs_pull_per_cu;
   return p;
 }
 
@@ -680,8 +676,11 @@ static int mbfs_enqueue_levels(const MbfsParams &p, const SegDev &d, uint32_t co
   const unsigned ge = (unsigned)cu_count() * 8;
   const unsigned gw = mgrid(p.nw, kMB);
   const unsigned gv = mgrid((size_t)p.nw * 32, kMB);  // one lane per vertex
-  const unsigned gp = std::max<unsigned>(
-      1u, std::min<unsigned>(d.role_start[5], (unsigned)cu_count() * OLV_MBFS_PULL_PER_CU));
+  // pull workgroups per CU, at most one per virtual block of the bucket
+  // plan (256 sources on RMAT-22: 48 gives 7.21 ms; 64: 7.21, 96: 7.25, 32:
+  // 7.30, 16: 7.44, 8: 8.17)
+  const unsigned gp =
+      std::max<unsigned>(1u, std::min<unsigned>(d.role_start[5], (unsigned)cu_count() * p.pull_per_cu));
   for (uint32_t i = 0; i < count; ++i) {
     hipLaunchKernelGGL(mbfs_expand_kernel, dim3(ge), dim3(kMB), 0, s, p);
     hipLaunchKernelGGL(mbfs_pull_kernel, dim3(gp), dim3(kMB), 0, s, p, d);
@@ -729,11 +728,11 @@ static int mbfs_pull_plan(olv_graph *g, unsigned flags, SegDev *d) {
   std::memset(d, 0, sizeof(*d));
   if (flags & OLV_NO_DIRECTION_OPT) return OLV_OK;
   if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
-  if (!g->plan_in_ready) {
+  if (!g->plan_in_ready || seg_plan_stale(g->plan_in, g->tune)) {
     OLV_TRY(build_seg_plan(g, true, g->plan_in));
     g->plan_in_ready = true;
   }
-  *d = make_seg_dev(g->plan_in);
+  *d = make_seg_dev(g->plan_in, g->tune);
   return OLV_OK;
 }
 
@@ -1024,6 +1023,7 @@ int olv_plan_bfs(const olv_graph *g, const uint32_t *srcs, uint32_t k, int32_t *
   *out = nullptr;
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_plan_bfs: source %u out of range", srcs[i]);
+  OLV_TRY(apply_cached_tuning(const_cast<olv_graph *>(g)));
   olv_plan *p = new olv_plan();
   p->kind = 1;
   p->g = g;
@@ -1057,6 +1057,7 @@ int olv_plan_bfs_batched(const olv_graph *cg, const uint32_t *srcs, uint32_t k, 
   olv_graph *g = const_cast<olv_graph *>(cg);
   for (uint32_t i = 0; i < k; ++i)
     if (srcs[i] >= g->n) return fail(OLV_EINVAL, "olv_plan_bfs_batched: source %u out of range", srcs[i]);
+  OLV_TRY(apply_cached_tuning(g));
   std::vector<uint32_t> seeds(srcs, srcs + k);
   olv_graph *rg = nullptr;
   OLV_TRY(mbfs_target(g, flags, seeds, &rg));
@@ -1159,57 +1160,6 @@ int olv_get_tuning(const olv_graph *g, uint32_t *wave_thr, uint32_t *block_thr, 
   if (block_thr) *block_thr = g->tune.td_block_thr;
   if (alpha) *alpha = g->tune.alpha;
   if (beta) *beta = g->tune.beta;
-  return OLV_OK;
-}
-
-// Time the plain BFS over a grid of (wave threshold, workgroup threshold,
-// alpha) on sources picked from the graph (its largest out-degree vertex and
-// two more with out-edges) and keep the fastest setting.
-int olv_autotune(olv_graph *g, uint32_t *thread_max, uint32_t *wave_max, uint32_t *block_threads) {
-  if (!g) return fail(OLV_EINVAL, "olv_autotune: NULL graph");
-  std::vector<uint32_t> rp((size_t)g->n + 1);
-  OLV_HIP(hipMemcpy(rp.data(), g->row_ptr, rp.size() * 4, hipMemcpyDeviceToHost));
-  std::vector<uint32_t> srcs;
-  uint32_t best_v = 0;
-  for (uint32_t v = 0; v < g->n; ++v)
-    if (rp[v + 1] - rp[v] > rp[best_v + 1] - rp[best_v]) best_v = v;
-  srcs.push_back(best_v);
-  for (uint32_t step = 1; step <= 2; ++step) {
-    uint32_t v = (uint32_t)(((uint64_t)g->n * step) / 3);
-    for (uint32_t t = 0; t < g->n && rp[v + 1] == rp[v]; ++t) v = (v + 1) % g->n;
-    srcs.push_back(v);
-  }
-  DevBuf lv;
-  OLV_TRY(lv.alloc((size_t)g->n * 4));
-  const uint32_t waves[] = {16, 32, 64};
-  const uint32_t blocks[] = {256, 1024, 4096};
-  const float alphas[] = {4.f, 14.f, 40.f};
-  Tuning best = g->tune;
-  double best_ms = 1e300;
-  for (uint32_t wt : waves)
-    for (uint32_t bt : blocks)
-      for (float al : alphas) {
-        g->tune.td_wave_thr = wt;
-        g->tune.td_block_thr = bt;
-        g->tune.alpha = al;
-        double tot = 0;
-        for (uint32_t s : srcs) {
-          olv_stats st;
-          // warm once, then time
-          OLV_TRY(bfs_multisource(g, &s, 1, true, lv.as<int32_t>(), 0, nullptr));
-          OLV_TRY(bfs_multisource(g, &s, 1, true, lv.as<int32_t>(), 0, &st));
-          tot += st.elapsed_ms;
-        }
-        if (tot < best_ms) {
-          best_ms = tot;
-          best = g->tune;
-        }
-      }
-  best.tuned = true;
-  g->tune = best;
-  if (thread_max) *thread_max = g->tune.td_wave_thr - 1;
-  if (wave_max) *wave_max = g->tune.td_block_thr - 1;
-  if (block_threads) *block_threads = g->tune.block_threads;
   return OLV_OK;
 }
 

@@ -32,19 +32,11 @@ namespace olv {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kSpinLimit = 1u << 24;
-// Workgroups per CU of the per-level kernels.  Expansion in the graph's own
-// numbering: 8 (full occupancy; 4 and 16 measured slower).  On the
-// renumbered twin: 6 (RMAT-22 0.148 ms; 7: 0.148, 5: 0.150, 4: 0.154,
-// 8: 0.160, 3: 0.165).  Compaction is insensitive.
-#ifndef OLV_BFS_EXPAND_PER_CU
-#define OLV_BFS_EXPAND_PER_CU 8
-#endif
-#ifndef OLV_BFS_EXPAND_PER_CU_TWIN
-#define OLV_BFS_EXPAND_PER_CU_TWIN 6
-#endif
-#ifndef OLV_BFS_COMPACT_PER_CU
-#define OLV_BFS_COMPACT_PER_CU 8
-#endif
+// Workgroups per CU of the per-level kernels come from the graph's tuning
+// (Tuning::bfs_*_per_cu).  Defaults: expansion in the graph's own numbering
+// 8 (full occupancy; 4 and 16 measured slower); on the renumbered twin 6
+// (RMAT-22 0.148 ms; 7: 0.148, 5: 0.150, 4: 0.154, 8: 0.160, 3: 0.165);
+// compaction 8 (insensitive).
 
 struct BfsState {              // per level parity
   unsigned long long mu;       // unexplored out-edges before this level
@@ -418,8 +410,8 @@ int bfs_enqueue_init(const olv_graph *g, BfsWork *w, const uint32_t *d_seeds, ui
 int bfs_enqueue_levels(const olv_graph *g, BfsWork *w, int32_t *levels, bool can_bu, uint32_t count) {
   const BfsArgs a = make_args(g, w, levels);
   const BfsDir dir = make_dir(g, w, can_bu);
-  const unsigned ge = (unsigned)cu_count() * (a.omap ? OLV_BFS_EXPAND_PER_CU_TWIN : OLV_BFS_EXPAND_PER_CU);
-  const unsigned gc = grid_for(w->nw, kBlock, OLV_BFS_COMPACT_PER_CU);
+  const unsigned ge = (unsigned)cu_count() * (a.omap ? g->tune.bfs_expand_per_cu_twin : g->tune.bfs_expand_per_cu);
+  const unsigned gc = grid_for(w->nw, kBlock, g->tune.bfs_compact_per_cu);
   for (uint32_t i = 0; i < count; ++i) {
     if (a.omap)
       hipLaunchKernelGGL(bfs_expand_kernel<true>, dim3(ge), dim3(kBlock), 0, stream(), a, w->ctl, w->bms, w->nwp);

@@ -9,17 +9,13 @@
 //
 // A frontier handed to a push step has two parts: a queue of vertices with
 // out-degree below hub_thr, and a list of hub items (vertex, chunk) that cut
-// each higher-degree out-list into kHubChunk-edge pieces, one workgroup per
+// each higher-degree out-list into hub_thr-edge pieces, one workgroup per
 // piece -- so no single workgroup walks a 10^5-edge list alone.
 #pragma once
 #include "device.hpp"
+#include "internal.hpp"
 
 namespace olv {
-
-#ifndef OLV_HUB_CHUNK
-#define OLV_HUB_CHUNK 1024
-#endif
-constexpr uint32_t kHubChunk = OLV_HUB_CHUNK;
 
 // Level counters.  Every workgroup adds its totals with device-scope
 // atomics, which on MI355X are performed at the memory side and serialise
@@ -88,15 +84,16 @@ struct BfsArgs {
   uint32_t n;
   uint32_t wave_thr;   // degree >= wave_thr: one wave per vertex
   uint32_t block_thr;  // degree >= block_thr: whole workgroup per vertex
-  uint32_t hub_thr;    // degree >= hub_thr: kHubChunk-edge hub items
+  uint32_t hub_thr;    // degree >= hub_thr: hub items of hub_thr edges
   // Mapped: stores go through omap (a template flag, not a run-time test:
   // the test in the claim path slowed small-level traversals by a quarter)
   template <bool Mapped>
   __device__ __forceinline__ void set_level(uint32_t v, int32_t l) const { levels[Mapped ? omap[v] : v] = l; }
 };
 
-// Hub items needed for the worst case of m edges.
-inline size_t hub_capacity(uint64_t m) { return 2 * (m / kHubChunk) + 64; }
+// Hub items needed for the worst case of m edges (items of at least
+// kHubThrMin edges).
+inline size_t hub_capacity(uint64_t m) { return 2 * (m / kHubThrMin) + 64; }
 
 // Visit edges e, e + stride, ... < end with U edges in flight per lane: the
 // U target loads are issued together, then the U state loads, then the
@@ -140,8 +137,8 @@ constexpr int kVisitUnroll = OLV_VISIT_UNROLL;
 // entry; edges are handled through visit_range (tgt / state / apply).
 template <int BLOCK, class Visitor>
 __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *hubs, uint32_t nhubs,
-                            const uint32_t *row_ptr, uint32_t wave_thr, uint32_t block_thr, uint32_t vb, uint32_t nvb,
-                            Visitor &vis, unsigned long long &insp) {
+                            const uint32_t *row_ptr, uint32_t wave_thr, uint32_t block_thr, uint32_t hub_chunk,
+                            uint32_t vb, uint32_t nvb, Visitor &vis, unsigned long long &insp) {
   using P = decltype(vis.load(0u));
   __shared__ uint32_t s_owner, s_beg, s_end;
   __shared__ P s_own_payload;
@@ -154,9 +151,9 @@ __device__ void push_expand(const uint32_t *queue, uint32_t qsize, const uint2 *
   for (uint32_t h = vb; h < nhubs; h += nvb) {
     const uint2 it = hubs[h];
     const uint32_t u = vis.row(it.x);
-    const uint32_t beg = row_ptr[u] + it.y * kHubChunk;
+    const uint32_t beg = row_ptr[u] + it.y * hub_chunk;
     const uint32_t e1 = row_ptr[u + 1];
-    const uint32_t end = e1 < beg + kHubChunk ? e1 : beg + kHubChunk;
+    const uint32_t end = e1 < beg + hub_chunk ? e1 : beg + hub_chunk;
     const P pl = vis.load(it.x);
     if (threadIdx.x == 0) insp += end - beg;
     visit_range<U>(vis, beg + threadIdx.x, end, BLOCK, pl);
@@ -270,7 +267,8 @@ template <int BLOCK, bool Mapped>
 __device__ void td_step(const BfsArgs &a, uint32_t qsize, uint32_t nhubs, int32_t nlevel, uint32_t vb, uint32_t nvb,
                         uint32_t &nf, unsigned long long &mf, unsigned long long &insp) {
   BfsVisitor<Mapped> vis(a, nlevel);
-  push_expand<BLOCK>(a.queue, qsize, a.hubs, nhubs, a.row_ptr, a.wave_thr, a.block_thr, vb, nvb, vis, insp);
+  push_expand<BLOCK>(a.queue, qsize, a.hubs, nhubs, a.row_ptr, a.wave_thr, a.block_thr, a.hub_thr, vb, nvb, vis,
+                     insp);
   nf += vis.nf;
   mf += vis.mf;
 }
@@ -426,7 +424,7 @@ __device__ void compact_words(uint32_t nwords, WordFn word_of, const uint32_t *r
       const uint32_t u = row(v);
       const uint32_t d = row_ptr[u + 1] - row_ptr[u];
       if (d >= hub_thr) {
-        const uint32_t k = (d + kHubChunk - 1) / kHubChunk;
+        const uint32_t k = (d + hub_thr - 1) / hub_thr;
         const uint32_t hb = atomicAdd(hcount, k);
         for (uint32_t j = 0; j < k; ++j) hubs[hb + j] = make_uint2(v, j);
       } else {
@@ -542,7 +540,7 @@ __device__ void seed_frontier(const uint32_t *seeds, uint32_t k, const uint32_t 
     const uint32_t d = row_ptr[u + 1] - row_ptr[u];
     atomicAdd(&s_deg, (unsigned long long)d);
     if (d >= hub_thr) {
-      const uint32_t kk = (d + kHubChunk - 1) / kHubChunk;
+      const uint32_t kk = (d + hub_thr - 1) / hub_thr;
       const uint32_t hb = atomicAdd(hcount, kk);
       for (uint32_t j = 0; j < kk; ++j) hubs[hb + j] = make_uint2(q, j);
     } else {

@@ -43,15 +43,28 @@ struct SegPlan {
   void release();
 };
 
+// Launch shapes and thresholds of one graph (olv_tuning in the C ABI; the
+// defaults are the settings measured best on RMAT graphs).
 struct Tuning {
+  uint32_t bfs_expand_per_cu = 8;       // BFS expansion workgroups per CU (graph numbering)
+  uint32_t bfs_expand_per_cu_twin = 6;  // the same on the renumbered twin
+  uint32_t bfs_compact_per_cu = 8;      // BFS compaction workgroups per CU
   uint32_t td_block_thr = 256;  // degree >= this: whole workgroup per vertex
   uint32_t td_wave_thr = 64;    // degree >= this: one wave per vertex
-  uint32_t hub_thr = OLV_HUB_CHUNK;  // degree >= this: split into OLV_HUB_CHUNK-edge items
+  uint32_t hub_thr = OLV_HUB_CHUNK;  // degree >= this: split into hub items of hub_thr edges
   uint32_t block_threads = 256;
   float alpha = 14.0f;          // top-down -> bottom-up when m_f > m_u / alpha
   float beta = 24.0f;           // bottom-up -> top-down when n_f < n / beta
+  uint32_t mbfs_pull_per_cu = 48;    // batched BFS bottom-up workgroups per CU
+  uint32_t seg_grid_per_cu = 32;     // row pulls: workgroups per CU
+  uint32_t seg_unroll = 8;           // row pulls: gathers in flight per lane (4, 8, 16)
+  uint32_t seg_bounds[3] = {8, 96, 1024};  // row pulls: 1 lane / 8 lanes / a wave up to these degrees
+  uint32_t sssp_relax_per_cu = 6;    // SSSP relaxation workgroups per CU
+  uint32_t source = 0;               // 0 defaults, 1 measured, 2 tuning cache, 3 set by the caller
+  double tune_ms = 0.0;
   bool tuned = false;
 };
+constexpr uint32_t kHubThrMin = 256;  // smallest hub item (sizes the hub item buffers)
 
 struct BfsWork;   // bfs.hip
 struct PrWork;    // pagerank.hip
@@ -95,7 +108,13 @@ namespace olv {
 // Device helpers implemented in runtime.hip
 int fill_u32(uint32_t *p, uint32_t v, size_t count);
 int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan);
-int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan);  // rp: host row offsets (n + 1)
+// rp: host row offsets (n + 1); bounds: the bucket degree bounds (Tuning::seg_bounds)
+int build_seg_plan_rp(const std::vector<uint32_t> &rp, const uint32_t *bounds, SegPlan &plan);
+// plan was built with bounds other than the graph's current ones
+bool seg_plan_stale(const SegPlan &plan, const Tuning &t);
+// A graph never tuned takes its signature's entry in the tuning cache named
+// by the OLV_TUNE_CACHE environment variable, if any (autotune.hip).
+int apply_cached_tuning(olv_graph *g);
 // In-degree numbering of g (g->rl_pi, g->rl_order, g->h_rl_pi), built
 // once; needs the in-CSR.  order (optional) receives new number -> vertex.
 int indeg_order(olv_graph *g, std::vector<uint32_t> *order);

@@ -126,11 +126,11 @@ static int get_pr_work(olv_graph *g, PrWork **out) {
 static int ensure_plan(olv_graph *g, bool in_edges) {
   if (in_edges) {
     if (!g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
-    if (!g->plan_in_ready) {
+    if (!g->plan_in_ready || seg_plan_stale(g->plan_in, g->tune)) {
       OLV_TRY(build_seg_plan(g, true, g->plan_in));
       g->plan_in_ready = true;
     }
-  } else if (!g->plan_out_ready) {
+  } else if (!g->plan_out_ready || seg_plan_stale(g->plan_out, g->tune)) {
     OLV_TRY(build_seg_plan(g, false, g->plan_out));
     g->plan_out_ready = true;
   }
@@ -178,7 +178,14 @@ static unsigned ew_grid(size_t n);
 // edge ids when with_eid (weighted solves).
 static int ensure_side(olv_graph *g, PrWork *w, bool in_edges, bool with_eid) {
   RelabelSide &sd = in_edges ? w->rl.in : w->rl.out;
-  if (sd.ready && (sd.eid || !with_eid)) return OLV_OK;
+  if (sd.ready && (sd.eid || !with_eid)) {
+    if (seg_plan_stale(sd.plan, g->tune)) {  // the bucket bounds changed: new plan, same arrays
+      std::vector<uint32_t> rp((size_t)g->n + 1);
+      OLV_HIP(hipMemcpy(rp.data(), sd.rp, rp.size() * 4, hipMemcpyDeviceToHost));
+      OLV_TRY(build_seg_plan_rp(rp, g->tune.seg_bounds, sd.plan));
+    }
+    return OLV_OK;
+  }
   sd.release();
   const uint32_t n = g->n, m = g->m;
   if (in_edges && !g->in_row_ptr) OLV_TRY(olv_graph_build_transpose(g));
@@ -204,7 +211,7 @@ static int ensure_side(olv_graph *g, PrWork *w, bool in_edges, bool with_eid) {
                        g->rl_pi, d_src_rp, in_edges ? g->in_col : g->col, in_edges ? g->in_eid : nullptr, sd.rp, n,
                        sd.col, sd.eid);
   OLV_HIP(hipGetLastError());
-  OLV_TRY(build_seg_plan_rp(rp, sd.plan));
+  OLV_TRY(build_seg_plan_rp(rp, g->tune.seg_bounds, sd.plan));
   OLV_HIP(hipStreamSynchronize(stream()));
   sd.ready = true;
   return OLV_OK;
@@ -434,12 +441,7 @@ static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, 
                              KernelProfile *prof = nullptr) {
   hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
   if (a) OLV_HIP(hipEventRecord(a, stream()));
-  if (d.contig)
-    hipLaunchKernelGGL((seg_iter_kernel<Op, true>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op, w->ctl,
-                       k);
-  else
-    hipLaunchKernelGGL((seg_iter_kernel<Op, false>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, op,
-                       w->ctl, k);
+  launch_seg(d, rp, op, w->ctl, k);
   if (b) OLV_HIP(hipEventRecord(b, stream()));
   OLV_HIP(hipGetLastError());
   return OLV_OK;
@@ -529,13 +531,13 @@ static int pull_view(olv_graph *g, PrWork *w, unsigned flags, PullView *v) {
   v->relabel = (flags & OLV_GRAPH_ORDER) == 0;
   if (v->relabel) {
     OLV_TRY(ensure_side(g, w, true, true));
-    v->d = make_seg_dev(w->rl.in.plan);
+    v->d = make_seg_dev(w->rl.in.plan, g->tune);
     v->rp = w->rl.in.rp;
     v->ci = w->rl.in.col;
     v->eid = w->rl.in.eid;
   } else {
     OLV_TRY(ensure_plan(g, true));
-    v->d = make_seg_dev(g->plan_in);
+    v->d = make_seg_dev(g->plan_in, g->tune);
     v->rp = g->in_row_ptr;
     v->ci = g->in_col;
     v->eid = g->in_eid;
@@ -566,7 +568,7 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (relabel) OLV_TRY(ensure_side(g, w, true, false));
   const bool blocked = relabel && !(flags & OLV_PR_ROW_PULL);
   if (blocked) OLV_TRY(ensure_blocked(g, w));
-  const SegDev dr = make_seg_dev(relabel ? w->rl.in.plan : g->plan_in);
+  const SegDev dr = make_seg_dev(relabel ? w->rl.in.plan : g->plan_in, g->tune);
   const uint32_t *rp = relabel ? w->rl.in.rp : g->in_row_ptr;
   const uint32_t *ci = relabel ? w->rl.in.col : g->in_col;
   const float *inv = relabel ? w->rl.inv : g->inv_outdeg;
@@ -763,7 +765,7 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   // the out-CSR the adjoint runs over: the renumbered copy or the graph's own
   if (relabel) OLV_TRY(ensure_side(g, w, false, true));
   else OLV_TRY(ensure_plan(g, false));
-  const SegDev d = make_seg_dev(relabel ? w->rl.out.plan : g->plan_out);
+  const SegDev d = make_seg_dev(relabel ? w->rl.out.plan : g->plan_out, g->tune);
   const uint32_t *rp = relabel ? w->rl.out.rp : g->row_ptr;
   const uint32_t *ci = relabel ? w->rl.out.col : g->col;
   const bool host = (flags & OLV_PTR_HOST) != 0;
@@ -825,16 +827,10 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
     // chunked hub rows write their edges inside the chunk blocks
     if (relabel) {
       GradOp<true> gop{ci, rv, lf, dgw, w->rl.out.eid, damping};
-      if (d.contig)
-        hipLaunchKernelGGL((seg_iter_kernel<GradOp<true>, true>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d,
-                           rp, gop, (IterCtl *)nullptr, 0u);
-      else
-        hipLaunchKernelGGL((seg_iter_kernel<GradOp<true>, false>), dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d,
-                           rp, gop, (IterCtl *)nullptr, 0u);
+      launch_seg(d, rp, gop, (IterCtl *)nullptr, 0u);
     } else {
       GradOp<false> gop{ci, rv, lf, dgw, nullptr, damping};
-      hipLaunchKernelGGL(seg_iter_kernel<GradOp<false>>, dim3(seg_grid(d)), dim3(kSegBlock), 0, stream(), d, rp, gop,
-                         (IterCtl *)nullptr, 0u);
+      launch_seg(d, rp, gop, (IterCtl *)nullptr, 0u);
     }
   }
   if (dgt) {

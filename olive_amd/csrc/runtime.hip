@@ -294,16 +294,20 @@ int build_seg_plan(const olv_graph *g, bool in_edges, SegPlan &plan) {
   if (!d_rp) return fail(OLV_EINVAL, "segment plan needs the %s CSR", in_edges ? "in-edge" : "out-edge");
   std::vector<uint32_t> rp((size_t)n + 1);
   OLV_HIP(hipMemcpy(rp.data(), d_rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
-  return build_seg_plan_rp(rp, plan);
+  return build_seg_plan_rp(rp, g->tune.seg_bounds, plan);
 }
 
-int build_seg_plan_rp(const std::vector<uint32_t> &rp, SegPlan &plan) {
+bool seg_plan_stale(const SegPlan &plan, const Tuning &t) {
+  return plan.bounds[0] != t.seg_bounds[0] || plan.bounds[1] != t.seg_bounds[1] || plan.bounds[2] != t.seg_bounds[2];
+}
+
+int build_seg_plan_rp(const std::vector<uint32_t> &rp, const uint32_t *bounds, SegPlan &plan) {
   plan.release();
   const uint32_t n = (uint32_t)(rp.size() - 1);
   // lanes per vertex: 1 (deg<=b0), 8 (<=b1), 64 (<=b2), workgroup (<=chunk), chunked hubs
-  plan.bounds[0] = 8;
-  plan.bounds[1] = 96;
-  plan.bounds[2] = 1024;
+  plan.bounds[0] = bounds[0];
+  plan.bounds[1] = bounds[1];
+  plan.bounds[2] = bounds[2];
   plan.chunk = 8192;
   plan.bounds[3] = plan.chunk;
   std::vector<uint32_t> lists[SegPlan::kBuckets];

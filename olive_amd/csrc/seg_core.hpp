@@ -38,6 +38,8 @@ struct SegDev {
   uint32_t live_end;
   bool contig;                          // list[i] = i + shift[bucket]
   uint32_t shift[SegPlan::kBuckets];
+  uint32_t grid_per_cu;                 // workgroups per CU of a launch (host side)
+  uint32_t unroll;                      // gathers in flight per lane: 4, 8 or 16 (host side)
   // Vertex at list position idx of bucket b (Contig: without the load).
   template <bool Contig>
   __device__ __forceinline__ uint32_t vertex(uint32_t idx, int b) const {
@@ -49,8 +51,10 @@ constexpr int kSegBlock = 256;
 // s_waitcnt operand (gfx9 encoding): vmcnt = 0, expcnt and lgkmcnt unbounded.
 constexpr int kWaitVmcnt0 = (0x7 << 4) | (0xF << 8);
 
-inline SegDev make_seg_dev(const SegPlan &p) {
+inline SegDev make_seg_dev(const SegPlan &p, const Tuning &t) {
   SegDev d;
+  d.grid_per_cu = t.seg_grid_per_cu;
+  d.unroll = t.seg_unroll;
   d.list = p.list;
   d.hub_vertex = p.hub_vertex;
   d.hub_first = p.hub_first;
@@ -78,20 +82,12 @@ inline SegDev make_seg_dev(const SegPlan &p) {
   return d;
 }
 
-// Workgroups per CU of a segment launch (0: one workgroup per virtual
-// block); never more workgroups than there are virtual blocks.
-#ifndef OLV_SEG_GRID_PER_CU
-#define OLV_SEG_GRID_PER_CU 32
-#endif
-// Independent gathers in flight per lane.
-#ifndef OLV_SEG_UNROLL
-#define OLV_SEG_UNROLL 8
-#endif
-
+// Workgroups of a segment launch: d.grid_per_cu per CU (0: one workgroup
+// per virtual block), never more than there are virtual blocks.
 inline uint32_t seg_grid(const SegDev &d) {
   const uint32_t nvb = d.role_start[5] ? d.role_start[5] : 1;
-  if (OLV_SEG_GRID_PER_CU == 0) return nvb;
-  const uint32_t cap = (uint32_t)cu_count() * OLV_SEG_GRID_PER_CU;
+  if (d.grid_per_cu == 0) return nvb;
+  const uint32_t cap = (uint32_t)cu_count() * d.grid_per_cu;
   return nvb < cap ? nvb : cap;
 }
 
@@ -103,9 +99,8 @@ __device__ __forceinline__ float group_sum(float v) {
 }
 
 // Sum of op.val(v, k, op.idx(k)) for k = beg + j, beg + j + G, ... < end.
-template <int G, class Op>
+template <int G, int U, class Op>
 __device__ __forceinline__ float lane_sum(const Op &op, uint32_t v, uint32_t beg, uint32_t end, uint32_t j) {
-  constexpr int U = OLV_SEG_UNROLL;
   float acc[U];
 #pragma unroll
   for (int i = 0; i < U; ++i) acc[i] = 0.f;
@@ -176,7 +171,7 @@ __device__ __forceinline__ double err_total(const IterCtl *ctl, uint32_t slot) {
 #define OLV_SEG_WAVES_PER_EU 8
 #endif
 
-template <class Op, bool Contig = false>
+template <class Op, bool Contig, int U>
 __global__ void __launch_bounds__(kSegBlock) __attribute__((amdgpu_waves_per_eu(OLV_SEG_WAVES_PER_EU)))
 seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, uint32_t k) {
   if (ctl) {
@@ -213,7 +208,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       const uint32_t c0 = d.hub_first[h], c1 = d.hub_first[h + 1];
       const uint32_t beg = rp[v] + (c - c0) * d.chunk;
       const uint32_t end = min(rp[v + 1], beg + d.chunk);
-      float s = lane_sum<kSegBlock>(op, v, beg, end, threadIdx.x);
+      float s = lane_sum<kSegBlock, U>(op, v, beg, end, threadIdx.x);
       s = block_sum<kSegBlock, float>(s, s_red);
       if (threadIdx.x == 0) {
         // The partials are device-coherent atomic stores and loads, so the
@@ -238,7 +233,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       }
     } else if (b < d.role_start[2]) {
       const uint32_t v = d.template vertex<Contig>(d.off[3] + (b - d.role_start[1]), 3);
-      float s = lane_sum<kSegBlock>(op, v, rp[v], rp[v + 1], threadIdx.x);
+      float s = lane_sum<kSegBlock, U>(op, v, rp[v], rp[v + 1], threadIdx.x);
       s = block_sum<kSegBlock, float>(s, s_red);
       if (threadIdx.x == 0) op.finish(v, s, err);
     } else if (b < d.role_start[3]) {
@@ -246,7 +241,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       const uint32_t j = threadIdx.x % 64;
       if (idx < d.off[3]) {
         const uint32_t v = d.template vertex<Contig>(idx, 2);
-        float s = lane_sum<64>(op, v, rp[v], rp[v + 1], j);
+        float s = lane_sum<64, U>(op, v, rp[v], rp[v + 1], j);
         s = group_sum<64>(s);
         if (j == 0) op.finish(v, s, err);
       }
@@ -258,7 +253,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       uint32_t v = 0;
       if (ok) {
         v = d.template vertex<Contig>(idx, 1);
-        s = lane_sum<8>(op, v, rp[v], rp[v + 1], j);
+        s = lane_sum<8, U>(op, v, rp[v], rp[v + 1], j);
       }
       s = group_sum<8>(s);
       if (ok && j == 0) op.finish(v, s, err);
@@ -266,7 +261,7 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
       const uint32_t idx = d.off[0] + (b - d.role_start[4]) * kSegBlock + threadIdx.x;
       if (idx < d.off[1]) {
         const uint32_t v = d.template vertex<Contig>(idx, 0);
-        const float s = lane_sum<1>(op, v, rp[v], rp[v + 1], 0u);
+        const float s = lane_sum<1, U>(op, v, rp[v], rp[v + 1], 0u);
         op.finish(v, s, err);
       }
     }
@@ -275,6 +270,24 @@ seg_iter_kernel(SegDev d, const uint32_t *__restrict__ rp, Op op, IterCtl *ctl, 
   err = block_sum<kSegBlock, float>(err, s_red);
   if (threadIdx.x == 0 && err != 0.f)
     atomicAdd(&ctl->errs[k % 3][blockIdx.x % kErrSlots].v, (double)err);
+}
+
+// One launch of seg_iter_kernel with the descriptor's unroll depth and list
+// form (on the library stream).
+template <class Op>
+inline void launch_seg(const SegDev &d, const uint32_t *rp, const Op &op, IterCtl *ctl, uint32_t k) {
+  const dim3 grid(seg_grid(d)), block(kSegBlock);
+#define OLV_SEG_LAUNCH(C, U) hipLaunchKernelGGL((seg_iter_kernel<Op, C, U>), grid, block, 0, stream(), d, rp, op, ctl, k)
+  if (d.contig) {
+    if (d.unroll == 4) OLV_SEG_LAUNCH(true, 4);
+    else if (d.unroll == 16) OLV_SEG_LAUNCH(true, 16);
+    else OLV_SEG_LAUNCH(true, 8);
+  } else {
+    if (d.unroll == 4) OLV_SEG_LAUNCH(false, 4);
+    else if (d.unroll == 16) OLV_SEG_LAUNCH(false, 16);
+    else OLV_SEG_LAUNCH(false, 8);
+  }
+#undef OLV_SEG_LAUNCH
 }
 
 }  // namespace olv

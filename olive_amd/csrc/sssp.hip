@@ -139,7 +139,8 @@ __device__ __forceinline__ void sssp_round(const SsspParams &p, uint32_t qsize, 
                                            int32_t split, uint32_t vb, uint32_t nvb, uint32_t &nf,
                                            unsigned long long &insp) {
   SsspVisitor<Multi> vis(p.col, p.w, p.dist, nextb, p.far_bm, split, p.n);
-  push_expand<kSBlock>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, vb, nvb, vis, insp);
+  push_expand<kSBlock>(p.queue, qsize, p.hubs, nhubs, p.row_ptr, p.wave_thr, p.block_thr, p.hub_thr, vb, nvb, vis,
+                       insp);
   nf += vis.nf;
 }
 
@@ -363,11 +364,6 @@ static int weight_stats(const int32_t *w, uint32_t m, int32_t *wmin, double *wme
   return OLV_OK;
 }
 
-// Workgroups per CU of the relax kernel (uniform 2^22 x 2^25 graph: 6 gives
-// 1.99 ms; 4: 2.00, 8: 2.03, 12: 2.01, 16: 2.05).
-#ifndef OLV_SSSP_RELAX_PER_CU
-#define OLV_SSSP_RELAX_PER_CU 6
-#endif
 #ifndef OLV_SSSP_DELTA_NUM
 #define OLV_SSSP_DELTA_NUM 32
 #endif
@@ -458,7 +454,9 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
     OLV_HIP(hipMemcpy(&hb, w.bar, sizeof(hb), hipMemcpyDeviceToHost));
     if (hb.err || !w.h_ctl->done) return fail(OLV_EINTERNAL, "olv_sssp: grid barrier timed out (co-residency lost)");
   } else {
-    const unsigned gr = (unsigned)cu_count() * OLV_SSSP_RELAX_PER_CU;
+    // relax workgroups per CU: the graph's tuning (uniform 2^22 x 2^25 graph:
+    // 6 gives 1.99 ms; 4: 2.00, 8: 2.03, 12: 2.01, 16: 2.05)
+    const unsigned gr = (unsigned)cu_count() * g->tune.sssp_relax_per_cu;
     const unsigned gc = sgrid(w.nw, kSBlock);
     // first chunk: the longest of the last few runs on this graph (8 at
     // first); longer runs continue in doubling chunks after a host check
