@@ -185,23 +185,36 @@ def bench_pagerank(args, olv):
     ref, _, _ = oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0)
     got = out.numpy().copy()
     err = float(np.max(np.abs(got.astype(np.float64) - ref) / ref))
-    # the same iteration in the graph's own vertex order (no relabelling):
-    # per-launch time for comparison, and the ranks must be bit-identical
-    gms, gl = 0.0, 0
+    # the per-row pull in the graph's own vertex order and over the
+    # renumbered CSR: per-launch times for comparison (the same ranks up to
+    # the summation order)
+    gms, gl, rms, rl = 0.0, 0, 0.0, 0
     if not args.no_modes:
         for _ in range(3):
             olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c,
                                               olv._lib.OLV_PROFILE | olv._lib.OLV_GRAPH_ORDER, ctypes.byref(st)))
             gms += st.kernel_ms
             gl += st.kernel_launches
+        g_order = out.numpy().copy()
+        for _ in range(3):
+            olv._lib.check(L.olv_pagerank_f32(g.h, 0.85, iters, 0.0, out.c,
+                                              olv._lib.OLV_PROFILE | olv._lib.OLV_PR_ROW_PULL, ctypes.byref(st)))
+            rms += st.kernel_ms
+            rl += st.kernel_launches
     cfg = {"workload": "pagerank_rmat20", "vertices": n, "edges": m, "iterations": iters, "damping": 0.85,
            "rmat": [0.57, 0.19, 0.19, 0.05], "seed": 1, "max_rel_err_vs_fp64": err,
            "first_call_ms": t_first * 1e3}
     if gl:
         cfg["graph_order_launch_us"] = gms / gl * 1e3
-        cfg["graph_order_bit_identical"] = bool(np.array_equal(out.numpy().view(np.uint32), got.view(np.uint32)))
+        cfg["row_pull_launch_us"] = rms / rl * 1e3
+        cfg["row_pull_bit_identical_to_graph_order"] = bool(np.array_equal(out.numpy().view(np.uint32),
+                                                                           g_order.view(np.uint32)))
+        cfg["max_rel_diff_vs_row_pull"] = float(np.max(np.abs(got.astype(np.float64) - out.numpy()) /
+                                                       out.numpy().astype(np.float64)))
     res = base_line(args, cfg, m * iters * args.steps / dt, "edges/s", dt / args.steps * 1e3, "f32")
-    res["roofline"] = roofline(8 * m + 12 * n, kms / kl / 1e3, "pagerank_rmat20", "seg_iter_kernel<PrOp>")
+    # one iteration = the segment pass and the combine (hipEvents around both)
+    res["roofline"] = roofline(8 * m + 12 * n, kms / kl / 1e3, "pagerank_rmat20",
+                               "blk_segment_kernel + blk_combine_kernel (one iteration)")
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         t, runs = timed_oracle(lambda: oracle.pagerank(csr.row_ptr, csr.col, iters, epsilon=0.0), 20.0)

@@ -124,17 +124,15 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   }
 
   // 3. entry arrays in piece order
-  std::vector<uint32_t> &pbeg = h.pbeg, &pslot = h.pslot, &lslot = h.lslot;
+  std::vector<uint32_t> &pbeg = h.pbeg, &lslot = h.lslot;
   std::vector<uint16_t> &lidx = h.lidx;
   pbeg.assign(npieces + 1, 0);
-  pslot.assign(npieces, 0);
   lidx.assign(m, 0);
   lslot.assign(with_slots ? m : 0, 0);
   uint32_t o = 0;
   for (size_t q = 0; q < npieces; ++q) {
     const Piece &pc = order[q];
     pbeg[q] = o;
-    pslot[q] = pc.slot;
     for (uint32_t t = 0; t < pc.len; ++t) {
       const uint32_t k = perm[pc.start + t];
       lidx[o + t] = (uint16_t)(col[k] & (kSegSize - 1));
@@ -143,6 +141,10 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     o += pc.len;
   }
   pbeg[npieces] = o;
+  // each row's partials in segment order: cmb_pos[cmb_rp[v] + j] is the
+  // piece (and partial) index of the row's j-th piece
+  h.cmb_pos.assign(npieces, 0);
+  for (size_t q = 0; q < npieces; ++q) h.cmb_pos[order[q].slot] = (uint32_t)q;
 
   // 4. contiguous unit ranges of about equal cost per workgroup; a sentinel
   //    unit closes the list (its first piece and entry are the totals)
@@ -186,10 +188,10 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   OLV_TRY(upload_vec(&bp.units, h.units));
   OLV_TRY(upload_vec(&bp.wg_unit, h.wg_unit));
   OLV_TRY(upload_vec(&bp.pbeg, h.pbeg));
-  OLV_TRY(upload_vec(&bp.pslot, h.pslot));
   OLV_TRY(upload_vec(&bp.lidx, h.lidx));
   if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));
   OLV_TRY(upload_vec(&bp.cmb_rp, h.cmb_rp));
+  OLV_TRY(upload_vec(&bp.cmb_pos, h.cmb_pos));
   OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bp.partial), std::max<size_t>(h.npieces, 1) * sizeof(float)));
   bp.ready = true;
   return OLV_OK;

@@ -13,23 +13,23 @@
 //   segment pass  each workgroup owns a run of work units (a unit: at most
 //                 kUnitPieces pieces and kUnitEdges entries of one segment
 //                 and one length bucket).  Per unit it stages, with
-//                 coalesced loads, the unit's entries, piece offsets and
-//                 piece slots in LDS (and the segment of the source vector,
-//                 when the segment changes), then sums every piece from LDS
-//                 alone -- 1, 8 or 64 lanes per piece by length -- and
-//                 stores one partial per piece at the piece's slot;
-//   combine       per destination, its partials (slots grouped by
-//                 destination, in segment order) are added in a fixed order
-//                 -- by one lane, or by a wave for the rows with more than
-//                 kCombineLane partials -- and the vertex update runs.
+//                 coalesced loads, the unit's entries and piece offsets in
+//                 LDS (and the segment of the source vector, when the
+//                 segment changes), then sums every piece from LDS alone --
+//                 1, 8 or 64 lanes per piece by length -- and stores one
+//                 partial per piece, in piece order (coalesced stores);
+//   combine       per destination, its partials (listed by cmb_pos in
+//                 segment order) are added in a fixed order -- by one lane,
+//                 or by a wave for the rows with more than kCombineLane
+//                 partials -- and the vertex update runs.
 //
 // A piece's terms are summed in a fixed order and the partials in segment
 // order, so an iteration is bit-reproducible from run to run.  With the
 // vertices numbered by descending in-degree (the renumbered CSRs of the
 // PageRank solver) a skewed graph's in-edges fall into few pieces: RMAT-20
 // has 16.8 M in-edges in 2.3 M pieces, most of them in segment 0.  Per
-// iteration the pass reads 2 B per edge and 12 B per piece from HBM, and
-// every gather is an LDS read.
+// iteration the pass reads 2 B per edge and 4 B per piece and writes 4 B per
+// piece, and every gather is an LDS read.
 #pragma once
 #include <vector>
 
@@ -56,18 +56,18 @@ struct BlockedPlan {
                                  // first entry); the last is a sentinel (-, -, npieces, m)
   uint32_t *wg_unit = nullptr;   // ngroups + 1: unit range of each segment-pass workgroup
   uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
-  uint32_t *pslot = nullptr;     // npieces: partial slot of each piece
   uint16_t *lidx = nullptr;      // m: source offset within the segment, in piece order
   uint32_t *lslot = nullptr;     // m (weighted solves only): CSR slot each entry came from
-  uint32_t *cmb_rp = nullptr;    // n + 1: partial slots of each row
+  uint32_t *cmb_rp = nullptr;    // n + 1: each row's range in cmb_pos
+  uint32_t *cmb_pos = nullptr;   // npieces: partial index of each row's pieces, in segment order
   float *partial = nullptr;      // npieces
   bool ready = false;
   void release() {
-    void *ps[] = {units, wg_unit, pbeg, pslot, lidx, lslot, cmb_rp, partial};
+    void *ps[] = {units, wg_unit, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     units = nullptr;
-    wg_unit = pbeg = pslot = lslot = cmb_rp = nullptr;
+    wg_unit = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
     lidx = nullptr;
     partial = nullptr;
     ready = false;
@@ -78,7 +78,7 @@ struct BlockedPlan {
 struct BlockedHost {
   uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
   std::vector<uint4> units;
-  std::vector<uint32_t> wg_unit, pbeg, pslot, lslot, cmb_rp;
+  std::vector<uint32_t> wg_unit, pbeg, lslot, cmb_rp, cmb_pos;
   std::vector<uint16_t> lidx;
 };
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
@@ -96,15 +96,15 @@ struct BlkDev {
   const uint4 *units;
   const uint32_t *wg_unit;
   const uint32_t *pbeg;
-  const uint32_t *pslot;
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
+  const uint32_t *cmb_pos;
   float *partial;
   uint32_t n, n_live, n_heavy;
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p) {
-  return BlkDev{p.units, p.wg_unit, p.pbeg, p.pslot, p.lidx, p.cmb_rp, p.partial, p.n, p.n_live, p.n_heavy};
+  return BlkDev{p.units, p.wg_unit, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
 }
 
 // Sum of sv(lidx[k]) for k = beg + j, beg + j + G, ... < end (lidx and the
@@ -151,7 +151,6 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   __shared__ float slice[kSegSize];
   __shared__ uint16_t s_lidx[kUnitEdges];
   __shared__ uint32_t s_pbeg[kUnitPieces + 1];
-  __shared__ uint32_t s_pslot[kUnitPieces];
   if (ctl) {
     if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     if (k > 0) {
@@ -177,7 +176,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   uint4 nu = make_uint4(0, 0, 0, 0);
   uint32_t n_np = 0, n_ne = 0;
   uint16_t li[kLi];
-  uint32_t pb[kPi + 1], ps[kPi];
+  uint32_t pb[kPi + 1];
   auto prefetch = [&](uint32_t u) {
     nu = d.units[u];
     const uint4 nx = d.units[u + 1];
@@ -193,18 +192,13 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       const uint32_t i = tid + j * kBlkThreads;
       pb[j] = i <= n_np ? d.pbeg[nu.z + i] : 0u;
     }
-#pragma unroll
-    for (uint32_t j = 0; j < kPi; ++j) {
-      const uint32_t i = tid + j * kBlkThreads;
-      ps[j] = i < n_np ? d.pslot[nu.z + i] : 0u;
-    }
   };
   uint32_t cur = 0xffffffffu;
   const uint32_t u0 = d.wg_unit[blockIdx.x], u1 = d.wg_unit[blockIdx.x + 1];
   if (u0 < u1) prefetch(u0);
   for (uint32_t u = u0; u < u1; ++u) {
     const uint4 un = nu;
-    const uint32_t np = n_np, ne = n_ne, e0 = un.w;
+    const uint32_t np = n_np, ne = n_ne, e0 = un.w, q0 = un.z;
     __syncthreads();  // the previous unit is done with the staged arrays
     if (un.x != cur) {
       const uint32_t base = un.x << kSegBits;
@@ -237,30 +231,25 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       const uint32_t i = tid + j * kBlkThreads;
       if (i <= np) s_pbeg[i] = pb[j] - e0;
     }
-#pragma unroll
-    for (uint32_t j = 0; j < kPi; ++j) {
-      const uint32_t i = tid + j * kBlkThreads;
-      if (i < np) s_pslot[i] = ps[j];
-    }
     if (u + 1 < u1) prefetch(u + 1);
     __syncthreads();
     sv.bind(slice, e0);
     if (un.y == 0) {  // one lane per piece (up to 8 entries)
       for (uint32_t q = tid; q < np; q += kBlkThreads)
-        d.partial[s_pslot[q]] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
+        d.partial[q0 + q] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
     } else if (un.y == 1) {  // 8 lanes per piece (up to 64)
       const uint32_t j = tid % 8;
       for (uint32_t q = tid / 8; q < np; q += kBlkThreads / 8) {
         float s = piece_sum<8, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
         s = blk_group_sum<8>(s);
-        if (j == 0) d.partial[s_pslot[q]] = s;
+        if (j == 0) d.partial[q0 + q] = s;
       }
     } else {  // one wave per piece (up to kPieceMax)
       const uint32_t j = tid % kWave;
       for (uint32_t q = tid / kWave; q < np; q += kBlkThreads / kWave) {
         float s = piece_sum<kWave, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
         s = wave_sum(s);
-        if (j == 0) d.partial[s_pslot[q]] = s;
+        if (j == 0) d.partial[q0 + q] = s;
       }
     }
   }
@@ -269,30 +258,38 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
 // Combine of iteration k: row v's partials added in a fixed order, then
 // fin.finish(v, sum, err); the residual goes to errs[k % 3] (ctl != nullptr).
 // Rows [0, n_heavy) go a wave each (lane j adds partials j, j + 64, ...,
-// then a shuffle tree); the others a lane each, in slot order.  Rows
-// without entries (v >= n_live) get a constant update: iterations 0 and 1
-// write it into both ping-pong buffers, later iterations skip them.
+// then a shuffle tree), in the first `heavy_blocks` workgroups; the others a
+// lane each, in segment order, in the remaining workgroups.  Rows without
+// entries (v >= n_live) get a constant update: iterations 0 and 1 write it
+// into both ping-pong buffers, later iterations skip them.
 template <class Fin>
-__global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, IterCtl *ctl, uint32_t k) {
+__global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, IterCtl *ctl, uint32_t k,
+                                                           uint32_t heavy_blocks) {
   if (ctl && __hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   __shared__ float s_red[256 / kWave];
   const uint32_t nv = (!ctl || k >= 2) ? d.n_live : d.n;
   float err = 0.f;
   const uint32_t lane = threadIdx.x % kWave;
-  for (uint32_t v = (blockIdx.x * 256 + threadIdx.x) / kWave; v < d.n_heavy; v += gridDim.x * (256 / kWave)) {
-    const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
-    float s = 0.f;
-    for (uint32_t q = a + lane; q < b; q += kWave) s += d.partial[q];
-    s = wave_sum(s);
-    if (lane == 0) fin.finish(v, s, err);
+  if (blockIdx.x < heavy_blocks) {
+    for (uint32_t v = (blockIdx.x * 256 + threadIdx.x) / kWave; v < d.n_heavy; v += heavy_blocks * (256 / kWave)) {
+      const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+      float s = 0.f;
+      for (uint32_t q = a + lane; q < b; q += kWave) s += d.partial[d.cmb_pos[q]];
+      s = wave_sum(s);
+      if (lane == 0) fin.finish(v, s, err);
+    }
   }
-  for (uint32_t v = d.n_heavy + blockIdx.x * 256 + threadIdx.x; v < nv; v += gridDim.x * 256) {
+  const uint32_t lb = blockIdx.x - heavy_blocks, nlb = gridDim.x - heavy_blocks;
+  for (uint32_t v = d.n_heavy + lb * 256 + threadIdx.x; blockIdx.x >= heavy_blocks && v < nv; v += nlb * 256) {
     const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
     float s = 0.f;
-    for (uint32_t q = a; q < b; q += 8) {  // eight loads in flight, added in slot order
+    for (uint32_t q = a; q < b; q += 8) {  // eight loads in flight, added in segment order
+      uint32_t ix[8];
+#pragma unroll
+      for (uint32_t i = 0; i < 8; ++i) ix[i] = q + i < b ? d.cmb_pos[q + i] : 0u;
       float p[8];
 #pragma unroll
-      for (uint32_t i = 0; i < 8; ++i) p[i] = q + i < b ? d.partial[q + i] : 0.f;
+      for (uint32_t i = 0; i < 8; ++i) p[i] = q + i < b ? d.partial[ix[i]] : 0.f;
 #pragma unroll
       for (uint32_t i = 0; i < 8; ++i)
         if (q + i < b) s += p[i];

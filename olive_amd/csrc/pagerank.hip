@@ -588,14 +588,17 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (blocked) {
     const BlockedPlan &bp = w->rl.blk_in;
     const BlkDev bd = make_blk_dev(bp);
-    const unsigned gc = ew_grid(g->n);
+    // combine: a wave per heavy row in the first workgroups, then a lane
+    // per row
+    const unsigned gh = std::min<unsigned>((bp.n_heavy + 3) / 4, (unsigned)cu_count() * 4);
+    const unsigned gc = gh + ew_grid(g->n);
     for (uint32_t it = 0; it < max_iters; ++it) {
       hipEvent_t a = prof.next(), b = prof.next();
       if (a) OLV_HIP(hipEventRecord(a, stream()));
       hipLaunchKernelGGL(blk_segment_kernel<SliceVal>, dim3(bp.ngroups), dim3(kBlkThreads), 0, stream(), bd, SliceVal{},
                          (const float *)w->buf[it & 1], w->ctl, it);
       hipLaunchKernelGGL(blk_combine_kernel<PrFin>, dim3(gc), dim3(256), 0, stream(), bd,
-                         PrFin{w->buf[(it + 1) & 1], ri, inv, damping, base}, w->ctl, it);
+                         PrFin{w->buf[(it + 1) & 1], ri, inv, damping, base}, w->ctl, it, gh);
       if (b) OLV_HIP(hipEventRecord(b, stream()));
     }
   } else {
