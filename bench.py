@@ -229,6 +229,7 @@ def bench_pagerank(args, olv):
         sec["grid128"] = grid128(olv, args)
         sec.update(batched_legs(olv, args, csr22, g22, sssp=False))
         del g22, csr22
+        sec["api_bfs"] = api_bfs_legs()
         sec["adjoint_rmat24"] = adjoint_leg(olv, args)
         res["secondary"] = sec
     return res
@@ -504,6 +505,27 @@ def bench_batched(args, olv):
     res = base_line(args, cfg, b["gteps"], "GTEPS", b["batched_ms"], "int32")
     res["detail"] = legs
     return res
+
+
+# ------------------------------------------------------------ Olive C++ API
+def api_bfs_legs():
+    """The reference BFS program's level loop (edgeFilter / vertexFilter)
+    through the Olive C++ API, beside olv_bfs on the same graph and sources:
+    olive_amd/_apps/olive_api_bench (built from olive_amd/examples), run as
+    a child process on RMAT-22 (8 sources) and the 128 x 128 grid."""
+    import subprocess
+    exe = os.path.join(ROOT, "olive_amd", "_apps", "olive_api_bench")
+    out = {}
+    for name, argv in (("rmat22", ["rmat", "22", "8"]), ("grid128", ["grid", "128", "128"])):
+        if not os.path.exists(exe):
+            out[name] = {"error": "olive_api_bench not built"}
+            continue
+        r = subprocess.run([exe] + argv, capture_output=True, text=True, timeout=300)
+        try:
+            out[name] = json.loads(r.stdout.strip().splitlines()[-1])
+        except (ValueError, IndexError):
+            out[name] = {"error": f"exit {r.returncode}", "stderr": r.stderr[-300:]}
+    return out
 
 
 # --------------------------------------------------------------------- grid

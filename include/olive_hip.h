@@ -142,6 +142,16 @@ int olv_queue_to_bitmap(const uint32_t *queue, const uint32_t *count_dev,
 int olv_frontier_bin(const olv_graph *g, const uint32_t *queue,
                      const uint32_t *count_dev, uint32_t *const bucket_queues[3],
                      uint32_t *bucket_counts_dev);
+/* Split queue[0..*count_dev) for the push kernels: entries with fewer than
+ * the graph's hub_thr out-edges go to out_queue (capacity n), every other
+ * entry becomes ceil(deg / hub_thr) hub items, (vertex, chunk) pairs of
+ * uint32 in hub_pairs (capacity olv_hub_capacity(m) pairs); counts_dev
+ * holds 2 uint32 (queue length, hub items; zeroed here).  Replaces the
+ * GroupSize lanes per vertex of edgeFilterKernel (oliverKernel.h:63-66). */
+int olv_frontier_split(const olv_graph *g, const uint32_t *queue, const uint32_t *count_dev,
+                       uint32_t *out_queue, uint32_t *counts_dev, uint32_t *hub_pairs);
+/* Hub items the split can produce for a graph of m edges. */
+size_t olv_hub_capacity(uint64_t m);
 /* Degree-bucket thresholds used by olv_frontier_bin for this graph. */
 int olv_graph_buckets(const olv_graph *g, uint32_t *thread_max,
                       uint32_t *wave_max);

@@ -11,6 +11,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "bfs_core.hpp"
 #include "device.hpp"
 #include "internal.hpp"
 
@@ -142,6 +143,39 @@ __global__ void frontier_bin_kernel(const uint32_t *__restrict__ queue, const ui
       basek = __shfl(basek, leader, kWave);
       if (b == k) qs[k][basek + lane_rank(mask)] = v;
     }
+  }
+}
+
+// Queue entries below hub_thr out-edges to out_q; every other entry becomes
+// ceil(deg / hub_thr) hub items (vertex, chunk).  One atomic per workgroup
+// and output per pass (a block-wide scan places the entries).
+__global__ void __launch_bounds__(256)
+    frontier_split_kernel(const uint32_t *__restrict__ queue, const uint32_t *__restrict__ count,
+                          const uint32_t *__restrict__ row_ptr, uint32_t hub_thr, uint32_t *__restrict__ out_q,
+                          uint32_t *__restrict__ out_count, uint2 *__restrict__ hubs, uint32_t *__restrict__ hub_count) {
+  __shared__ uint32_t s_scan[256 / kWave + 1];
+  __shared__ uint32_t s_base[2];
+  const uint32_t c = *count;
+  for (uint32_t base = blockIdx.x * 256; base < c; base += gridDim.x * 256) {
+    const uint32_t i = base + threadIdx.x;
+    uint32_t v = 0, deg = 0;
+    if (i < c) {
+      v = queue[i];
+      deg = row_ptr[v + 1] - row_ptr[v];
+    }
+    const bool light = i < c && deg < hub_thr;
+    const uint32_t nh = (i < c && !light) ? (deg + hub_thr - 1) / hub_thr : 0u;
+    uint32_t tl, th;
+    const uint32_t ol = block_exclusive_scan<256>(light ? 1u : 0u, s_scan, &tl);
+    const uint32_t oh = block_exclusive_scan<256>(nh, s_scan, &th);
+    if (threadIdx.x == 0) {
+      s_base[0] = tl ? atomicAdd(out_count, tl) : 0u;
+      s_base[1] = th ? atomicAdd(hub_count, th) : 0u;
+    }
+    __syncthreads();
+    if (light) out_q[s_base[0] + ol] = v;
+    for (uint32_t k = 0; k < nh; ++k) hubs[s_base[1] + oh + k] = make_uint2(v, k);
+    __syncthreads();
   }
 }
 
@@ -677,6 +711,21 @@ int olv_frontier_bin(const olv_graph *g, const uint32_t *queue, const uint32_t *
   OLV_HIP(hipGetLastError());
   return OLV_OK;
 }
+
+int olv_frontier_split(const olv_graph *g, const uint32_t *queue, const uint32_t *count_dev, uint32_t *out_queue,
+                       uint32_t *counts_dev, uint32_t *hub_pairs) {
+  uint2 *hubs = reinterpret_cast<uint2 *>(hub_pairs);
+  if (!g || !queue || !count_dev || !out_queue || !counts_dev || !hubs)
+    return fail(OLV_EINVAL, "olv_frontier_split: NULL pointer");
+  OLV_TRY(fill_u32(counts_dev, 0u, 2));
+  const unsigned grid = std::max(1u, std::min((g->n + 255) / 256, (unsigned)cu_count() * 4));
+  hipLaunchKernelGGL(frontier_split_kernel, dim3(grid), dim3(256), 0, g_stream, queue, count_dev, g->row_ptr,
+                     g->tune.hub_thr, out_queue, counts_dev, hubs, counts_dev + 1);
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
+size_t olv_hub_capacity(uint64_t m) { return hub_capacity(m); }
 
 int olv_graph_buckets(const olv_graph *g, uint32_t *thread_max, uint32_t *wave_max) {
   if (!g) return fail(OLV_EINVAL, "olv_graph_buckets: NULL graph");

@@ -193,6 +193,8 @@ class Olive {
       OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * (size_t)(vertexCount ? vertexCount : 1)));
       buckets[b] = reinterpret_cast<uint32_t *>(d);
     }
+    OLV_CHECK(olv_malloc(&d, 2 * sizeof(uint32_t) * olv_hub_capacity(edgeCount)));
+    hubPairs = reinterpret_cast<uint32_t *>(d);
     OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * 4));
     bucketCounts = reinterpret_cast<uint32_t *>(d);
     OLV_CHECK(olv_malloc(&d, sizeof(uint32_t) * (size_t)(vertexCount ? vertexCount : 1)));
@@ -264,6 +266,7 @@ class Olive {
     rowPtr = colIdx = NULL;
     buckets[0] = buckets[1] = buckets[2] = NULL;
     bucketCounts = mapQueue = mapCount = NULL;
+    hubPairs = NULL;
     reduceScratch = NULL;
     OLV_CHECK(olv_device_info(&cuCount, NULL, NULL));
   }
@@ -277,6 +280,8 @@ class Olive {
     if (mapQueue) olv_free(mapQueue);
     if (mapCount) olv_free(mapCount);
     if (reduceScratch) olv_free(reduceScratch);
+    if (hubPairs) olv_free(hubPairs);
+    hubPairs = NULL;
     buckets[0] = buckets[1] = buckets[2] = NULL;
     bucketCounts = mapQueue = mapCount = NULL;
     reduceScratch = NULL;
@@ -285,25 +290,21 @@ class Olive {
     edgeValues.del();
   }
 
-  // Bin queue[0..*count) by out-degree and push each bucket with its lane
-  // width.  Grids are sized from the capacity (the count stays on the
-  // device) and every kernel strides over the device-side count.
+  // Split queue[0..*count) into low-degree entries and hub items, then push
+  // both in one launch (olv::push_expand with the user's functors).  The
+  // grid is sized from the CU count (the counts stay on the device); the
+  // thresholds are the graph's tuning.
   template <typename F, bool MARK>
   void expand(const uint32_t *queue, const uint32_t *count, uint32_t *dstWords, F f) {
     if (!vertexCount) return;
-    uint32_t *const qs[3] = {buckets[0], buckets[1], buckets[2]};
-    OLV_CHECK(olv_frontier_bin(g, queue, count, qs, bucketCounts));
-    const unsigned full = (unsigned)cuCount * 8;
-    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 0, MARK>),
-                       dim3(grid(vertexCount)), dim3(olive_detail::kBlock), 0, stream(), buckets[0], bucketCounts + 0,
-                       rowPtr, colIdx, vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice,
-                       dstWords, f);
-    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 1, MARK>), dim3(full),
-                       dim3(olive_detail::kBlock), 0, stream(), buckets[1], bucketCounts + 1, rowPtr, colIdx,
-                       vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice, dstWords, f);
-    hipLaunchKernelGGL((olive_detail::expand_kernel<VertexValue, AccumValue, EdgeValue, F, 2, MARK>), dim3(full),
-                       dim3(olive_detail::kBlock), 0, stream(), buckets[2], bucketCounts + 2, rowPtr, colIdx,
-                       vertexValues.elemsDevice, accumulators.elemsDevice, edgeValues.elemsDevice, dstWords, f);
+    OLV_CHECK(olv_frontier_split(g, queue, count, buckets[0], bucketCounts, hubPairs));
+    olv_tuning t;
+    OLV_CHECK(olv_get_tuning_ex(g, &t));
+    hipLaunchKernelGGL((olive_detail::push_kernel<VertexValue, AccumValue, EdgeValue, F, MARK>),
+                       dim3((unsigned)cuCount * 8), dim3(olive_detail::kBlock), 0, stream(), buckets[0], bucketCounts,
+                       reinterpret_cast<const uint2 *>(hubPairs), rowPtr, colIdx, vertexValues.elemsDevice,
+                       accumulators.elemsDevice, edgeValues.elemsDevice, dstWords, f, t.td_wave_thr, t.td_block_thr,
+                       t.hub_thr);
     HIP_CHECK(hipGetLastError());
   }
 
@@ -317,6 +318,7 @@ class Olive {
   uint32_t *bucketCounts;
   uint32_t *mapQueue;
   uint32_t *mapCount;
+  uint32_t *hubPairs;   // hub items (vertex, chunk) of the push
   void *reduceScratch;  // partial sums of the device reductions + the result
 
   GRD<VertexValue> vertexValues;

@@ -4,11 +4,16 @@
 // bitmap -> queue compaction, fills, device memory -- is in libolive_hip.so.
 //
 // Replaces the reference's K1-K5 (oliverKernel.h:45-221):
-//  * K1/K3 (push expansion): a frontier is first split by out-degree into
-//    three buckets (olv_frontier_bin); each bucket runs with one lane, one
-//    wave64 or one 256-lane workgroup per vertex, grid-stride over a
-//    device-side count (the reference gives every vertex a fixed GroupSize
-//    of lanes, and K3 one thread per vertex however large its list).
+//  * K1/K3 (push expansion): the frontier queue is split into low-degree
+//    entries and hub items of hub_thr edges (olv_frontier_split), and one
+//    launch of the engine's push (olv::push_expand) runs the user's
+//    gather/reduce: hub items a workgroup each, the rest by degree bucket --
+//    a workgroup or a wave per vertex, and the short lists dealt to
+//    consecutive lanes by a block-wide scan -- with four edge visits in
+//    flight per lane; destination bits are set with a read first and a
+//    non-returning atomic (the reference gives every vertex a fixed
+//    GroupSize of lanes, and K3 one thread per vertex however large its
+//    list).
 //  * K2 (vertexFilter): one lane per 32-vertex bitmap word applies cond /
 //    update and the kept ids are compacted into the queue with a wave64
 //    inclusive scan, a workgroup prefix over the waves and one global atomic
@@ -22,6 +27,9 @@
 
 #include "bitmap.h"
 #include "common.h"
+// the engine's workload-balanced push (olv::push_expand), shared with the
+// library's own BFS and SSSP kernels
+#include "../csrc/bfs_core.hpp"
 
 namespace olive_detail {
 
@@ -45,48 +53,57 @@ __device__ inline E &edge_ref(E *ev, EdgeId e) {
     return ev[e];
 }
 
-// Push over one degree bucket.  MODE 0: one lane per vertex, 1: one wave,
-// 2: one workgroup.  MARK: set the destination's bit in dst_words.
-template <class V, class A, class E, class F, int MODE, bool MARK>
-__global__ void __launch_bounds__(kBlock)
-    expand_kernel(const uint32_t *bucket, const uint32_t *bucket_count, const EdgeId *row_ptr, const VertexId *col,
-                  V *vals, A *accs, E *evals, uint32_t *dst_words, F f) {
-  const uint32_t cnt = *bucket_count;
-  if (MODE == 0) {
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
-      const VertexId u = bucket[i];
-      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
-      const V sv = vals[u];
-      for (EdgeId e = beg; e < end; ++e) {
-        const VertexId v = col[e];
-        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
-        if (MARK) Bitmap::set(dst_words, v);
-      }
-    }
-  } else if (MODE == 1) {
-    const uint32_t waves = gridDim.x * (kBlock / kWave);
-    for (uint32_t i = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < cnt; i += waves) {
-      const VertexId u = bucket[i];
-      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
-      const V sv = vals[u];
-      for (EdgeId e = beg + lane(); e < end; e += kWave) {
-        const VertexId v = col[e];
-        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
-        if (MARK) Bitmap::set(dst_words, v);
-      }
-    }
-  } else {
-    for (uint32_t i = blockIdx.x; i < cnt; i += gridDim.x) {
-      const VertexId u = bucket[i];
-      const EdgeId beg = row_ptr[u], end = row_ptr[u + 1];
-      const V sv = vals[u];
-      for (EdgeId e = beg + threadIdx.x; e < end; e += kBlock) {
-        const VertexId v = col[e];
-        f.reduce(accs[v], f.gather(sv, end - beg, edge_ref(evals, e)));
-        if (MARK) Bitmap::set(dst_words, v);
-      }
-    }
+// Sets bit v of a flag bitmap (no result: a non-returning atomic, skipped
+// when the bit is already set).
+__device__ inline void mark(uint32_t *w, VertexId v) {
+  const uint32_t bit = 1u << (v & 31);
+  if (!(w[v >> 5] & bit)) __hip_atomic_fetch_or(&w[v >> 5], bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Visitor of olv::push_expand for a user functor: the payload of a source u
+// is (out-degree << 32 | u); an edge's target carries the gathered value,
+// which apply() reduces into the accumulator (and marks the destination).
+template <class V, class A, class E, class F, bool MARK>
+struct PushVisitor {
+  const EdgeId *row_ptr;
+  const VertexId *col;
+  V *vals;
+  A *accs;
+  E *evals;
+  uint32_t *dst_words;
+  F f;
+  struct Tgt {
+    VertexId v;
+    A a;
+  };
+  __device__ inline uint32_t row(uint32_t q) const { return q; }
+  __device__ inline unsigned long long load(uint32_t u) const {
+    return ((unsigned long long)(row_ptr[u + 1] - row_ptr[u]) << 32) | u;
   }
+  __device__ inline Tgt tgt(uint32_t e, unsigned long long pl) {
+    const uint32_t u = (uint32_t)pl;
+    return Tgt{col[e], f.gather(vals[u], (EdgeId)(pl >> 32), edge_ref(evals, e))};
+  }
+  __device__ inline int state(const Tgt &) const { return 0; }
+  __device__ inline void apply(const Tgt &t, int, unsigned long long) {
+    f.reduce(accs[t.v], t.a);
+    if (MARK) mark(dst_words, t.v);
+  }
+};
+
+// Push of a frontier split into a queue of low-degree vertices and hub
+// items (olv_frontier_split; counts[0] queue entries, counts[1] items):
+// hub items a workgroup each, the queue by the degree-bucketed,
+// scan-balanced olv::push_expand with four edge visits in flight per lane.
+template <class V, class A, class E, class F, bool MARK>
+__global__ void __launch_bounds__(kBlock)
+    push_kernel(const uint32_t *queue, const uint32_t *counts, const uint2 *hubs, const EdgeId *row_ptr,
+                const VertexId *col, V *vals, A *accs, E *evals, uint32_t *dst_words, F f, uint32_t wave_thr,
+                uint32_t block_thr, uint32_t hub_chunk) {
+  PushVisitor<V, A, E, F, MARK> vis{row_ptr, col, vals, accs, evals, dst_words, f};
+  unsigned long long insp = 0;
+  olv::push_expand<kBlock>(queue, counts[0], hubs, counts[1], row_ptr, wave_thr, block_thr, hub_chunk, blockIdx.x,
+                           gridDim.x, vis, insp);
 }
 
 // Exclusive prefix of `x` over the workgroup; *total gets the sum.
