@@ -71,6 +71,13 @@ enum {
                                         the solver's slot order (ignored for
                                         host pointers or when no such copy is
                                         kept)                                    */
+#define OLV_SSSP_DIRECT      0x0800u /* olv_sssp, olv_sssp_batched: relax with a
+                                        dist probe and device-scope atomics per
+                                        edge instead of binning the candidates
+                                        by destination partition (kept to
+                                        compare; also taken when the binned
+                                        scheme does not fit: more than 2^25
+                                        virtual vertices or 2^32 bin entries) */
 
 /* weight dtypes for olv_graph_create */
 enum { OLV_W_NONE = 0, OLV_W_I32 = 1, OLV_W_F32 = 2 };
@@ -266,7 +273,11 @@ int olv_bfs_noalloc(const olv_graph *g, uint32_t src, int32_t *levels,
 
 /* SSSP distances over int32 weights (graph weights, or `weights` if not
  * NULL), INF = 0x7fffffff.  Frontier relaxation as SSSP.cu:146-163 with
- * cond(v, acc[v]) (SSSP.cu:72-75). */
+ * cond(v, acc[v]) (SSSP.cu:72-75), run to the same fixed point in near-far
+ * order; with non-negative weights each round's candidates are binned by
+ * destination partition and min-reduced in LDS (OLV_SSSP_DIRECT: per-edge
+ * probe and atomics instead).  The first binned call on a graph keeps its
+ * bins (k x m pairs of 8 bytes) on the graph until olv_graph_destroy. */
 int olv_sssp(const olv_graph *g, uint32_t src, const int32_t *weights,
              int32_t *dist, unsigned flags, olv_stats *st);
 

@@ -28,6 +28,7 @@ OLV_GRAPH_ORDER = 0x40
 OLV_BUILD_TRANSPOSE = 0x100
 OLV_W_UNCHANGED = 0x200
 OLV_PR_ROW_PULL = 0x400
+OLV_SSSP_DIRECT = 0x800
 OLV_TUNE_BFS, OLV_TUNE_BATCHED, OLV_TUNE_PULL, OLV_TUNE_SSSP, OLV_TUNE_ALL = 0x1, 0x2, 0x4, 0x8, 0xF
 
 OLV_W_NONE, OLV_W_I32, OLV_W_F32 = 0, 1, 2
@@ -134,6 +135,8 @@ SIGNATURES = {
     "olv_queue_to_bitmap": (_int, [_vp, _vp, _vp, _u32, _int]),
     "olv_frontier_bin": (_int, [_vp, _vp, _vp, ctypes.POINTER(_vp), _vp]),
     "olv_graph_buckets": (_int, [_vp, _u32p, _u32p]),
+    "olv_frontier_split": (_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
+    "olv_hub_capacity": (_sz, [_u64]),
     "olv_graph_create": (_int, [ctypes.POINTER(_vp), _u32, _u32, _vp, _vp, _vp, _int, _uint]),
     "olv_graph_destroy": (_int, [_vp]),
     "olv_graph_build_transpose": (_int, [_vp]),

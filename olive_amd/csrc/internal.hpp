@@ -68,6 +68,7 @@ constexpr uint32_t kHubThrMin = 256;  // smallest hub item (sizes the hub item b
 
 struct BfsWork;   // bfs.hip
 struct PrWork;    // pagerank.hip
+struct SsspBinWork;  // sssp_bin.hip
 
 }  // namespace olv
 
@@ -92,6 +93,7 @@ struct olv_graph {
   bool plan_in_ready = false, plan_out_ready = false;
   olv::BfsWork *bfs = nullptr;
   olv::PrWork *pr = nullptr;
+  olv::SsspBinWork *sssp_bin = nullptr;  // binned SSSP work (for the last k sources)
   // In-degree numbering: vertices by descending in-degree, ties by id
   // (built on first use; shared by the PageRank pull and the BFS twin).
   uint32_t *rl_pi = nullptr;       // vertex -> new number
@@ -129,6 +131,11 @@ void free_bfs_work(olv_graph *g);
 // (looked up once per device).
 int noalloc_scratch(unsigned char **out);
 void free_pr_work(olv_graph *g);
+void free_sssp_bin(olv_graph *g);
+// Binned SSSP (sssp_bin.hip) for k instances from device sources; *used =
+// false when the graph does not fit it (the caller then relaxes directly).
+int sssp_bin_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, const int32_t *weights, int32_t *dist,
+                 int32_t delta, unsigned flags, olv_stats *st, bool *used);
 
 // RAII device buffer for temporaries inside an entry point.
 struct DevBuf {

@@ -398,6 +398,7 @@ static void graph_free(olv_graph *g) {
   if (!g) return;
   free_bfs_work(g);
   free_pr_work(g);
+  free_sssp_bin(g);
   g->plan_in.release();
   g->plan_out.release();
   graph_free(g->twin);

@@ -407,6 +407,14 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
   const uint64_t nv = (uint64_t)k * g->n;
   if (nv >= (1ull << 32)) return fail(OLV_ERANGE, "sssp: %u sources x %u vertices exceed 2^32 ids", k, g->n);
   const bool multi = k > 1;
+  int32_t delta = 0;
+  if (!(flags & OLV_PERSISTENT)) OLV_TRY(sssp_delta(g, weights, &delta));
+  // destination binning (sssp_bin.hip) unless asked to relax directly
+  if (delta > 0 && !(flags & (OLV_PERSISTENT | OLV_SSSP_DIRECT))) {
+    bool used = false;
+    OLV_TRY(sssp_bin_run(g, d_srcs, k, weights, dist, delta, flags, st, &used));
+    if (used) return OLV_OK;
+  }
   SsspWork w;
   OLV_TRY(w.alloc(nv, k, g->m));
   SsspParams p;
@@ -426,8 +434,6 @@ static int sssp_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, cons
   p.wave_thr = g->tune.td_wave_thr;
   p.block_thr = g->tune.td_block_thr;
   p.hub_thr = g->tune.hub_thr;
-  int32_t delta = 0;
-  if (!(flags & OLV_PERSISTENT)) OLV_TRY(sssp_delta(g, weights, &delta));
   p.delta = delta;
   p.far_bm = delta ? w.bms + (size_t)2 * w.nwp : nullptr;
   // plain rounds keep the reference's negative-cycle rule (n rounds); with

@@ -85,9 +85,9 @@ def test_sssp_batched(olv):
     got = out.numpy().reshape(len(srcs), csr.n)
     for i, s in enumerate(srcs):
         assert np.array_equal(got[i], oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, int(s)))
-    for fl in (0x8,):
+    for fl in (0x8, 0x800, 0x808):
         out2, _ = g.sssp_batched(srcs, flags=fl)
-        assert np.array_equal(out2.numpy(), out.numpy())
+        assert np.array_equal(out2.numpy(), out.numpy()), fl
 
 
 def test_multigraph_union(olv):

@@ -23,6 +23,8 @@ MODES = {
     "persistent_push_only": 0x20 | 0x2,
     "bottom_up": 0x4,                   # every level after the first pulls
     "persistent_bottom_up": 0x20 | 0x4,
+    "direct": 0x800,                    # SSSP: per-edge probe + atomics, no binning
+    "direct_host_loop": 0x800 | 0x8,
     # the same drivers in the graph's own numbering (the others run on the
     # graph renumbered by descending in-degree)
     "graph_order": 0x40,
@@ -134,7 +136,7 @@ def test_bfs_errors(olv):
 
 
 # -------------------------------------------------------------------- SSSP
-@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent"])
+@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent", "direct"])
 def test_sssp_weighted_data_files(olv, mode):
     for name in DATA_FILES:
         csr = olv.load_edge_list(data_file(name))
@@ -147,7 +149,7 @@ def test_sssp_weighted_data_files(olv, mode):
             assert np.array_equal(out.numpy(), G[f"{name}/sssp/{s}"]), (name, s)
 
 
-@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent"])
+@pytest.mark.parametrize("mode", ["pipeline", "host_loop", "persistent", "direct", "direct_host_loop"])
 def test_sssp_uniform(olv, mode):
     say(f"sssp uniform {mode}")
     csr = olv.with_weights_i32(olv.gen_uniform(1 << 16, 1 << 19, seed=7), 1, 64, seed=7)
@@ -209,11 +211,12 @@ def test_sssp_zero_and_large_weights(olv):
         assert np.array_equal(g.sssp(0)[0].numpy(), ref), (lo, hi)
 
 
-@pytest.mark.parametrize("flags", [0, 0x8])
+@pytest.mark.parametrize("flags", [0, 0x8, 0x800, 0x808])
 def test_sssp_chain_with_one_heavy_edge(olv, flags):
-    """A 1000-vertex chain with unit weights and one weight-1e6 edge: the
-    near-far step (~32k) needs ~31 promotions with nothing to relax, so the
-    run takes more rounds than the chain has vertices."""
+    """A 1000-vertex chain with unit weights and one weight-1e6 edge: with
+    direct relaxation (0x800) the near-far step (~32k) needs ~31 promotions
+    with nothing to relax, so the run takes more rounds than the chain has
+    vertices; the binned path re-splits past the heavy edge at once."""
     n = 1000
     rp = np.arange(n + 1, dtype=np.uint32)
     rp[n] = n - 1
@@ -227,6 +230,9 @@ def test_sssp_chain_with_one_heavy_edge(olv, flags):
     assert ref[-1] == n - 2 + 1_000_000
     assert np.array_equal(out.numpy(), ref)
     say(f"  chain: {st.iterations} rounds")
+    if not flags & 0x800:
+        assert st.iterations <= n + 2  # one round per hop, plus one re-split
+    assert st.edges_inspected == n - 1  # every edge relaxed exactly once
 
 
 def test_sssp_negative_cycle_reports_enoconv(olv):
@@ -242,6 +248,50 @@ def test_sssp_full_size_uniform(olv):
     say("sssp uniform 2^22 x 2^25")
     csr = olv.with_weights_i32(olv.gen_uniform(1 << 22, 1 << 25, seed=1), 1, 64, seed=1)
     g = olv.Graph(csr, transpose=False)
-    out, st = g.sssp(0)
     ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 0)
-    assert np.array_equal(out.numpy(), ref)
+    for fl in (0, 0x800):
+        out, st = g.sssp(0, flags=fl)
+        assert np.array_equal(out.numpy(), ref), fl
+        say(f"  flags {fl:#x}: {st.iterations} rounds, {st.edges_inspected} edges read, {st.elapsed_ms:.3f} ms")
+    # a second call reuses the graph's bins and must reset every round's state
+    out2, _ = g.sssp(0)
+    assert np.array_equal(out2.numpy(), ref)
+
+
+def sources(csr, k, seed):
+    """k distinct-ish sources among vertices with out-edges."""
+    nz = np.flatnonzero(csr.outdeg())
+    rng = np.random.default_rng(seed)
+    return rng.choice(nz, size=k, replace=k > nz.size).astype(np.uint32)
+
+
+def test_sssp_binned_hubs_and_instances(olv):
+    """Binned relaxation with hub items (RMAT, out-degree >= hub_thr) and k
+    instances whose virtual ranges straddle partitions (n not a multiple of
+    the 8192-vertex slice), against Dijkstra; zero-weight edges included."""
+    say("sssp binned: rmat hubs, 6 instances")
+    csr = olv.with_weights_i32(olv.gen_rmat(16, 16, seed=5), 0, 9, seed=5)
+    assert int(csr.outdeg().max()) >= 1024
+    g = olv.Graph(csr, transpose=False)
+    srcs = sources(csr, 6, 11)
+    out, st = g.sssp_batched(srcs)
+    got = out.numpy().reshape(len(srcs), csr.n)
+    for i, s_ in enumerate(srcs):
+        assert np.array_equal(got[i], oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, int(s_))), i
+    one, _ = g.sssp(int(srcs[3]))
+    assert np.array_equal(one.numpy(), got[3])
+
+
+def test_sssp_binned_wide_slices(olv):
+    """k instances of a 300,007-vertex graph: 12.0M virtual vertices (16K-vertex
+    slices), 24.0M (32K-vertex slices) and 60.0M (past the binned scheme's
+    2^25: relaxed directly)."""
+    csr = olv.with_weights_i32(olv.gen_uniform(300_007, 600_000, seed=9), 1, 100, seed=9)
+    g = olv.Graph(csr, transpose=False)
+    for k in (40, 80, 200):
+        say(f"sssp binned wide slices: {k} instances")
+        srcs = sources(csr, k, k)
+        out, st = g.sssp_batched(srcs)
+        got = out.numpy().reshape(k, csr.n)
+        for i in (0, k // 2, k - 1):
+            assert np.array_equal(got[i], oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, int(srcs[i]))), (k, i)
