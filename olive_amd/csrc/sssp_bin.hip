@@ -495,11 +495,11 @@ __global__ void ssspb_hub_kernel(const uint32_t *rp, uint32_t n, uint32_t thr, u
 // smallest dirty far distance + delta; nothing dirty -> done.  Half the
 // kernel boundaries of the expand/apply pair above, and none of its
 // reservation atomics.
-// Slices of 2^14 vertices (64 KB of dist; ~125 KB of LDS, one workgroup per
-// CU) up to 8M virtual vertices.  (2^13-vertex slices fit two workgroups per
+// Slices of 2^14 vertices (64 KB of dist; ~135 KB of LDS, one workgroup per
+// CU) up to 16M virtual vertices.  (2^13-vertex slices fit two workgroups per
 // CU by LDS, but the second ones started up to 120 us late on MI355X and a
 // 4M-vertex run took 1.78 ms against 1.30 ms.)
-constexpr uint32_t kFParts = 512;                // partitions at most
+constexpr uint32_t kFParts = 1024;               // partitions at most (16M virtual vertices)
 constexpr int kFB = 1024;                        // workgroup
 
 struct alignas(128) FusedSlot {
@@ -1155,11 +1155,26 @@ int sssp_bin_run(const olv_graph *g, const uint32_t *d_srcs, uint32_t k, const i
   if (delta <= 0 || g->m == 0 || g->n == 0) return OLV_OK;
   const uint64_t nv = (uint64_t)k * g->n;
   if ((uint64_t)k * g->m >= (1ull << 32) || nv >= (1ull << 32)) return OLV_OK;
-  // one kernel per round when no out-list needs splitting (hub items) and
-  // the partitions fit the fused kernel's LDS tables
-  if (g->max_outdeg < g->tune.hub_thr && ((nv + (1u << 14) - 1) >> 14) <= kFParts) {
+  // one kernel per round when no out-list needs splitting (hub items);
+  // more instances than the fused kernel's 16M virtual vertices run in
+  // groups of as many as fit
+  const uint64_t fmax = (uint64_t)kFParts << 14;
+  if (g->max_outdeg < g->tune.hub_thr && g->n <= fmax) {
     *used = true;
-    return fused_run(g, d_srcs, k, weights, dist, delta, flags, st);
+    const uint32_t gk = (uint32_t)std::min<uint64_t>(k, fmax / g->n);
+    if (gk == k) return fused_run(g, d_srcs, k, weights, dist, delta, flags, st);
+    olv_stats tot;
+    std::memset(&tot, 0, sizeof(tot));
+    for (uint32_t s0 = 0; s0 < k; s0 += gk) {
+      const uint32_t kk = std::min(gk, k - s0);
+      olv_stats one;
+      OLV_TRY(fused_run(g, d_srcs + s0, kk, weights, dist + (size_t)s0 * g->n, delta, flags, &one));
+      tot.elapsed_ms += one.elapsed_ms;
+      tot.iterations += one.iterations;
+      tot.edges_inspected += one.edges_inspected;
+    }
+    if (st) *st = tot;
+    return OLV_OK;
   }
   uint32_t sb = kBinSbMin;
   while (sb < kBinSbMax && ((nv + (1ull << sb) - 1) >> sb) > kBinPartMax) ++sb;

@@ -280,12 +280,20 @@ def test_sssp_binned_hubs_and_instances(olv):
         assert np.array_equal(got[i], oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, int(s_))), i
     one, _ = g.sssp(int(srcs[3]))
     assert np.array_equal(one.numpy(), got[3])
+    # 260 instances: 17.0M virtual vertices, the expand/apply pair on
+    # 32K-vertex slices
+    say("sssp binned: rmat hubs, 260 instances")
+    srcs = sources(csr, 260, 12)
+    out, st = g.sssp_batched(srcs)
+    got = out.numpy().reshape(len(srcs), csr.n)
+    for i in (0, 129, 259):
+        assert np.array_equal(got[i], oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, int(srcs[i]))), i
 
 
 def test_sssp_binned_wide_slices(olv):
-    """k instances of a 300,007-vertex graph: 12.0M virtual vertices (16K-vertex
-    slices), 24.0M (32K-vertex slices) and 60.0M (past the binned scheme's
-    2^25: relaxed directly)."""
+    """k instances of a 300,007-vertex graph: 12.0M virtual vertices (one
+    fused run), 24.0M and 60.0M (past the fused kernel's 16M: groups of 55
+    instances, the last one shorter)."""
     csr = olv.with_weights_i32(olv.gen_uniform(300_007, 600_000, seed=9), 1, 100, seed=9)
     g = olv.Graph(csr, transpose=False)
     for k in (40, 80, 200):
