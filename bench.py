@@ -336,7 +336,7 @@ def bench_sssp(args, olv):
     b = 12 * er + 16 * vr + 4 * csr.n
     res["roofline"] = {"bound": "hbm", "achieved": b / t / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                        "frac": b / t / 1e9 / HBM_PEAK_GBS, "traffic": load_traffic("sssp_uniform"),
-                       "kernel": "sssp_relax_kernel + sssp_compact_kernel (whole run)",
+                       "kernel": "ssspf_round_kernel (whole run: one launch per round)",
                        "bytes_model": "12*E_r + 16*V_r + 4*V"}
     if not args.no_cpu_baseline:
         tc, runs = timed_oracle(lambda: oracle.sssp(csr.row_ptr, csr.col, csr.weights, 0), 30.0, 2)
