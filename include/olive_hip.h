@@ -384,7 +384,10 @@ typedef struct olv_tuning {
   uint32_t seg_bounds[3];           /* row pulls: 1 lane per row up to seg_bounds[0]
                                        entries, 8 lanes up to [1], a wave up to [2],
                                        then a workgroup                                  */
-  uint32_t sssp_relax_per_cu;       /* SSSP relaxation workgroups per CU                 */
+  uint32_t sssp_relax_per_cu;       /* SSSP direct relaxation (OLV_SSSP_DIRECT)
+                                       workgroups per CU                                 */
+  uint32_t sssp_delta_num;          /* SSSP near-far step = mean weight x this / mean
+                                       out-degree (1..1024; distances do not change)    */
   uint32_t source;                  /* 0 defaults, 1 measured by olv_autotune_ex, 2 read
                                        from a tuning cache, 3 set by the caller          */
   double tune_ms;                   /* host time olv_autotune_ex spent (0 on a hit)      */
@@ -393,7 +396,7 @@ typedef struct olv_tuning {
 #define OLV_TUNE_BFS     0x1u  /* plain BFS: grids, bucket thresholds, hubs, alpha/beta */
 #define OLV_TUNE_BATCHED 0x2u  /* 256-source BFS: bottom-up grid                         */
 #define OLV_TUNE_PULL    0x4u  /* row pulls: grid, unroll, bucket bounds                 */
-#define OLV_TUNE_SSSP    0x8u  /* SSSP relaxation grid (graphs with int32 weights)       */
+#define OLV_TUNE_SSSP    0x8u  /* SSSP near-far step, direct relaxation grid (int32 w)   */
 #define OLV_TUNE_ALL     0xFu
 
 int olv_get_tuning_ex(const olv_graph *g, olv_tuning *out);

@@ -76,6 +76,7 @@ class Tuning(ctypes.Structure):
         ("seg_unroll", ctypes.c_uint32),
         ("seg_bounds", ctypes.c_uint32 * 3),
         ("sssp_relax_per_cu", ctypes.c_uint32),
+        ("sssp_delta_num", ctypes.c_uint32),
         ("source", ctypes.c_uint32),
         ("tune_ms", ctypes.c_double),
     ]

@@ -39,6 +39,7 @@ void to_c(const Tuning &t, olv_tuning *o) {
   o->seg_unroll = t.seg_unroll;
   for (int i = 0; i < 3; ++i) o->seg_bounds[i] = t.seg_bounds[i];
   o->sssp_relax_per_cu = t.sssp_relax_per_cu;
+  o->sssp_delta_num = t.sssp_delta_num;
   o->source = t.source;
   o->tune_ms = t.tune_ms;
 }
@@ -58,6 +59,7 @@ int from_c(const olv_tuning *o, Tuning &t) {
       o->seg_bounds[2] > 8192)
     return bad("seg_bounds");
   if (o->sssp_relax_per_cu < 1 || o->sssp_relax_per_cu > 64) return bad("sssp_relax_per_cu");
+  if (o->sssp_delta_num < 1 || o->sssp_delta_num > 1024) return bad("sssp_delta_num");
   t.bfs_expand_per_cu = o->bfs_expand_per_cu;
   t.bfs_expand_per_cu_twin = o->bfs_expand_per_cu_twin;
   t.bfs_compact_per_cu = o->bfs_compact_per_cu;
@@ -71,6 +73,7 @@ int from_c(const olv_tuning *o, Tuning &t) {
   t.seg_unroll = o->seg_unroll;
   for (int i = 0; i < 3; ++i) t.seg_bounds[i] = o->seg_bounds[i];
   t.sssp_relax_per_cu = o->sssp_relax_per_cu;
+  t.sssp_delta_num = o->sssp_delta_num;
   return OLV_OK;
 }
 
@@ -112,6 +115,7 @@ constexpr Field kFields[] = {
     {"alpha", true},              {"beta", true},                    {"mbfs_pull_per_cu", false},
     {"seg_grid_per_cu", false},   {"seg_unroll", false},             {"seg_bound0", false},
     {"seg_bound1", false},        {"seg_bound2", false},             {"sssp_relax_per_cu", false},
+    {"sssp_delta_num", false},
 };
 
 void get_fields(const olv_tuning &t, double *v) {
@@ -119,7 +123,8 @@ void get_fields(const olv_tuning &t, double *v) {
                       (double)t.td_wave_thr,       (double)t.td_block_thr,           (double)t.hub_thr,
                       (double)t.alpha,             (double)t.beta,                   (double)t.mbfs_pull_per_cu,
                       (double)t.seg_grid_per_cu,   (double)t.seg_unroll,             (double)t.seg_bounds[0],
-                      (double)t.seg_bounds[1],     (double)t.seg_bounds[2],          (double)t.sssp_relax_per_cu};
+                      (double)t.seg_bounds[1],     (double)t.seg_bounds[2],          (double)t.sssp_relax_per_cu,
+                      (double)t.sssp_delta_num};
   for (size_t i = 0; i < sizeof(f) / sizeof(f[0]); ++i) v[i] = f[i];
 }
 
@@ -139,6 +144,7 @@ void set_fields(olv_tuning &t, const double *v) {
   t.seg_bounds[1] = (uint32_t)v[12];
   t.seg_bounds[2] = (uint32_t)v[13];
   t.sssp_relax_per_cu = (uint32_t)v[14];
+  t.sssp_delta_num = (uint32_t)v[15];
 }
 
 constexpr int kNumFields = sizeof(kFields) / sizeof(kFields[0]);
@@ -245,13 +251,13 @@ struct Searcher {
     *ms = best;
     return OLV_OK;
   }
-  int time_sssp(double *ms) {  // the first two sample sources
+  int time_sssp(double *ms, unsigned flags) {  // the first two sample sources
     double tot = 0;
     for (size_t i = 0; i < srcs.size() && i < 2; ++i) {
       double best = 1e300;
       for (int r = 0; r < 2; ++r) {
         olv_stats st;
-        OLV_TRY(olv_sssp(g, srcs[i], nullptr, levels, 0, &st));
+        OLV_TRY(olv_sssp(g, srcs[i], nullptr, levels, flags, &st));
         best = std::min(best, st.elapsed_ms);
       }
       tot += best;
@@ -501,9 +507,15 @@ int olv_autotune_ex(olv_graph *g, unsigned what, const char *cache_path, olv_tun
                       {}, [&](double *ms) { return S.time_pull(ms); }));
   }
   if ((what & OLV_TUNE_SSSP) && g->wdtype == OLV_W_I32) {
+    // the step on the default (binned) path; the grid on the direct path,
+    // the only one it shapes
     double warm;
-    OLV_TRY(S.time_sssp(&warm));
-    OLV_TRY(S.descend({{&T.sssp_relax_per_cu, {4, 6, 8, 12}}}, {}, [&](double *ms) { return S.time_sssp(ms); }));
+    OLV_TRY(S.time_sssp(&warm, 0u));
+    OLV_TRY(S.descend({{&T.sssp_delta_num, {16, 24, 32, 48, 64}}}, {},
+                      [&](double *ms) { return S.time_sssp(ms, 0u); }));
+    OLV_TRY(S.time_sssp(&warm, OLV_SSSP_DIRECT));
+    OLV_TRY(S.descend({{&T.sssp_relax_per_cu, {4, 6, 8, 12}}}, {},
+                      [&](double *ms) { return S.time_sssp(ms, OLV_SSSP_DIRECT); }));
   }
   T.source = 1;
   T.tuned = true;

@@ -59,7 +59,8 @@ struct Tuning {
   uint32_t seg_grid_per_cu = 32;     // row pulls: workgroups per CU
   uint32_t seg_unroll = 8;           // row pulls: gathers in flight per lane (4, 8, 16)
   uint32_t seg_bounds[3] = {8, 96, 1024};  // row pulls: 1 lane / 8 lanes / a wave up to these degrees
-  uint32_t sssp_relax_per_cu = 6;    // SSSP relaxation workgroups per CU
+  uint32_t sssp_relax_per_cu = 6;    // SSSP relaxation workgroups per CU (direct path)
+  uint32_t sssp_delta_num = 32;      // SSSP near-far step: mean weight x this / mean out-degree
   uint32_t source = 0;               // 0 defaults, 1 measured, 2 tuning cache, 3 set by the caller
   double tune_ms = 0.0;
   bool tuned = false;

@@ -364,13 +364,8 @@ static int weight_stats(const int32_t *w, uint32_t m, int32_t *wmin, double *wme
   return OLV_OK;
 }
 
-#ifndef OLV_SSSP_DELTA_NUM
-#define OLV_SSSP_DELTA_NUM 32
-#endif
-
-// Near-far step: about OLV_SSSP_DELTA_NUM edges' worth of distance per
-// vertex (mean weight x NUM / mean out-degree); 0 (plain rounds) when a
-// weight is negative.
+// Near-far step: mean weight x the graph's sssp_delta_num (32 unless tuned)
+// / mean out-degree; 0 (plain rounds) when a weight is negative.
 static int sssp_delta(const olv_graph *g, const int32_t *w, int32_t *delta) {
   *delta = 0;
   if (!g->m || !g->n) return OLV_OK;
@@ -390,7 +385,7 @@ static int sssp_delta(const olv_graph *g, const int32_t *w, int32_t *delta) {
   }
   if (wmin < 0) return OLV_OK;
   const double deg = (double)g->m / g->n;
-  const double d = wmean * OLV_SSSP_DELTA_NUM / (deg > 1.0 ? deg : 1.0);
+  const double d = wmean * g->tune.sssp_delta_num / (deg > 1.0 ? deg : 1.0);
   *delta = d < 1.0 ? 1 : (d > 1e9 ? 1000000000 : (int32_t)(d + 0.5));
   return OLV_OK;
 }

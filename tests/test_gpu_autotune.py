@@ -99,3 +99,29 @@ def test_set_tuning_validates(olv):
         with pytest.raises(olv.OliveError) as e:
             g.set_tuning(t)
         assert e.value.code == 1 and field.split("_")[0] in str(e.value)
+
+
+def test_sssp_step_changes_rounds_not_distances(olv):
+    """The near-far step (sssp_delta_num) shapes the rounds of both SSSP
+    paths; every setting gives Dijkstra's distances, and an out-of-range one
+    is refused."""
+    from olive_amd import _lib
+    csr = olv.with_weights_i32(olv.gen_uniform(1 << 15, 1 << 18, seed=8), 1, 64, seed=8)
+    g = olv.Graph(csr, transpose=False)
+    ref = oracle.dijkstra(csr.row_ptr, csr.col, csr.weights, 5)
+    rounds = {}
+    for num in (1, 8, 32, 256, 1024):
+        t = g.tuning()
+        t.sssp_delta_num = num
+        g.set_tuning(t)
+        for fl in (0, _lib.OLV_SSSP_DIRECT):
+            out, st = g.sssp(5, flags=fl)
+            assert np.array_equal(out.numpy(), ref), (num, fl)
+            rounds[(num, fl)] = st.iterations
+    say("rounds by step:", rounds)
+    assert rounds[(1, 0)] > rounds[(1024, 0)]
+    t = g.tuning()
+    t.sssp_delta_num = 0
+    with pytest.raises(olv.OliveError) as e:
+        g.set_tuning(t)
+    assert e.value.code == 1 and "sssp_delta_num" in str(e.value)

@@ -13,7 +13,7 @@ def _tuning(**kw):
     t = _lib.Tuning()
     base = dict(bfs_expand_per_cu=8, bfs_expand_per_cu_twin=6, bfs_compact_per_cu=8, td_wave_thr=64,
                 td_block_thr=256, hub_thr=1024, alpha=14.0, beta=24.0, mbfs_pull_per_cu=48, seg_grid_per_cu=32,
-                seg_unroll=8, sssp_relax_per_cu=6)
+                seg_unroll=8, sssp_relax_per_cu=6, sssp_delta_num=32)
     bounds = kw.pop("seg_bounds", [8, 96, 1024])
     base.update(kw)
     for k, v in base.items():
