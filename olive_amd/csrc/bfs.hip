@@ -490,6 +490,124 @@ static int collect_stats(const olv_graph *g, BfsWork *w, const int32_t *levels, 
   return OLV_OK;
 }
 
+// ------------------------------------------------------------- small graphs
+// A graph of at most 16,384 vertices whose out-lists are short runs its
+// whole traversal in one workgroup: the row offsets, the visited bits and
+// both frontier queues sit in LDS (~130 KB), so a level costs the column
+// loads (L2 hits after the first level), LDS claims and one barrier -- not
+// two kernel launches.  The 128 x 128 grid (255 levels of at most 128
+// vertices) was launch-bound at ~11 us per level in the pipeline.
+constexpr uint32_t kSmallN = 16384;
+constexpr uint32_t kSmallMaxDeg = 256;
+constexpr int kSmallBlock = 1024;
+
+// out[0] = levels run (the deepest level + 1), out[1] = edges read (= the
+// out-degree sum of reached vertices), out[2] = vertices reached.
+__global__ void __launch_bounds__(kSmallBlock) bfs_small_kernel(const uint32_t *__restrict__ rp,
+                                                                const uint32_t *__restrict__ col, uint32_t n,
+                                                                const uint32_t *__restrict__ seeds, uint32_t k,
+                                                                int32_t *__restrict__ levels,
+                                                                unsigned long long *__restrict__ out) {
+  __shared__ uint32_t s_rp[kSmallN + 1];
+  __shared__ uint32_t s_vis[kSmallN / 32];
+  __shared__ uint16_t s_q[2][kSmallN];
+  __shared__ uint32_t s_cnt[2];
+  __shared__ unsigned long long s_red[kSmallBlock / kWave];
+  const uint32_t t = threadIdx.x;
+  for (uint32_t i = t; i <= n; i += kSmallBlock) s_rp[i] = rp[i];
+  for (uint32_t i = t; i < kSmallN / 32; i += kSmallBlock) s_vis[i] = 0u;
+  for (uint32_t v = t; v < n; v += kSmallBlock) levels[v] = kInf;
+  if (t < 2) s_cnt[t] = 0u;
+  __syncthreads();
+  for (uint32_t i = t; i < k; i += kSmallBlock) {
+    const uint32_t v = seeds[i], bit = 1u << (v & 31);
+    if (!(atomicOr(&s_vis[v >> 5], bit) & bit)) {
+      s_q[0][atomicAdd(&s_cnt[0], 1u)] = (uint16_t)v;
+      levels[v] = 0;
+    }
+  }
+  __syncthreads();
+  uint32_t cur = 0, L = 0;
+  unsigned long long insp = 0, reached = 0;
+  for (;;) {
+    const uint32_t qs = s_cnt[cur];
+    if (qs == 0 || L > n) break;
+    reached += t < qs ? (qs - t + kSmallBlock - 1) / kSmallBlock : 0u;
+    if (t == 0) s_cnt[cur ^ 1] = 0u;
+    __syncthreads();
+    const uint32_t nxt = cur ^ 1;
+    for (uint32_t i = t; i < qs; i += kSmallBlock) {
+      const uint32_t u = s_q[cur][i];
+      const uint32_t b = s_rp[u], e = s_rp[u + 1];
+      insp += e - b;
+      for (uint32_t x = b; x < e; x += 4) {
+        uint32_t v[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) v[j] = x + j < e ? col[x + j] : 0xffffffffu;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          if (v[j] == 0xffffffffu) continue;
+          const uint32_t bit = 1u << (v[j] & 31);
+          if (s_vis[v[j] >> 5] & bit) continue;
+          if (!(atomicOr(&s_vis[v[j] >> 5], bit) & bit)) {
+            levels[v[j]] = (int32_t)(L + 1);
+            s_q[nxt][atomicAdd(&s_cnt[nxt], 1u)] = (uint16_t)v[j];
+          }
+        }
+      }
+    }
+    __syncthreads();
+    cur = nxt;
+    ++L;
+  }
+  const unsigned long long ti = block_sum<kSmallBlock, unsigned long long>(insp, s_red);
+  const unsigned long long tr = block_sum<kSmallBlock, unsigned long long>(reached, s_red);
+  if (t == 0) {
+    out[0] = L;
+    out[1] = ti;
+    out[2] = tr;
+  }
+}
+
+static bool small_bfs_fits(const olv_graph *g, unsigned flags) {
+  return g->n <= kSmallN && g->max_outdeg <= kSmallMaxDeg &&
+         !(flags & (OLV_PERSISTENT | OLV_HOST_LOOP | OLV_FORCE_BOTTOM_UP));
+}
+
+static int bfs_small(const olv_graph *g, const std::vector<uint32_t> &seeds, int32_t *levels, bool host,
+                     olv_stats *st) {
+  const uint32_t k = (uint32_t)seeds.size();
+  DevBuf d_seeds, d_out, d_levels;
+  OLV_TRY(d_seeds.alloc((size_t)k * 4));
+  OLV_TRY(d_out.alloc(3 * sizeof(unsigned long long)));
+  OLV_HIP(hipMemcpyAsync(d_seeds.p, seeds.data(), (size_t)k * 4, hipMemcpyHostToDevice, stream()));
+  int32_t *outl = levels;
+  if (host) {
+    OLV_TRY(d_levels.alloc((size_t)g->n * 4));
+    outl = d_levels.as<int32_t>();
+  }
+  Timer t;
+  t.start();
+  hipLaunchKernelGGL(bfs_small_kernel, dim3(1), dim3(kSmallBlock), 0, stream(), g->row_ptr, g->col, g->n,
+                     d_seeds.as<uint32_t>(), k, outl, d_out.as<unsigned long long>());
+  OLV_HIP(hipGetLastError());
+  t.stop();
+  unsigned long long h[3];
+  OLV_HIP(hipMemcpyAsync(h, d_out.p, sizeof(h), hipMemcpyDeviceToHost, stream()));
+  OLV_HIP(hipStreamSynchronize(stream()));
+  if (h[0] > g->n) return fail(OLV_EINTERNAL, "olv_bfs: more than n levels");
+  if (host) OLV_HIP(hipMemcpy(levels, outl, (size_t)g->n * 4, hipMemcpyDeviceToHost));
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->elapsed_ms = t.ms();
+    st->iterations = (uint32_t)h[0];
+    st->edges_inspected = h[1];
+    st->edges_traversed = h[1];
+    st->vertices_reached = h[2];
+  }
+  return OLV_OK;
+}
+
 // Multi-source BFS on g (levels = distance to the nearest seed).
 // seeds_host: seeds is a host array (levels follow OLV_PTR_HOST in flags).
 // Direction-optimising runs go to g's twin -- g renumbered by descending
@@ -511,6 +629,7 @@ int bfs_multisource(const olv_graph *g0, const uint32_t *seeds, uint32_t k, bool
   const bool can_bu = !(flags & OLV_NO_DIRECTION_OPT);
   if (!can_bu && (flags & OLV_FORCE_BOTTOM_UP))
     return fail(OLV_EINVAL, "olv_bfs: OLV_FORCE_BOTTOM_UP and OLV_NO_DIRECTION_OPT together");
+  if (small_bfs_fits(g0, flags)) return bfs_small(g0, h_seeds, levels, host, st);
   const bool twin = can_bu && !(flags & OLV_GRAPH_ORDER);
   const olv_graph *g = g0;
   if (twin) {
