@@ -19,7 +19,13 @@ int upload_vec(T **dst, const std::vector<T> &src) {
   return OLV_OK;
 }
 
-inline uint32_t piece_bucket(uint32_t len) { return len <= 8 ? 0u : len <= 64 ? 1u : 2u; }
+// Lanes per piece by length: about eight entries per lane (1, 2, 4, 8
+// lanes up to 8, 16, 32, 64 entries; a wave up to kPieceMax).
+inline uint32_t piece_bucket(uint32_t len) {
+  return len <= 8 ? 0u : len <= 16 ? 1u : len <= 32 ? 2u : len <= 64 ? 3u : 4u;
+}
+constexpr uint32_t kBucketLanes[5] = {1, 2, 4, 8, 64};
+constexpr int kBuckets = 5;
 
 }  // namespace
 
@@ -84,8 +90,8 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   order.reserve(npieces);
   std::vector<uint4> &units = h.units;
   units.clear();
-  std::vector<uint32_t> unit_cost;
-  std::vector<Piece> byb[3];
+  std::vector<double> unit_cost;
+  std::vector<Piece> byb[kBuckets];
   uint32_t edge_cursor = 0;  // first entry of the next unit
   for (uint32_t s = 0; s < nseg; ++s) {
     for (auto &b : byb) b.clear();
@@ -99,7 +105,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
       }
       p = e;
     }
-    for (int b = 2; b >= 0; --b) {
+    for (int b = kBuckets - 1; b >= 0; --b) {
       // units of at most kUnitEdges entries and kUnitPieces pieces over this
       // (segment, bucket) group
       size_t i = 0;
@@ -114,31 +120,45 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
         }
         units.push_back(make_uint4(s, (uint32_t)b, first, edge_cursor));
         for (uint32_t q = first; q < (uint32_t)order.size(); ++q) edge_cursor += order[q].len;
-        // cost in steps of the whole workgroup: the staging (measured at
-        // about three steps), then one step per 1024 / 128 / 16 pieces (1, 8
-        // or 64 lanes per piece)
-        const uint32_t per_step = b == 0 ? 1024u : b == 1 ? 128u : 16u;
-        unit_cost.push_back(3 + (pieces + per_step - 1) / per_step);
+        // every unit starts on an 8-entry (16-byte) boundary: staged with
+        // 16-byte loads and LDS stores
+        if ((uint64_t)edge_cursor + 7 >= 0xffffffffull) return fail(OLV_ERANGE, "blocked plan: entries overflow");
+        edge_cursor = (edge_cursor + 7) & ~7u;
+        // cost in us, fitted to per-unit timestamps of RMAT-20 iterations
+        // on MI355X: staging and the barriers around it ~2.4, then about
+        // 0.5-0.65 per step (one piece per group of G lanes) and 0.04 per
+        // thousand entries
+        const uint32_t per_step = kBlkThreads / kBucketLanes[b];
+        const uint32_t steps = (pieces + per_step - 1) / per_step;
+        const double step_us = 0.465 + 0.047 * b;
+        unit_cost.push_back(2.8 + step_us * steps + 4e-5 * edges);
       }
     }
   }
 
-  // 3. entry arrays in piece order
+  // 3. entry arrays in piece order, each unit from its (aligned) first
+  //    entry.  The last piece of a unit runs up to the next unit's first
+  //    entry: the gap holds kSegSize, the staged slice's zero slot, as do 8
+  //    spare entries at the end for the staging's 16-byte reads.
   std::vector<uint32_t> &pbeg = h.pbeg, &lslot = h.lslot;
   std::vector<uint16_t> &lidx = h.lidx;
+  const uint32_t o = edge_cursor;
   pbeg.assign(npieces + 1, 0);
-  lidx.assign(m, 0);
-  lslot.assign(with_slots ? m : 0, 0);
-  uint32_t o = 0;
-  for (size_t q = 0; q < npieces; ++q) {
-    const Piece &pc = order[q];
-    pbeg[q] = o;
-    for (uint32_t t = 0; t < pc.len; ++t) {
-      const uint32_t k = perm[pc.start + t];
-      lidx[o + t] = (uint16_t)(col[k] & (kSegSize - 1));
-      if (with_slots) lslot[o + t] = k;
+  lidx.assign((size_t)o + 8, (uint16_t)kSegSize);
+  lslot.assign(with_slots ? (size_t)o + 8 : 0, 0);
+  for (size_t un = 0; un < units.size(); ++un) {
+    uint32_t e = units[un].w;
+    const uint32_t qe = un + 1 < units.size() ? units[un + 1].z : (uint32_t)npieces;
+    for (uint32_t q = units[un].z; q < qe; ++q) {
+      const Piece &pc = order[q];
+      pbeg[q] = e;
+      for (uint32_t t = 0; t < pc.len; ++t) {
+        const uint32_t k = perm[pc.start + t];
+        lidx[e + t] = (uint16_t)(col[k] & (kSegSize - 1));
+        if (with_slots) lslot[e + t] = k;
+      }
+      e += pc.len;
     }
-    o += pc.len;
   }
   pbeg[npieces] = o;
   // each row's partials in segment order: cmb_pos[cmb_rp[v] + j] is the
@@ -146,24 +166,46 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   h.cmb_pos.assign(npieces, 0);
   for (size_t q = 0; q < npieces; ++q) h.cmb_pos[order[q].slot] = (uint32_t)q;
 
-  // 4. contiguous unit ranges of about equal cost per workgroup; a sentinel
-  //    unit closes the list (its first piece and entry are the totals)
+  // 4. units to workgroups: longest first, each to the workgroup where it
+  //    ends earliest, counting a slice load (~1.4 us) when the workgroup
+  //    has no unit of that segment yet; each workgroup then runs its units
+  //    in segment order.  A sentinel unit closes the list (its first piece
+  //    and entry are the totals).
   const uint32_t nunits = (uint32_t)units.size();
   units.push_back(make_uint4(nseg, 0, (uint32_t)npieces, o));
   const uint32_t ng = std::max<uint32_t>(1, std::min<uint32_t>(groups ? groups : 1, std::max<uint32_t>(nunits, 1)));
-  std::vector<uint32_t> &wg = h.wg_unit;
-  wg.assign(ng + 1, nunits);
-  wg[0] = 0;
   {
-    double total = 0;
-    for (uint32_t c : unit_cost) total += c;
-    double acc = 0;
-    uint32_t w = 1;
-    for (uint32_t u = 0; u < nunits && w < ng; ++u) {
-      acc += unit_cost[u];
-      while (w < ng && acc >= total * w / ng) wg[w++] = u + 1;
+    std::vector<uint32_t> byc(nunits);
+    for (uint32_t u = 0; u < nunits; ++u) byc[u] = u;
+    std::stable_sort(byc.begin(), byc.end(), [&](uint32_t a, uint32_t b) { return unit_cost[a] > unit_cost[b]; });
+    std::vector<double> load(ng, 0.0);
+    std::vector<std::vector<uint32_t>> mine(ng);
+    std::vector<uint8_t> has((size_t)ng * nseg, 0);  // workgroup w already loads segment s
+    constexpr double kSliceUs = 1.4;
+    for (uint32_t u : byc) {
+      const uint32_t sg = units[u].x;
+      uint32_t best = 0;
+      double bv = 0;
+      for (uint32_t w = 0; w < ng; ++w) {
+        const double v = load[w] + unit_cost[u] + (has[(size_t)w * nseg + sg] ? 0.0 : kSliceUs);
+        if (w == 0 || v < bv) {
+          bv = v;
+          best = w;
+        }
+      }
+      load[best] = bv;
+      has[(size_t)best * nseg + sg] = 1;
+      mine[best].push_back(u);
     }
-    for (; w < ng; ++w) wg[w] = nunits;
+    h.wg_ptr.assign(ng + 1, 0);
+    h.wg_list.clear();
+    h.wg_list.reserve(nunits);
+    for (uint32_t w = 0; w < ng; ++w) {
+      std::sort(mine[w].begin(), mine[w].end());  // unit order = segment order
+      h.wg_ptr[w] = (uint32_t)h.wg_list.size();
+      h.wg_list.insert(h.wg_list.end(), mine[w].begin(), mine[w].end());
+    }
+    h.wg_ptr[ng] = (uint32_t)h.wg_list.size();
   }
 
   h.npieces = (uint32_t)npieces;
@@ -186,7 +228,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   bp.nunits = h.nunits;
   bp.ngroups = h.ngroups;
   OLV_TRY(upload_vec(&bp.units, h.units));
-  OLV_TRY(upload_vec(&bp.wg_unit, h.wg_unit));
+  OLV_TRY(upload_vec(&bp.wg_ptr, h.wg_ptr));
+  OLV_TRY(upload_vec(&bp.wg_list, h.wg_list));
   OLV_TRY(upload_vec(&bp.pbeg, h.pbeg));
   OLV_TRY(upload_vec(&bp.lidx, h.lidx));
   if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));

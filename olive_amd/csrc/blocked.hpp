@@ -40,11 +40,12 @@
 namespace olv {
 
 constexpr uint32_t kSegBits = 14;
-constexpr uint32_t kSegSize = 1u << kSegBits;  // sources per LDS slice (64 KB of fp32)
+constexpr uint32_t kSegSize = 1u << kSegBits;  // sources per LDS slice (64 KB of fp32), plus a zero slot
+static_assert(kSegSize <= 0xffff, "entries hold a slice offset or the zero slot in 16 bits");
 constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 145 KB of LDS)
 constexpr uint32_t kPieceMax = 512;            // longest piece (one wave step); longer ones are cut into chunks
 constexpr uint32_t kUnitEdges = 24576;         // entries per work unit, at most (48 KB staged)
-constexpr uint32_t kUnitPieces = 4096;         // pieces per work unit, at most (2 x 16 KB staged)
+constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
 constexpr uint32_t kCombineLane = 32;          // rows with more partials are combined by a wave
 
 // Device arrays of a segmented pull over one CSR.
@@ -52,22 +53,25 @@ struct BlockedPlan {
   uint32_t n = 0, m = 0, n_live = 0;  // vertices, entries, rows with entries (a prefix of the rows)
   uint32_t n_heavy = 0;               // rows [0, n_heavy) hold every row with > kCombineLane partials
   uint32_t nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
-  uint4 *units = nullptr;        // nunits + 1: (segment, bucket 0/1/2 = 1/8/64 lanes per piece, first piece,
-                                 // first entry); the last is a sentinel (-, -, npieces, m)
-  uint32_t *wg_unit = nullptr;   // ngroups + 1: unit range of each segment-pass workgroup
+  uint4 *units = nullptr;        // nunits + 1: (segment, bucket 0..4 = 1/2/4/8/64 lanes per piece, first
+                                 // piece, first entry); the last is a sentinel (-, -, npieces, m)
+  uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_list
+  uint32_t *wg_list = nullptr;   // nunits: the units of each workgroup, in segment order
   uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
-  uint16_t *lidx = nullptr;      // m: source offset within the segment, in piece order
-  uint32_t *lslot = nullptr;     // m (weighted solves only): CSR slot each entry came from
+  uint16_t *lidx = nullptr;      // source offset within the segment, in piece order; each unit starts
+                                 // on an 8-entry boundary; the gaps and 8 spare entries at the end
+                                 // hold kSegSize (the slice's zero slot)
+  uint32_t *lslot = nullptr;     // (weighted solves only) CSR slot each entry came from, aligned as lidx
   uint32_t *cmb_rp = nullptr;    // n + 1: each row's range in cmb_pos
   uint32_t *cmb_pos = nullptr;   // npieces: partial index of each row's pieces, in segment order
   float *partial = nullptr;      // npieces
   bool ready = false;
   void release() {
-    void *ps[] = {units, wg_unit, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
+    void *ps[] = {units, wg_ptr, wg_list, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     units = nullptr;
-    wg_unit = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
+    wg_ptr = wg_list = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
     lidx = nullptr;
     partial = nullptr;
     ready = false;
@@ -78,7 +82,7 @@ struct BlockedPlan {
 struct BlockedHost {
   uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
   std::vector<uint4> units;
-  std::vector<uint32_t> wg_unit, pbeg, lslot, cmb_rp, cmb_pos;
+  std::vector<uint32_t> wg_ptr, wg_list, pbeg, lslot, cmb_rp, cmb_pos;
   std::vector<uint16_t> lidx;
 };
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
@@ -94,7 +98,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
 
 struct BlkDev {
   const uint4 *units;
-  const uint32_t *wg_unit;
+  const uint32_t *wg_ptr;
+  const uint32_t *wg_list;
   const uint32_t *pbeg;
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
@@ -104,7 +109,7 @@ struct BlkDev {
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p) {
-  return BlkDev{p.units, p.wg_unit, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
+  return BlkDev{p.units, p.wg_ptr, p.wg_list, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
 }
 
 // Sum of sv(lidx[k]) for k = beg + j, beg + j + G, ... < end (lidx and the
@@ -138,6 +143,19 @@ __device__ __forceinline__ float blk_group_sum(float v) {
   return v;
 }
 
+// Pieces of a unit summed G lanes each (G > 1); lane 0 of the group stores
+// the partial.
+template <int G, class SV>
+__device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const uint16_t *s_lidx,
+                                           const uint32_t *s_pbeg, uint32_t np, uint32_t q0) {
+  const uint32_t j = threadIdx.x % G;
+  for (uint32_t q = threadIdx.x / G; q < np; q += kBlkThreads / G) {
+    float s = piece_sum<G, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
+    s = blk_group_sum<G>(s);
+    if (j == 0) d.partial[q0 + q] = s;
+  }
+}
+
 // Segment pass of iteration k.  With a control block (ctl != nullptr) it
 // first applies the stop rule of the iteration loop: returns at once when
 // done; stops (sets done) when iteration k - 1 left an L1 residual below
@@ -148,8 +166,8 @@ __device__ __forceinline__ float blk_group_sum(float v) {
 template <class SV>
 __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV sv, const float *__restrict__ src,
                                                                    IterCtl *ctl, uint32_t k) {
-  __shared__ float slice[kSegSize];
-  __shared__ uint16_t s_lidx[kUnitEdges];
+  __shared__ float slice[kSegSize + 4];  // slice[kSegSize] is the zero slot of the padding entries
+  __shared__ __align__(16) uint16_t s_lidx[kUnitEdges];
   __shared__ uint32_t s_pbeg[kUnitPieces + 1];
   if (ctl) {
     if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
@@ -170,22 +188,25 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     }
   }
   const uint32_t tid = threadIdx.x;
-  constexpr uint32_t kLi = kUnitEdges / kBlkThreads, kPi = kUnitPieces / kBlkThreads;
+  if (tid == 0) slice[kSegSize] = 0.f;  // read after the loop's first barrier
+  constexpr uint32_t kLi = kUnitEdges / (8 * kBlkThreads), kPi = kUnitPieces / kBlkThreads;
   // staging data of the next unit, loaded into registers while the current
-  // unit is summed (every load of a thread issued before its LDS stores)
+  // unit is summed (every load of a thread issued before its LDS stores);
+  // entries move 8 at a time (units start on 8-entry boundaries)
   uint4 nu = make_uint4(0, 0, 0, 0);
   uint32_t n_np = 0, n_ne = 0;
-  uint16_t li[kLi];
+  uint4 li[kLi];
   uint32_t pb[kPi + 1];
   auto prefetch = [&](uint32_t u) {
     nu = d.units[u];
     const uint4 nx = d.units[u + 1];
     n_np = nx.z - nu.z;
     n_ne = nx.w - nu.w;
+    const uint4 *lsrc = reinterpret_cast<const uint4 *>(d.lidx + nu.w);
 #pragma unroll
     for (uint32_t j = 0; j < kLi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      li[j] = i < n_ne ? d.lidx[nu.w + i] : (uint16_t)0;
+      li[j] = i * 8 < n_ne ? lsrc[i] : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
     for (uint32_t j = 0; j <= kPi; ++j) {
@@ -194,8 +215,8 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     }
   };
   uint32_t cur = 0xffffffffu;
-  const uint32_t u0 = d.wg_unit[blockIdx.x], u1 = d.wg_unit[blockIdx.x + 1];
-  if (u0 < u1) prefetch(u0);
+  const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
+  if (u0 < u1) prefetch(d.wg_list[u0]);
   for (uint32_t u = u0; u < u1; ++u) {
     const uint4 un = nu;
     const uint32_t np = n_np, ne = n_ne, e0 = un.w, q0 = un.z;
@@ -224,33 +245,34 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
 #pragma unroll
     for (uint32_t j = 0; j < kLi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      if (i < ne) s_lidx[i] = li[j];
+      if (i * 8 < ne) reinterpret_cast<uint4 *>(s_lidx)[i] = li[j];
     }
 #pragma unroll
     for (uint32_t j = 0; j <= kPi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
       if (i <= np) s_pbeg[i] = pb[j] - e0;
     }
-    if (u + 1 < u1) prefetch(u + 1);
+    if (u + 1 < u1) prefetch(d.wg_list[u + 1]);
     __syncthreads();
     sv.bind(slice, e0);
-    if (un.y == 0) {  // one lane per piece (up to 8 entries)
-      for (uint32_t q = tid; q < np; q += kBlkThreads)
-        d.partial[q0 + q] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
-    } else if (un.y == 1) {  // 8 lanes per piece (up to 64)
-      const uint32_t j = tid % 8;
-      for (uint32_t q = tid / 8; q < np; q += kBlkThreads / 8) {
-        float s = piece_sum<8, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
-        s = blk_group_sum<8>(s);
-        if (j == 0) d.partial[q0 + q] = s;
-      }
-    } else {  // one wave per piece (up to kPieceMax)
-      const uint32_t j = tid % kWave;
-      for (uint32_t q = tid / kWave; q < np; q += kBlkThreads / kWave) {
-        float s = piece_sum<kWave, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
-        s = wave_sum(s);
-        if (j == 0) d.partial[q0 + q] = s;
-      }
+    // G lanes per piece, about eight entries per lane
+    switch (un.y) {
+      case 0:  // up to 8 entries
+        for (uint32_t q = tid; q < np; q += kBlkThreads)
+          d.partial[q0 + q] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
+        break;
+      case 1:
+        blk_pieces<2>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 2:
+        blk_pieces<4>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 3:
+        blk_pieces<8>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      default:  // up to kPieceMax
+        blk_pieces<kWave>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
     }
   }
 }
