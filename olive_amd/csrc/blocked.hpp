@@ -136,10 +136,28 @@ __device__ __forceinline__ float piece_sum(const SV &sv, const uint16_t *lidx, u
   return acc[0];
 }
 
+template <int Ctrl>
+__device__ __forceinline__ float blk_dpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, 0xf, 0xf, false));
+}
+
+// Sum over aligned groups of G lanes with DPP moves inside rows of 16 lanes
+// (no LDS round trip) and, for the whole wave, the four row sums read by
+// lane; every lane of a group ends with the same sum.
 template <int G>
 __device__ __forceinline__ float blk_group_sum(float v) {
-#pragma unroll
-  for (int o = G / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  static_assert(G == 2 || G == 4 || G == 8 || G == 16 || G == kWave, "group of 2, 4, 8, 16 or 64 lanes");
+  v += blk_dpp<0xb1>(v);                          // quad_perm [1,0,3,2]: lane ^ 1
+  if constexpr (G >= 4) v += blk_dpp<0x4e>(v);    // quad_perm [2,3,0,1]: lane ^ 2
+  if constexpr (G >= 8) v += blk_dpp<0x141>(v);   // row_half_mirror: lane 7 - i of 8
+  if constexpr (G >= 16) v += blk_dpp<0x140>(v);  // row_mirror: lane 15 - i of 16
+  if constexpr (G == kWave) {
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 47));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    v = (r0 + r1) + (r2 + r3);
+  }
   return v;
 }
 
@@ -278,7 +296,8 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
 }
 
 // Combine of iteration k: row v's partials added in a fixed order, then
-// fin.finish(v, sum, err); the residual goes to errs[k % 3] (ctl != nullptr).
+// fin.finish(v, sum, err, fin.pre(v)) (pre: the row's other inputs, loaded
+// ahead); the residual goes to errs[k % 3] (ctl != nullptr).
 // Rows [0, n_heavy) go a wave each (lane j adds partials j, j + 64, ...,
 // then a shuffle tree), in the first `heavy_blocks` workgroups; the others a
 // lane each, in segment order, in the remaining workgroups.  Rows without
@@ -298,25 +317,26 @@ __global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, Ite
       float s = 0.f;
       for (uint32_t q = a + lane; q < b; q += kWave) s += d.partial[d.cmb_pos[q]];
       s = wave_sum(s);
-      if (lane == 0) fin.finish(v, s, err);
+      if (lane == 0) fin.finish(v, s, err, fin.pre(v));
     }
   }
   const uint32_t lb = blockIdx.x - heavy_blocks, nlb = gridDim.x - heavy_blocks;
   for (uint32_t v = d.n_heavy + lb * 256 + threadIdx.x; blockIdx.x >= heavy_blocks && v < nv; v += nlb * 256) {
     const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+    const auto pre = fin.pre(v);
+    // at most kCombineLane partials (heavier rows lie below n_heavy): every
+    // load in flight at once, then added in segment order
+    uint32_t ix[kCombineLane];
+#pragma unroll
+    for (uint32_t i = 0; i < kCombineLane; ++i) ix[i] = a + i < b ? d.cmb_pos[a + i] : 0u;
+    float p[kCombineLane];
+#pragma unroll
+    for (uint32_t i = 0; i < kCombineLane; ++i) p[i] = a + i < b ? d.partial[ix[i]] : 0.f;
     float s = 0.f;
-    for (uint32_t q = a; q < b; q += 8) {  // eight loads in flight, added in segment order
-      uint32_t ix[8];
 #pragma unroll
-      for (uint32_t i = 0; i < 8; ++i) ix[i] = q + i < b ? d.cmb_pos[q + i] : 0u;
-      float p[8];
-#pragma unroll
-      for (uint32_t i = 0; i < 8; ++i) p[i] = q + i < b ? d.partial[ix[i]] : 0.f;
-#pragma unroll
-      for (uint32_t i = 0; i < 8; ++i)
-        if (q + i < b) s += p[i];
-    }
-    fin.finish(v, s, err);
+    for (uint32_t i = 0; i < kCombineLane; ++i)
+      if (a + i < b) s += p[i];
+    fin.finish(v, s, err, pre);
   }
   if (!ctl) return;
   err = block_sum<256, float>(err, s_red);

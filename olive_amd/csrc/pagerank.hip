@@ -262,11 +262,16 @@ struct PrFin {
   float *r;
   const float *inv_outdeg;
   float d, base;
-  __device__ __forceinline__ void finish(uint32_t v, float s, float &err) const {
+  struct Pre {
+    float r, inv;
+  };
+  // the row's own inputs, loaded before its partials arrive
+  __device__ __forceinline__ Pre pre(uint32_t v) const { return Pre{r[v], inv_outdeg[v]}; }
+  __device__ __forceinline__ void finish(uint32_t v, float s, float &err, const Pre &p) const {
     const float rn = d * s + base;
-    err += fabsf(rn - r[v]);
+    err += fabsf(rn - p.r);
     r[v] = rn;
-    contrib_next[v] = rn * inv_outdeg[v];
+    contrib_next[v] = rn * p.inv;
   }
 };
 
