@@ -114,7 +114,8 @@ inline BlkDev make_blk_dev(const BlockedPlan &p) {
 
 // Sum of sv(lidx[k]) for k = beg + j, beg + j + G, ... < end (lidx and the
 // slice in LDS) with U reads in flight per lane, U interleaved accumulators
-// added pairwise.
+// added pairwise.  Lanes past the end read the slice's zero slot (kSegSize)
+// instead of branching, so the U slice reads of a step issue together.
 template <int G, int U, class SV>
 __device__ __forceinline__ float piece_sum(const SV &sv, const uint16_t *lidx, uint32_t beg, uint32_t end,
                                            uint32_t j) {
@@ -124,10 +125,9 @@ __device__ __forceinline__ float piece_sum(const SV &sv, const uint16_t *lidx, u
   for (uint32_t k = beg + j; k < end; k += (uint32_t)U * G) {
     uint32_t ix[U];
 #pragma unroll
-    for (int i = 0; i < U; ++i) ix[i] = k + (uint32_t)i * G < end ? (uint32_t)lidx[k + (uint32_t)i * G] : 0u;
+    for (int i = 0; i < U; ++i) ix[i] = k + (uint32_t)i * G < end ? (uint32_t)lidx[k + (uint32_t)i * G] : kSegSize;
 #pragma unroll
-    for (int i = 0; i < U; ++i)
-      if (k + (uint32_t)i * G < end) acc[i] += sv(k + (uint32_t)i * G, ix[i]);
+    for (int i = 0; i < U; ++i) acc[i] += sv(k + (uint32_t)i * G, ix[i]);
   }
 #pragma unroll
   for (int w = U / 2; w > 0; w >>= 1)
@@ -180,7 +180,9 @@ __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const 
 // eps; workgroup 0 clears the residual slots of iteration k + 1.
 // SV: void bind(const float *slice, uint32_t entry0) and
 // float operator()(uint32_t k, uint32_t i) const -- the summand of the
-// unit's entry k (entry0 + k in the plan) whose source is slice[i].
+// unit's entry k (entry0 + k in the plan) whose source is slice[i]; it is
+// also called with i = kSegSize (the zero slot) for lanes past a piece's
+// end, and must then return +0 without reading per-entry data.
 template <class SV>
 __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV sv, const float *__restrict__ src,
                                                                    IterCtl *ctl, uint32_t k) {
