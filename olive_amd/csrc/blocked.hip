@@ -21,11 +21,16 @@ int upload_vec(T **dst, const std::vector<T> &src) {
 
 // Lanes per piece by length: about eight entries per lane (1, 2, 4, 8
 // lanes up to 8, 16, 32, 64 entries; a wave up to kPieceMax).
-inline uint32_t piece_bucket(uint32_t len) {
-  return len <= 8 ? 0u : len <= 16 ? 1u : len <= 32 ? 2u : len <= 64 ? 3u : 4u;
-}
 constexpr uint32_t kBucketLanes[5] = {1, 2, 4, 8, 64};
 constexpr int kBuckets = 5;
+static_assert(kPieceMax <= kStepReads * kBucketLanes[kBuckets - 1], "the widest bucket covers a piece");
+// the fewest lanes that cover the piece with kStepReads reads each (the
+// segment pass sums every piece in one step)
+inline uint32_t piece_bucket(uint32_t len) {
+  uint32_t b = 0;
+  while (b + 1 < kBuckets && len > kStepReads * kBucketLanes[b]) ++b;
+  return b;
+}
 
 }  // namespace
 

@@ -112,25 +112,29 @@ inline BlkDev make_blk_dev(const BlockedPlan &p) {
   return BlkDev{p.units, p.wg_ptr, p.wg_list, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
 }
 
-// Sum of sv(lidx[k]) for k = beg + j, beg + j + G, ... < end (lidx and the
-// slice in LDS) with U reads in flight per lane, U interleaved accumulators
-// added pairwise.  Lanes past the end read the slice's zero slot (kSegSize)
-// instead of branching, so the U slice reads of a step issue together.
-template <int G, int U, class SV>
-__device__ __forceinline__ float piece_sum(const SV &sv, const uint16_t *lidx, uint32_t beg, uint32_t end,
-                                           uint32_t j) {
-  float acc[U];
+// Lane j's share of a piece of a unit whose pieces take G lanes each: the
+// sum of sv(lidx[k]) over k = beg + j + i G, i < kStepReads (lidx and the
+// slice in LDS), all reads in flight at once and the terms added pairwise.
+// Such a piece has at most kStepReads G entries (the length buckets), so
+// this covers it; the k past its end (up to `end`, the unit's alignment
+// padding, or beyond) read the slice's zero slot (kSegSize) instead of
+// branching.
+constexpr int kStepReads = 8;
+static_assert(kPieceMax <= kStepReads * kWave, "a piece fits one step of a wave");
+template <int G, class SV>
+__device__ __forceinline__ float piece_step(const SV &sv, const uint16_t *lidx, uint32_t beg, uint32_t end,
+                                            uint32_t j) {
+  uint32_t ix[kStepReads];
 #pragma unroll
-  for (int i = 0; i < U; ++i) acc[i] = 0.f;
-  for (uint32_t k = beg + j; k < end; k += (uint32_t)U * G) {
-    uint32_t ix[U];
-#pragma unroll
-    for (int i = 0; i < U; ++i) ix[i] = k + (uint32_t)i * G < end ? (uint32_t)lidx[k + (uint32_t)i * G] : kSegSize;
-#pragma unroll
-    for (int i = 0; i < U; ++i) acc[i] += sv(k + (uint32_t)i * G, ix[i]);
+  for (int i = 0; i < kStepReads; ++i) {
+    const uint32_t k = beg + j + (uint32_t)i * G;
+    ix[i] = k < end ? (uint32_t)lidx[k] : kSegSize;
   }
+  float acc[kStepReads];
 #pragma unroll
-  for (int w = U / 2; w > 0; w >>= 1)
+  for (int i = 0; i < kStepReads; ++i) acc[i] = sv(beg + j + (uint32_t)i * G, ix[i]);
+#pragma unroll
+  for (int w = kStepReads / 2; w > 0; w >>= 1)
 #pragma unroll
     for (int i = 0; i < w; ++i) acc[i] += acc[i + w];
   return acc[0];
@@ -146,7 +150,8 @@ __device__ __forceinline__ float blk_dpp(float x) {
 // lane; every lane of a group ends with the same sum.
 template <int G>
 __device__ __forceinline__ float blk_group_sum(float v) {
-  static_assert(G == 2 || G == 4 || G == 8 || G == 16 || G == kWave, "group of 2, 4, 8, 16 or 64 lanes");
+  static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == kWave, "group of 1 to 16 or 64 lanes");
+  if constexpr (G == 1) return v;
   v += blk_dpp<0xb1>(v);                          // quad_perm [1,0,3,2]: lane ^ 1
   if constexpr (G >= 4) v += blk_dpp<0x4e>(v);    // quad_perm [2,3,0,1]: lane ^ 2
   if constexpr (G >= 8) v += blk_dpp<0x141>(v);   // row_half_mirror: lane 7 - i of 8
@@ -161,15 +166,14 @@ __device__ __forceinline__ float blk_group_sum(float v) {
   return v;
 }
 
-// Pieces of a unit summed G lanes each (G > 1); lane 0 of the group stores
-// the partial.
+// Pieces of a unit summed G lanes each; lane 0 of the group stores the
+// partial.
 template <int G, class SV>
 __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const uint16_t *s_lidx,
                                            const uint32_t *s_pbeg, uint32_t np, uint32_t q0) {
   const uint32_t j = threadIdx.x % G;
   for (uint32_t q = threadIdx.x / G; q < np; q += kBlkThreads / G) {
-    float s = piece_sum<G, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j);
-    s = blk_group_sum<G>(s);
+    const float s = blk_group_sum<G>(piece_step<G>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j));
     if (j == 0) d.partial[q0 + q] = s;
   }
 }
@@ -275,11 +279,10 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     if (u + 1 < u1) prefetch(d.wg_list[u + 1]);
     __syncthreads();
     sv.bind(slice, e0);
-    // G lanes per piece, about eight entries per lane
+    // G lanes per piece, at most kStepReads entries per lane
     switch (un.y) {
       case 0:  // up to 8 entries
-        for (uint32_t q = tid; q < np; q += kBlkThreads)
-          d.partial[q0 + q] = piece_sum<1, 8>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], 0u);
+        blk_pieces<1>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       case 1:
         blk_pieces<2>(d, sv, s_lidx, s_pbeg, np, q0);
@@ -326,18 +329,19 @@ __global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, Ite
   for (uint32_t v = d.n_heavy + lb * 256 + threadIdx.x; blockIdx.x >= heavy_blocks && v < nv; v += nlb * 256) {
     const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
     const auto pre = fin.pre(v);
-    // at most kCombineLane partials (heavier rows lie below n_heavy): every
-    // load in flight at once, then added in segment order
-    uint32_t ix[kCombineLane];
-#pragma unroll
-    for (uint32_t i = 0; i < kCombineLane; ++i) ix[i] = a + i < b ? d.cmb_pos[a + i] : 0u;
-    float p[kCombineLane];
-#pragma unroll
-    for (uint32_t i = 0; i < kCombineLane; ++i) p[i] = a + i < b ? d.partial[ix[i]] : 0.f;
+    // sixteen loads in flight, added in segment order
     float s = 0.f;
+    for (uint32_t q = a; q < b; q += 16) {
+      uint32_t ix[16];
 #pragma unroll
-    for (uint32_t i = 0; i < kCombineLane; ++i)
-      if (a + i < b) s += p[i];
+      for (uint32_t i = 0; i < 16; ++i) ix[i] = q + i < b ? d.cmb_pos[q + i] : 0u;
+      float p[16];
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i) p[i] = q + i < b ? d.partial[ix[i]] : 0.f;
+#pragma unroll
+      for (uint32_t i = 0; i < 16; ++i)
+        if (q + i < b) s += p[i];
+    }
     fin.finish(v, s, err, pre);
   }
   if (!ctl) return;
