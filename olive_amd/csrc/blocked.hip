@@ -212,6 +212,12 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     }
     h.wg_ptr[ng] = (uint32_t)h.wg_list.size();
   }
+  static_assert(kUnitPieces < 0x10000 && kUnitEdges < 0x10000, "a unit's counts pack into 16 bits each");
+  h.wg_units.resize(h.wg_list.size());
+  for (size_t i = 0; i < h.wg_list.size(); ++i) {
+    const uint4 a = units[h.wg_list[i]], b = units[h.wg_list[i] + 1];
+    h.wg_units[i] = make_uint4(a.x << 3 | a.y, a.z, a.w, (b.z - a.z) | (b.w - a.w) << 16);
+  }
 
   h.npieces = (uint32_t)npieces;
   h.nunits = nunits;
@@ -232,9 +238,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   bp.npieces = h.npieces;
   bp.nunits = h.nunits;
   bp.ngroups = h.ngroups;
-  OLV_TRY(upload_vec(&bp.units, h.units));
   OLV_TRY(upload_vec(&bp.wg_ptr, h.wg_ptr));
-  OLV_TRY(upload_vec(&bp.wg_list, h.wg_list));
+  OLV_TRY(upload_vec(&bp.wg_units, h.wg_units));
   OLV_TRY(upload_vec(&bp.pbeg, h.pbeg));
   OLV_TRY(upload_vec(&bp.lidx, h.lidx));
   if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));

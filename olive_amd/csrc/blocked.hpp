@@ -53,10 +53,10 @@ struct BlockedPlan {
   uint32_t n = 0, m = 0, n_live = 0;  // vertices, entries, rows with entries (a prefix of the rows)
   uint32_t n_heavy = 0;               // rows [0, n_heavy) hold every row with > kCombineLane partials
   uint32_t nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
-  uint4 *units = nullptr;        // nunits + 1: (segment, bucket 0..4 = 1/2/4/8/64 lanes per piece, first
-                                 // piece, first entry); the last is a sentinel (-, -, npieces, m)
-  uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_list
-  uint32_t *wg_list = nullptr;   // nunits: the units of each workgroup, in segment order
+  uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_units
+  uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
+                                 // (segment << 3 | bucket, first piece, first entry,
+                                 // pieces | entries << 16); bucket 0..4 = 1/2/4/8/64 lanes per piece
   uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
   uint16_t *lidx = nullptr;      // source offset within the segment, in piece order; each unit starts
                                  // on an 8-entry boundary; the gaps and 8 spare entries at the end
@@ -67,21 +67,23 @@ struct BlockedPlan {
   float *partial = nullptr;      // npieces
   bool ready = false;
   void release() {
-    void *ps[] = {units, wg_ptr, wg_list, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
+    void *ps[] = {wg_ptr, wg_units, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    units = nullptr;
-    wg_ptr = wg_list = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
+    wg_units = nullptr;
+    wg_ptr = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
     lidx = nullptr;
     partial = nullptr;
     ready = false;
   }
 };
 
-// Host form of the plan (the arrays before upload).
+// Host form of the plan (the arrays before upload).  units: nunits + 1
+// (segment, bucket, first piece, first entry), the last a sentinel (-, -,
+// npieces, entries); wg_list: the units of each workgroup.
 struct BlockedHost {
   uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
-  std::vector<uint4> units;
+  std::vector<uint4> units, wg_units;
   std::vector<uint32_t> wg_ptr, wg_list, pbeg, lslot, cmb_rp, cmb_pos;
   std::vector<uint16_t> lidx;
 };
@@ -97,9 +99,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
                        BlockedPlan &bp);
 
 struct BlkDev {
-  const uint4 *units;
   const uint32_t *wg_ptr;
-  const uint32_t *wg_list;
+  const uint4 *wg_units;
   const uint32_t *pbeg;
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
@@ -109,7 +110,7 @@ struct BlkDev {
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p) {
-  return BlkDev{p.units, p.wg_ptr, p.wg_list, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
+  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the
@@ -212,59 +213,66 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     }
   }
   const uint32_t tid = threadIdx.x;
+  const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
+  if (u0 >= u1) return;
   if (tid == 0) slice[kSegSize] = 0.f;  // read after the loop's first barrier
   constexpr uint32_t kLi = kUnitEdges / (8 * kBlkThreads), kPi = kUnitPieces / kBlkThreads;
-  // staging data of the next unit, loaded into registers while the current
-  // unit is summed (every load of a thread issued before its LDS stores);
-  // entries move 8 at a time (units start on 8-entry boundaries)
-  uint4 nu = make_uint4(0, 0, 0, 0);
-  uint32_t n_np = 0, n_ne = 0;
+  constexpr uint32_t kSl = kSegSize / (4 * kBlkThreads);
+  // The next unit's staged data -- and its segment of the source vector when
+  // that changes -- are loaded into registers while the current unit is
+  // summed, and the descriptor of the unit after it too: no load waits on
+  // another.  Entries move 8 at a time (units start on 8-entry boundaries).
   uint4 li[kLi];
   uint32_t pb[kPi + 1];
-  auto prefetch = [&](uint32_t u) {
-    nu = d.units[u];
-    const uint4 nx = d.units[u + 1];
-    n_np = nx.z - nu.z;
-    n_ne = nx.w - nu.w;
-    const uint4 *lsrc = reinterpret_cast<const uint4 *>(d.lidx + nu.w);
+  float4 sl[kSl];
+  auto load_unit = [&](const uint4 c, uint32_t have) {
+    const uint32_t np = c.w & 0xffffu, ne = c.w >> 16, seg = c.x >> 3;
+    const uint4 *lsrc = reinterpret_cast<const uint4 *>(d.lidx + c.z);
 #pragma unroll
     for (uint32_t j = 0; j < kLi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      li[j] = i * 8 < n_ne ? lsrc[i] : make_uint4(0, 0, 0, 0);
+      li[j] = i * 8 < ne ? lsrc[i] : make_uint4(0, 0, 0, 0);
     }
 #pragma unroll
     for (uint32_t j = 0; j <= kPi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      pb[j] = i <= n_np ? d.pbeg[nu.z + i] : 0u;
+      pb[j] = i <= np ? d.pbeg[c.y + i] : 0u;
     }
-  };
-  uint32_t cur = 0xffffffffu;
-  const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
-  if (u0 < u1) prefetch(d.wg_list[u0]);
-  for (uint32_t u = u0; u < u1; ++u) {
-    const uint4 un = nu;
-    const uint32_t np = n_np, ne = n_ne, e0 = un.w, q0 = un.z;
-    __syncthreads();  // the previous unit is done with the staged arrays
-    if (un.x != cur) {
-      const uint32_t base = un.x << kSegBits;
-      const uint32_t cnt = min(kSegSize, d.n - base);
-      constexpr uint32_t kSl = kSegSize / (4 * kBlkThreads);
-      float4 v[kSl];
+    if (seg != have) {
+      const uint32_t base = seg << kSegBits, cnt = d.n - base;
+      if (cnt >= kSegSize) {  // uniform: the descriptor is scalar
 #pragma unroll
-      for (uint32_t j = 0; j < kSl; ++j) {
-        const uint32_t i = (tid + j * kBlkThreads) * 4;
-        if (i + 3 < cnt) {
-          v[j] = *reinterpret_cast<const float4 *>(src + base + i);
-        } else {
-          v[j].x = i < cnt ? src[base + i] : 0.f;
-          v[j].y = i + 1 < cnt ? src[base + i + 1] : 0.f;
-          v[j].z = i + 2 < cnt ? src[base + i + 2] : 0.f;
-          v[j].w = 0.f;
+        for (uint32_t j = 0; j < kSl; ++j)
+          sl[j] = *reinterpret_cast<const float4 *>(src + base + (tid + j * kBlkThreads) * 4);
+      } else {  // the last, partial segment
+#pragma unroll
+        for (uint32_t j = 0; j < kSl; ++j) {
+          const uint32_t i = (tid + j * kBlkThreads) * 4;
+          sl[j].x = i < cnt ? src[base + i] : 0.f;
+          sl[j].y = i + 1 < cnt ? src[base + i + 1] : 0.f;
+          sl[j].z = i + 2 < cnt ? src[base + i + 2] : 0.f;
+          sl[j].w = i + 3 < cnt ? src[base + i + 3] : 0.f;
         }
       }
+    }
+  };
+  // descriptors are the same for the whole workgroup: held in scalar registers
+  auto scalar = [](const uint4 c) {
+    return make_uint4(__builtin_amdgcn_readfirstlane(c.x), __builtin_amdgcn_readfirstlane(c.y),
+                      __builtin_amdgcn_readfirstlane(c.z), __builtin_amdgcn_readfirstlane(c.w));
+  };
+  uint4 cd = scalar(d.wg_units[u0]);  // the unit whose data the registers hold
+  load_unit(cd, 0xffffffffu);
+  uint4 nd = u0 + 1 < u1 ? d.wg_units[u0 + 1] : cd;
+  uint32_t cur = 0xffffffffu;
+  for (uint32_t u = u0; u < u1; ++u) {
+    const uint32_t seg = cd.x >> 3, bucket = cd.x & 7u, q0 = cd.y, e0 = cd.z;
+    const uint32_t np = cd.w & 0xffffu, ne = cd.w >> 16;
+    __syncthreads();  // the previous unit is done with the staged arrays
+    if (seg != cur) {
 #pragma unroll
-      for (uint32_t j = 0; j < kSl; ++j) *reinterpret_cast<float4 *>(&slice[(tid + j * kBlkThreads) * 4]) = v[j];
-      cur = un.x;
+      for (uint32_t j = 0; j < kSl; ++j) *reinterpret_cast<float4 *>(&slice[(tid + j * kBlkThreads) * 4]) = sl[j];
+      cur = seg;
     }
 #pragma unroll
     for (uint32_t j = 0; j < kLi; ++j) {
@@ -276,11 +284,15 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       const uint32_t i = tid + j * kBlkThreads;
       if (i <= np) s_pbeg[i] = pb[j] - e0;
     }
-    if (u + 1 < u1) prefetch(d.wg_list[u + 1]);
+    if (u + 1 < u1) {
+      cd = scalar(nd);
+      load_unit(cd, seg);
+      if (u + 2 < u1) nd = d.wg_units[u + 2];
+    }
     __syncthreads();
     sv.bind(slice, e0);
     // G lanes per piece, at most kStepReads entries per lane
-    switch (un.y) {
+    switch (bucket) {
       case 0:  // up to 8 entries
         blk_pieces<1>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
