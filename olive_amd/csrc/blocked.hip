@@ -184,6 +184,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     for (uint32_t u = 0; u < nunits; ++u) byc[u] = u;
     std::stable_sort(byc.begin(), byc.end(), [&](uint32_t a, uint32_t b) { return unit_cost[a] > unit_cost[b]; });
     std::vector<double> load(ng, 0.0);
+    load[0] = 1.5;  // workgroup 0 also applies the stop rule (a residual sum and two barriers)
     std::vector<std::vector<uint32_t>> mine(ng);
     std::vector<uint8_t> has((size_t)ng * nseg, 0);  // workgroup w already loads segment s
     constexpr double kSliceUs = 1.4;

@@ -181,8 +181,10 @@ __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const 
 
 // Segment pass of iteration k.  With a control block (ctl != nullptr) it
 // first applies the stop rule of the iteration loop: returns at once when
-// done; stops (sets done) when iteration k - 1 left an L1 residual below
-// eps; workgroup 0 clears the residual slots of iteration k + 1.
+// done; workgroup 0 alone sets done when iteration k - 1 left an L1 residual
+// below eps (the other workgroups' partials of that pass are then unused:
+// the combine launch returns on done) and clears the residual slots of
+// iteration k + 1.
 // SV: void bind(const float *slice, uint32_t entry0) and
 // float operator()(uint32_t k, uint32_t i) const -- the summand of the
 // unit's entry k (entry0 + k in the plan) whose source is slice[i]; it is
@@ -194,26 +196,27 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   __shared__ float slice[kSegSize + 4];  // slice[kSegSize] is the zero slot of the padding entries
   __shared__ __align__(16) uint16_t s_lidx[kUnitEdges];
   __shared__ uint32_t s_pbeg[kUnitPieces + 1];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
+  const uint4 first = u0 < u1 ? d.wg_units[u0] : make_uint4(0, 0, 0, 0);  // in flight during the checks
   if (ctl) {
     if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
-    if (k > 0) {
-      const double res = err_total(ctl, (k - 1) % 3);
-      if (res < ctl->eps) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
-          ctl->iters = k;
-          ctl->residual = res;
-          __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        return;
-      }
-    }
     if (blockIdx.x == 0) {
+      if (k > 0) {
+        const double res = err_total(ctl, (k - 1) % 3);
+        if (res < ctl->eps) {
+          if (threadIdx.x == 0) {
+            ctl->iters = k;
+            ctl->residual = res;
+            __hip_atomic_store(&ctl->done, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          }
+          return;
+        }
+      }
       double *z = &ctl->errs[(k + 1) % 3][0].v;
       for (uint32_t i = threadIdx.x; i < kErrSlots * (sizeof(ErrSlot) / sizeof(double)); i += kBlkThreads) z[i] = 0.0;
     }
   }
-  const uint32_t tid = threadIdx.x;
-  const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
   if (u0 >= u1) return;
   if (tid == 0) slice[kSegSize] = 0.f;  // read after the loop's first barrier
   constexpr uint32_t kLi = kUnitEdges / (8 * kBlkThreads), kPi = kUnitPieces / kBlkThreads;
@@ -261,7 +264,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     return make_uint4(__builtin_amdgcn_readfirstlane(c.x), __builtin_amdgcn_readfirstlane(c.y),
                       __builtin_amdgcn_readfirstlane(c.z), __builtin_amdgcn_readfirstlane(c.w));
   };
-  uint4 cd = scalar(d.wg_units[u0]);  // the unit whose data the registers hold
+  uint4 cd = scalar(first);  // the unit whose data the registers hold
   load_unit(cd, 0xffffffffu);
   uint4 nd = u0 + 1 < u1 ? d.wg_units[u0 + 1] : cd;
   uint32_t cur = 0xffffffffu;
