@@ -19,10 +19,10 @@ int upload_vec(T **dst, const std::vector<T> &src) {
   return OLV_OK;
 }
 
-// Lanes per piece by length: about eight entries per lane (1, 2, 4, 8
-// lanes up to 8, 16, 32, 64 entries; a wave up to kPieceMax).
-constexpr uint32_t kBucketLanes[5] = {1, 2, 4, 8, 64};
-constexpr int kBuckets = 5;
+// Lanes per piece by length: at most eight entries per lane (1, 2, 4, ...,
+// 32 lanes up to 8, 16, 32, ..., 256 entries; a wave up to kPieceMax).
+constexpr uint32_t kBucketLanes[7] = {1, 2, 4, 8, 16, 32, 64};
+constexpr int kBuckets = 7;
 static_assert(kPieceMax <= kStepReads * kBucketLanes[kBuckets - 1], "the widest bucket covers a piece");
 // the fewest lanes that cover the piece with kStepReads reads each (the
 // segment pass sums every piece in one step)
@@ -130,13 +130,13 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
         if ((uint64_t)edge_cursor + 7 >= 0xffffffffull) return fail(OLV_ERANGE, "blocked plan: entries overflow");
         edge_cursor = (edge_cursor + 7) & ~7u;
         // cost in us, fitted to per-unit timestamps of RMAT-20 iterations
-        // on MI355X: staging and the barriers around it ~2.4, then about
-        // 0.5-0.65 per step (one piece per group of G lanes) and 0.04 per
+        // on MI355X: staging and the barriers around it ~1.2, then
+        // kStepUs[b] per step (one piece per group of G lanes) and 0.05 per
         // thousand entries
+        constexpr double kStepUs[kBuckets] = {0.38, 0.21, 0.25, 0.28, 0.32, 0.38, 0.49};
         const uint32_t per_step = kBlkThreads / kBucketLanes[b];
         const uint32_t steps = (pieces + per_step - 1) / per_step;
-        const double step_us = 0.465 + 0.047 * b;
-        unit_cost.push_back(2.8 + step_us * steps + 4e-5 * edges);
+        unit_cost.push_back(1.2 + kStepUs[b] * steps + 5e-5 * edges);
       }
     }
   }
@@ -172,7 +172,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   for (size_t q = 0; q < npieces; ++q) h.cmb_pos[order[q].slot] = (uint32_t)q;
 
   // 4. units to workgroups: longest first, each to the workgroup where it
-  //    ends earliest, counting a slice load (~1.4 us) when the workgroup
+  //    ends earliest, counting a slice load (~0.6 us) when the workgroup
   //    has no unit of that segment yet; each workgroup then runs its units
   //    in segment order.  A sentinel unit closes the list (its first piece
   //    and entry are the totals).
@@ -187,7 +187,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     load[0] = 1.5;  // workgroup 0 also applies the stop rule (a residual sum and two barriers)
     std::vector<std::vector<uint32_t>> mine(ng);
     std::vector<uint8_t> has((size_t)ng * nseg, 0);  // workgroup w already loads segment s
-    constexpr double kSliceUs = 1.4;
+    constexpr double kSliceUs = 0.6;
     for (uint32_t u : byc) {
       const uint32_t sg = units[u].x;
       uint32_t best = 0;

@@ -56,7 +56,7 @@ struct BlockedPlan {
   uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_units
   uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
                                  // (segment << 3 | bucket, first piece, first entry,
-                                 // pieces | entries << 16); bucket 0..4 = 1/2/4/8/64 lanes per piece
+                                 // pieces | entries << 16); bucket 0..6 = 1/2/4/.../64 lanes per piece
   uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
   uint16_t *lidx = nullptr;      // source offset within the segment, in piece order; each unit starts
                                  // on an 8-entry boundary; the gaps and 8 spare entries at the end
@@ -141,22 +141,26 @@ __device__ __forceinline__ float piece_step(const SV &sv, const uint16_t *lidx, 
   return acc[0];
 }
 
-template <int Ctrl>
+// DPP move of x (rows of 16 lanes outside RowMask receive 0)
+template <int Ctrl, int RowMask = 0xf>
 __device__ __forceinline__ float blk_dpp(float x) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, 0xf, 0xf, false));
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), Ctrl, RowMask, 0xf, false));
 }
 
-// Sum over aligned groups of G lanes with DPP moves inside rows of 16 lanes
-// (no LDS round trip) and, for the whole wave, the four row sums read by
-// lane; every lane of a group ends with the same sum.
+// Sum over aligned groups of G lanes with DPP moves (no LDS round trip):
+// inside rows of 16 lanes every lane of the group ends with the sum; 32
+// lanes add the lower row's sum into the upper row (row_bcast:15), so there
+// the upper row holds it; for the whole wave the four row sums are read by
+// lane and every lane holds the sum.  Lane G - 1 of a group always does.
 template <int G>
 __device__ __forceinline__ float blk_group_sum(float v) {
-  static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == kWave, "group of 1 to 16 or 64 lanes");
+  static_assert(G == 1 || G == 2 || G == 4 || G == 8 || G == 16 || G == 32 || G == kWave, "group of 2^i lanes");
   if constexpr (G == 1) return v;
   v += blk_dpp<0xb1>(v);                          // quad_perm [1,0,3,2]: lane ^ 1
   if constexpr (G >= 4) v += blk_dpp<0x4e>(v);    // quad_perm [2,3,0,1]: lane ^ 2
   if constexpr (G >= 8) v += blk_dpp<0x141>(v);   // row_half_mirror: lane 7 - i of 8
   if constexpr (G >= 16) v += blk_dpp<0x140>(v);  // row_mirror: lane 15 - i of 16
+  if constexpr (G == 32) v += blk_dpp<0x142, 0xa>(v);  // row_bcast:15 into rows 1 and 3
   if constexpr (G == kWave) {
     const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 15));
     const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 31));
@@ -167,7 +171,7 @@ __device__ __forceinline__ float blk_group_sum(float v) {
   return v;
 }
 
-// Pieces of a unit summed G lanes each; lane 0 of the group stores the
+// Pieces of a unit summed G lanes each; lane G - 1 of the group stores the
 // partial.
 template <int G, class SV>
 __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const uint16_t *s_lidx,
@@ -175,7 +179,7 @@ __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const 
   const uint32_t j = threadIdx.x % G;
   for (uint32_t q = threadIdx.x / G; q < np; q += kBlkThreads / G) {
     const float s = blk_group_sum<G>(piece_step<G>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j));
-    if (j == 0) d.partial[q0 + q] = s;
+    if (j == G - 1) d.partial[q0 + q] = s;
   }
 }
 
@@ -307,6 +311,12 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
         break;
       case 3:
         blk_pieces<8>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 4:
+        blk_pieces<16>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 5:
+        blk_pieces<32>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       default:  // up to kPieceMax
         blk_pieces<kWave>(d, sv, s_lidx, s_pbeg, np, q0);
