@@ -61,8 +61,9 @@ enum {
                                         pull adds its terms in another fixed
                                         order)                                   */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
-#define OLV_PR_ROW_PULL      0x0400u /* olv_pagerank_f32: one pull per row over the
-                                        renumbered in-CSR instead of the
+#define OLV_PR_ROW_PULL      0x0400u /* olv_pagerank_f32 and the weighted
+                                        forward, JVP and VJP: one pull per row
+                                        over the renumbered CSR instead of the
                                         source-segmented pull (kept to compare) */
 #define OLV_W_UNCHANGED      0x0200u /* weighted PageRank, VJP, JVP: the weights
                                         at this address are the ones an earlier

@@ -35,7 +35,9 @@ inline uint32_t piece_bucket(uint32_t len) {
 }  // namespace
 
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
-                       BlockedHost &h) {
+                       uint32_t unit_edges, BlockedHost &h) {
+  if (unit_edges < kPieceMax || unit_edges > kUnitEdges || unit_edges % 8)
+    return fail(OLV_EINTERNAL, "blocked plan: %u entries per unit", unit_edges);
   const uint32_t n = (uint32_t)(rp.size() - 1);
   const uint32_t m = rp[n];
   h.n = n;
@@ -111,14 +113,14 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
       p = e;
     }
     for (int b = kBuckets - 1; b >= 0; --b) {
-      // units of at most kUnitEdges entries and kUnitPieces pieces over this
+      // units of at most unit_edges entries and kUnitPieces pieces over this
       // (segment, bucket) group
       size_t i = 0;
       while (i < byb[b].size()) {
         const uint32_t first = (uint32_t)order.size();
         uint32_t edges = 0, pieces = 0;
         while (i < byb[b].size() &&
-               (pieces == 0 || (pieces < kUnitPieces && edges + byb[b][i].len <= kUnitEdges))) {
+               (pieces == 0 || (pieces < kUnitPieces && edges + byb[b][i].len <= unit_edges))) {
           edges += byb[b][i].len;
           ++pieces;
           order.push_back(byb[b][i++]);
@@ -150,7 +152,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   const uint32_t o = edge_cursor;
   pbeg.assign(npieces + 1, 0);
   lidx.assign((size_t)o + 8, (uint16_t)kSegSize);
-  lslot.assign(with_slots ? (size_t)o + 8 : 0, 0);
+  lslot.assign(with_slots ? (size_t)o + 8 : 0, kNoSlot);
   for (size_t un = 0; un < units.size(); ++un) {
     uint32_t e = units[un].w;
     const uint32_t qe = un + 1 < units.size() ? units[un + 1].z : (uint32_t)npieces;
@@ -221,16 +223,19 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   }
 
   h.npieces = (uint32_t)npieces;
+  h.nent = o + 8;
   h.nunits = nunits;
   h.ngroups = ng;
   return OLV_OK;
 }
 
 int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
-                       BlockedPlan &bp) {
+                       uint32_t unit_edges, BlockedPlan &bp) {
   bp.release();
   BlockedHost h;
-  OLV_TRY(build_blocked_host(rp, col, with_slots, groups, h));
+  OLV_TRY(build_blocked_host(rp, col, with_slots, groups, unit_edges, h));
+  bp.unit_edges = unit_edges;
+  bp.nent = h.nent;
   bp.n = h.n;
   bp.m = h.m;
   bp.n_live = h.n_live;

@@ -45,6 +45,8 @@ static_assert(kSegSize <= 0xffff, "entries hold a slice offset or the zero slot 
 constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 145 KB of LDS)
 constexpr uint32_t kPieceMax = 512;            // longest piece (one wave step); longer ones are cut into chunks
 constexpr uint32_t kUnitEdges = 24576;         // entries per work unit, at most (48 KB staged)
+constexpr uint32_t kUnitEdgesW = 8192;         // with per-entry weights (16 + 32 KB staged)
+constexpr uint32_t kNoSlot = 0xffffffffu;      // lslot of the alignment gaps
 constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
 constexpr uint32_t kCombineLane = 32;          // rows with more partials are combined by a wave
 
@@ -53,6 +55,7 @@ struct BlockedPlan {
   uint32_t n = 0, m = 0, n_live = 0;  // vertices, entries, rows with entries (a prefix of the rows)
   uint32_t n_heavy = 0;               // rows [0, n_heavy) hold every row with > kCombineLane partials
   uint32_t nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
+  uint32_t unit_edges = 0, nent = 0;  // entries per unit at most; entries with gaps and spares
   uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_units
   uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
                                  // (segment << 3 | bucket, first piece, first entry,
@@ -62,6 +65,7 @@ struct BlockedPlan {
                                  // on an 8-entry boundary; the gaps and 8 spare entries at the end
                                  // hold kSegSize (the slice's zero slot)
   uint32_t *lslot = nullptr;     // (weighted solves only) CSR slot each entry came from, aligned as lidx
+                                 // (kNoSlot in the gaps and spares)
   uint32_t *cmb_rp = nullptr;    // n + 1: each row's range in cmb_pos
   uint32_t *cmb_pos = nullptr;   // npieces: partial index of each row's pieces, in segment order
   float *partial = nullptr;      // npieces
@@ -82,21 +86,23 @@ struct BlockedPlan {
 // (segment, bucket, first piece, first entry), the last a sentinel (-, -,
 // npieces, entries); wg_list: the units of each workgroup.
 struct BlockedHost {
-  uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
+  uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0, nent = 0;
   std::vector<uint4> units, wg_units;
   std::vector<uint32_t> wg_ptr, wg_list, pbeg, lslot, cmb_rp, cmb_pos;
   std::vector<uint16_t> lidx;
 };
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
-                       BlockedHost &h);
+                       uint32_t unit_edges, BlockedHost &h);
 
 // Builds the plan for a CSR whose rows are given by host offsets rp (n + 1)
 // and host entries col (m, source ids); rows must be ordered so that the rows
 // without entries come last (n_live = first such row).  with_slots keeps the
 // CSR slot of every entry (to gather per-slot data such as edge weights into
-// piece order).  groups: segment-pass workgroups to balance the units over.
+// piece order).  groups: segment-pass workgroups to balance the units over;
+// unit_edges: entries per unit at most (kUnitEdges, or kUnitEdgesW for a
+// pass that stages per-entry weights).
 int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
-                       BlockedPlan &bp);
+                       uint32_t unit_edges, BlockedPlan &bp);
 
 struct BlkDev {
   const uint32_t *wg_ptr;
@@ -106,11 +112,12 @@ struct BlkDev {
   const uint32_t *cmb_rp;
   const uint32_t *cmb_pos;
   float *partial;
+  const float *wpiece;  // per-entry weights in the plan's entry order (weighted passes)
   uint32_t n, n_live, n_heavy;
 };
 
-inline BlkDev make_blk_dev(const BlockedPlan &p) {
-  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, p.n, p.n_live, p.n_heavy};
+inline BlkDev make_blk_dev(const BlockedPlan &p, const float *wpiece = nullptr) {
+  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, wpiece, p.n, p.n_live, p.n_heavy};
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the
@@ -189,16 +196,21 @@ __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const 
 // below eps (the other workgroups' partials of that pass are then unused:
 // the combine launch returns on done) and clears the residual slots of
 // iteration k + 1.
-// SV: void bind(const float *slice, uint32_t entry0) and
+// SV: void bind(const float *slice, const float *wts) and
 // float operator()(uint32_t k, uint32_t i) const -- the summand of the
-// unit's entry k (entry0 + k in the plan) whose source is slice[i]; it is
-// also called with i = kSegSize (the zero slot) for lanes past a piece's
-// end, and must then return +0 without reading per-entry data.
-template <class SV>
+// unit's entry k whose source is slice[i] (wts: the unit's staged weights,
+// kW only); it is also called with i = kSegSize (the zero slot) and k up to
+// kPieceMax past a piece's end, where slice[i] is 0 and wts[k] finite.
+// kW: units of at most kUnitEdgesW entries (plans built with that limit)
+// whose weights (d.wpiece) are staged in LDS beside their offsets.
+template <class SV, bool kW = false>
 __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV sv, const float *__restrict__ src,
                                                                    IterCtl *ctl, uint32_t k) {
+  constexpr uint32_t kUE = kW ? kUnitEdgesW : kUnitEdges;
+  constexpr uint32_t kWS = kW ? kUnitEdgesW + kPieceMax : 4;  // staged weights and a finite margin
   __shared__ float slice[kSegSize + 4];  // slice[kSegSize] is the zero slot of the padding entries
-  __shared__ __align__(16) uint16_t s_lidx[kUnitEdges];
+  __shared__ __align__(16) uint16_t s_lidx[kUE];
+  __shared__ __align__(16) float s_w[kWS];
   __shared__ uint32_t s_pbeg[kUnitPieces + 1];
   const uint32_t tid = threadIdx.x;
   const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
@@ -223,13 +235,17 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   }
   if (u0 >= u1) return;
   if (tid == 0) slice[kSegSize] = 0.f;  // read after the loop's first barrier
-  constexpr uint32_t kLi = kUnitEdges / (8 * kBlkThreads), kPi = kUnitPieces / kBlkThreads;
+  if constexpr (kW)
+    for (uint32_t i = tid; i < kWS; i += kBlkThreads) s_w[i] = 0.f;  // finite past any unit's weights
+  constexpr uint32_t kLi = kUE / (8 * kBlkThreads), kPi = kUnitPieces / kBlkThreads;
+  constexpr uint32_t kWv = kW ? kUE / (4 * kBlkThreads) : 1;
   constexpr uint32_t kSl = kSegSize / (4 * kBlkThreads);
   // The next unit's staged data -- and its segment of the source vector when
   // that changes -- are loaded into registers while the current unit is
   // summed, and the descriptor of the unit after it too: no load waits on
   // another.  Entries move 8 at a time (units start on 8-entry boundaries).
   uint4 li[kLi];
+  float4 wv[kWv];
   uint32_t pb[kPi + 1];
   float4 sl[kSl];
   auto load_unit = [&](const uint4 c, uint32_t have) {
@@ -239,6 +255,14 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     for (uint32_t j = 0; j < kLi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
       li[j] = i * 8 < ne ? lsrc[i] : make_uint4(0, 0, 0, 0);
+    }
+    if constexpr (kW) {
+      const float4 *wsrc = reinterpret_cast<const float4 *>(d.wpiece + c.z);
+#pragma unroll
+      for (uint32_t j = 0; j < kWv; ++j) {
+        const uint32_t i = tid + j * kBlkThreads;
+        wv[j] = i * 4 < ne ? wsrc[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+      }
     }
 #pragma unroll
     for (uint32_t j = 0; j <= kPi; ++j) {
@@ -286,6 +310,13 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       const uint32_t i = tid + j * kBlkThreads;
       if (i * 8 < ne) reinterpret_cast<uint4 *>(s_lidx)[i] = li[j];
     }
+    if constexpr (kW) {
+#pragma unroll
+      for (uint32_t j = 0; j < kWv; ++j) {
+        const uint32_t i = tid + j * kBlkThreads;
+        if (i * 4 < ne) reinterpret_cast<float4 *>(s_w)[i] = wv[j];
+      }
+    }
 #pragma unroll
     for (uint32_t j = 0; j <= kPi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
@@ -297,7 +328,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       if (u + 2 < u1) nd = d.wg_units[u + 2];
     }
     __syncthreads();
-    sv.bind(slice, e0);
+    sv.bind(slice, s_w);
     // G lanes per piece, at most kStepReads entries per lane
     switch (bucket) {
       case 0:  // up to 8 entries

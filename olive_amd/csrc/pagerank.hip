@@ -54,11 +54,16 @@ struct RelabelSide {
 struct PrRelabel {
   RelabelSide in, out;
   BlockedPlan blk_in;    // source-segmented pull over the renumbered in-CSR
+  // the same for the weighted solves, with each entry's slot: [0] over the
+  // renumbered in-CSR (forward), [1] over the out-CSR (adjoint)
+  BlockedPlan blk_w[2];
   float *inv = nullptr;  // n: 1/outdeg by new number
   void release() {
     in.release();
     out.release();
     blk_in.release();
+    blk_w[0].release();
+    blk_w[1].release();
     if (inv) (void)hipFree(inv);
     inv = nullptr;
   }
@@ -75,6 +80,11 @@ struct PrWork {
   float *wslot[2] = {nullptr, nullptr};  // m floats each
   const float *wslot_src[2] = {nullptr, nullptr};
   const uint32_t *wslot_eid[2] = {nullptr, nullptr};
+  // The same in the entry order of the segmented pulls (rl.blk_w[side]):
+  // w[eid[lslot[e]]] at entry e, 0 in the alignment gaps.
+  float *wpiece[2] = {nullptr, nullptr};  // rl.blk_w[side].nent floats each
+  const float *wpiece_src[2] = {nullptr, nullptr};
+  const uint32_t *wpiece_eid[2] = {nullptr, nullptr};
   IterCtl *ctl = nullptr;
   IterCtl *h_ctl = nullptr;
   double *part = nullptr;  // per-workgroup partial sums (kPartials)
@@ -88,6 +98,8 @@ void free_pr_work(olv_graph *g) {
   for (float *p : w->buf)
     if (p) (void)hipFree(p);
   for (float *p : w->wslot)
+    if (p) (void)hipFree(p);
+  for (float *p : w->wpiece)
     if (p) (void)hipFree(p);
   if (w->ctl) (void)hipFree(w->ctl);
   if (w->h_ctl) (void)hipHostFree(w->h_ctl);
@@ -226,7 +238,22 @@ static int ensure_blocked(olv_graph *g, PrWork *w) {
   std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint32_t>(m, 1));
   OLV_HIP(hipMemcpy(rp.data(), w->rl.in.rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
   if (m) OLV_HIP(hipMemcpy(col.data(), w->rl.in.col, (size_t)m * 4, hipMemcpyDeviceToHost));
-  return build_blocked_plan(rp, col.data(), false, (uint32_t)cu_count(), bp);
+  return build_blocked_plan(rp, col.data(), false, (uint32_t)cu_count(), kUnitEdges, bp);
+}
+
+// The segmented pull plan of a weighted solve over the renumbered in-CSR
+// (side 0) or out-CSR (side 1), with every entry's slot and units small
+// enough to stage their weights (built once per side).
+static int ensure_wblocked(olv_graph *g, PrWork *w, int side) {
+  BlockedPlan &bp = w->rl.blk_w[side];
+  if (bp.ready) return OLV_OK;
+  OLV_TRY(ensure_side(g, w, side == 0, true));
+  const RelabelSide &sd = side == 0 ? w->rl.in : w->rl.out;
+  const uint32_t n = g->n, m = g->m;
+  std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint32_t>(m, 1));
+  OLV_HIP(hipMemcpy(rp.data(), sd.rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
+  if (m) OLV_HIP(hipMemcpy(col.data(), sd.col, (size_t)m * 4, hipMemcpyDeviceToHost));
+  return build_blocked_plan(rp, col.data(), true, (uint32_t)cu_count(), kUnitEdgesW, bp);
 }
 
 // ----------------------------------------------------------------- ops
@@ -252,8 +279,53 @@ struct PrOp {
 // Summand of the segmented pull for plain PageRank: the staged contribution.
 struct SliceVal {
   const float *sl = nullptr;
-  __device__ __forceinline__ void bind(const float *s, uint32_t) { sl = s; }
+  __device__ __forceinline__ void bind(const float *s, const float *) { sl = s; }
   __device__ __forceinline__ float operator()(uint32_t, uint32_t i) const { return sl[i]; }
+};
+
+// Summand of a weighted segmented pull: the entry's staged weight times the
+// staged source value (the iterate for the forward, lambda for the adjoint).
+struct WSliceVal {
+  const float *sl = nullptr, *w = nullptr;
+  __device__ __forceinline__ void bind(const float *s, const float *ws) {
+    sl = s;
+    w = ws;
+  }
+  __device__ __forceinline__ float operator()(uint32_t k, uint32_t i) const { return w[k] * sl[i]; }
+};
+
+// The update of WprOp::finish for the combine of the segmented pull.
+struct WprFin {
+  const float *r_cur;
+  float *r_next;
+  const float *t;
+  float d, omd;
+  struct Pre {
+    float r, t;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t v) const { return Pre{r_cur[v], t[v]}; }
+  __device__ __forceinline__ void finish(uint32_t v, float s, float &err, const Pre &p) const {
+    const float rn = d * s + omd * p.t;
+    err += fabsf(rn - p.r);
+    r_next[v] = rn;
+  }
+};
+
+// The update of AdjOp::finish for the combine of the segmented pull.
+struct AdjFin {
+  const float *l_cur;
+  float *l_next;
+  const float *gbar;
+  float d;
+  struct Pre {
+    float l, g;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t u) const { return Pre{l_cur[u], gbar[u]}; }
+  __device__ __forceinline__ void finish(uint32_t u, float s, float &err, const Pre &p) const {
+    const float ln = p.g + d * s;
+    err += fabsf(ln - p.l);
+    l_next[u] = ln;
+  }
 };
 
 // The update of PrOp::finish for the combine of the segmented pull.
@@ -397,6 +469,34 @@ static int slot_weights(const olv_graph *g, PrWork *pw, int side, const float *w
   return OLV_OK;
 }
 
+// out[e] = w[eid[lslot[e]]], 0 where lslot[e] is kNoSlot
+__global__ void piece_weight_kernel(const float *__restrict__ w, const uint32_t *__restrict__ eid,
+                                    const uint32_t *__restrict__ lslot, uint32_t nent, float *__restrict__ out) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += gridDim.x * blockDim.x) {
+    const uint32_t s = lslot[e];
+    out[e] = s != kNoSlot ? w[eid[s]] : 0.f;
+  }
+}
+
+// The weights w in the entry order of the weighted segmented pull of side 0
+// (in-CSR) or 1 (out-CSR), cached like slot_weights' copies.
+static int piece_weights(const olv_graph *g, PrWork *pw, int side, const float *w, const uint32_t *eid, bool reuse,
+                         bool staged, const float **out) {
+  const BlockedPlan &bp = pw->rl.blk_w[side];
+  if (!pw->wpiece[side]) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&pw->wpiece[side]), (size_t)bp.nent * 4));
+  if (!(reuse && !staged && pw->wpiece_src[side] == w && pw->wpiece_eid[side] == eid)) {
+    hipLaunchKernelGGL(piece_weight_kernel, dim3(ew_grid(bp.nent)), dim3(256), 0, stream(), w, eid, bp.lslot, bp.nent,
+                       pw->wpiece[side]);
+    OLV_HIP(hipGetLastError());
+    pw->wpiece_src[side] = staged ? nullptr : w;
+    pw->wpiece_eid[side] = eid;
+    if (!reuse && pw->wpiece_src[side ^ 1] == w) pw->wpiece_src[side ^ 1] = nullptr;
+  }
+  *out = pw->wpiece[side];
+  (void)g;
+  return OLV_OK;
+}
+
 // y[i] = a * x[idx[i]]
 __global__ void gather_scale_kernel(const float *__restrict__ x, const uint32_t *__restrict__ idx, float a, uint32_t n,
                                     float *__restrict__ y) {
@@ -452,6 +552,24 @@ static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, 
   return OLV_OK;
 }
 
+// One iteration of a segmented pull: the segment pass over `src` with
+// summand SV, then the combine applying `fin`.
+template <class SV, bool kW, class Fin>
+static int enqueue_blocked(const BlockedPlan &bp, const BlkDev &bd, const float *src, const Fin &fin, PrWork *w,
+                           uint32_t k, KernelProfile *prof = nullptr) {
+  // combine: a wave per heavy row in the first workgroups, then a lane per row
+  const unsigned gh = std::min<unsigned>((bp.n_heavy + 3) / 4, (unsigned)cu_count() * 4);
+  const unsigned gc = gh + ew_grid(bp.n);
+  hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
+  if (a) OLV_HIP(hipEventRecord(a, stream()));
+  hipLaunchKernelGGL((blk_segment_kernel<SV, kW>), dim3(bp.ngroups), dim3(kBlkThreads), 0, stream(), bd, SV{}, src,
+                     w->ctl, k);
+  hipLaunchKernelGGL(blk_combine_kernel<Fin>, dim3(gc), dim3(256), 0, stream(), bd, fin, w->ctl, k, gh);
+  if (b) OLV_HIP(hipEventRecord(b, stream()));
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
+}
+
 static int read_ctl(PrWork *w) {
   OLV_HIP(hipMemcpyAsync(w->h_ctl, w->ctl, sizeof(IterCtl), hipMemcpyDeviceToHost, stream()));
   OLV_HIP(hipStreamSynchronize(stream()));
@@ -499,12 +617,14 @@ struct Stage {
   }
 };
 
-// Fixed point r = d * P^T r + (1-d) t over the in-slot weights ws,
-// iterated from r0 = t until the L1 change drops below tol; the result is
-// left in w->buf[res->iters & 1].  t must not be w->buf[0] or w->buf[1].
+// Fixed point r = d * P^T r + (1-d) t over the in-slot weights ws (or,
+// with bp, by the segmented pull of plan bp over the weights wp in its entry
+// order), iterated from r0 = t until the L1 change drops below tol; the
+// result is left in w->buf[res->iters & 1].  t must not be w->buf[0] or
+// w->buf[1].
 static int weighted_solve(const olv_graph *g, const SegDev &d, const uint32_t *rp, const uint32_t *ci, PrWork *w,
                           const float *ws, const float *t, float damping, float tol, uint32_t max_iters, Timer &timer,
-                          IterResult *res) {
+                          IterResult *res, const BlockedPlan *bp = nullptr, const float *wp = nullptr) {
   OLV_HIP(hipMemcpyAsync(w->buf[0], t, (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
   hipLaunchKernelGGL(ctl_init_kernel, dim3(1), dim3(256), 0, stream(), w->ctl, (double)tol);
   const uint32_t group = 8;
@@ -513,8 +633,13 @@ static int weighted_solve(const olv_graph *g, const SegDev &d, const uint32_t *r
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
-      WprOp op{ci, ws, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
-      OLV_TRY(enqueue_iteration(d, rp, op, w, it));
+      if (bp) {
+        const WprFin fin{w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
+        OLV_TRY((enqueue_blocked<WSliceVal, true>(*bp, make_blk_dev(*bp, wp), w->buf[it & 1], fin, w, it)));
+      } else {
+        WprOp op{ci, ws, w->buf[it & 1], w->buf[(it + 1) & 1], t, damping, 1.0f - damping};
+        OLV_TRY(enqueue_iteration(d, rp, op, w, it));
+      }
     }
     timer.stop();
     OLV_TRY(read_ctl(w));
@@ -593,19 +718,9 @@ int olv_pagerank_f32(const olv_graph *cg, float damping, uint32_t max_iters, dou
   if (blocked) {
     const BlockedPlan &bp = w->rl.blk_in;
     const BlkDev bd = make_blk_dev(bp);
-    // combine: a wave per heavy row in the first workgroups, then a lane
-    // per row
-    const unsigned gh = std::min<unsigned>((bp.n_heavy + 3) / 4, (unsigned)cu_count() * 4);
-    const unsigned gc = gh + ew_grid(g->n);
-    for (uint32_t it = 0; it < max_iters; ++it) {
-      hipEvent_t a = prof.next(), b = prof.next();
-      if (a) OLV_HIP(hipEventRecord(a, stream()));
-      hipLaunchKernelGGL(blk_segment_kernel<SliceVal>, dim3(bp.ngroups), dim3(kBlkThreads), 0, stream(), bd, SliceVal{},
-                         (const float *)w->buf[it & 1], w->ctl, it);
-      hipLaunchKernelGGL(blk_combine_kernel<PrFin>, dim3(gc), dim3(256), 0, stream(), bd,
-                         PrFin{w->buf[(it + 1) & 1], ri, inv, damping, base}, w->ctl, it, gh);
-      if (b) OLV_HIP(hipEventRecord(b, stream()));
-    }
+    for (uint32_t it = 0; it < max_iters; ++it)
+      OLV_TRY((enqueue_blocked<SliceVal, false>(bp, bd, w->buf[it & 1], PrFin{w->buf[(it + 1) & 1], ri, inv, damping, base},
+                                                w, it, &prof)));
   } else {
     for (uint32_t it = 0; it < max_iters; ++it) {
       PrOp op{ci, w->buf[it & 1], w->buf[(it + 1) & 1], ri, inv, damping, base};
@@ -646,6 +761,8 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
   OLV_TRY(get_pr_work(g, &w));
   PullView pv;
   OLV_TRY(pull_view(g, w, flags, &pv));
+  const bool blocked = pv.relabel && !(flags & OLV_PR_ROW_PULL);
+  if (blocked) OLV_TRY(ensure_wblocked(g, w, 0));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, st_t, so;
@@ -664,10 +781,12 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
     gather_f32(dt, g->rl_order, g->n, tb.as<float>());
     tt = tb.as<float>();
   }
-  const float *ws = nullptr;
-  OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
+  const float *ws = nullptr, *wp = nullptr;
+  if (blocked) OLV_TRY(piece_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+  else OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
-  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, tt, damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, tt, damping, tol, max_iters, t, &res,
+                         blocked ? &w->rl.blk_w[0] : nullptr, wp));
   const double ms = t.ms();
   const uint32_t iters = res.iters;
   if (pv.relabel) gather_f32(w->buf[iters & 1], g->rl_pi, g->n, r);
@@ -700,6 +819,8 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   OLV_TRY(get_pr_work(g, &w));
   PullView pv;
   OLV_TRY(pull_view(g, w, flags, &pv));
+  const bool blocked = pv.relabel && !(flags & OLV_PR_ROW_PULL);
+  if (blocked) OLV_TRY(ensure_wblocked(g, w, 0));
   const bool host = (flags & OLV_PTR_HOST) != 0;
   int s = OLV_OK;
   Stage sw, sr, sdw, sdt, so;
@@ -737,10 +858,12 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   // dr = d * P^T dr + s, i.e. the forward fixed point with teleport s / (1-d)
   hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), w->buf[2], 1.0f / (1.0f - damping),
                      g->n, w->buf[3]);
-  const float *ws = nullptr;
-  OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
+  const float *ws = nullptr, *wp = nullptr;
+  if (blocked) OLV_TRY(piece_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+  else OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
-  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, w->buf[3], damping, tol, max_iters, t, &res));
+  OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, w->buf[3], damping, tol, max_iters, t, &res,
+                         blocked ? &w->rl.blk_w[0] : nullptr, wp));
   const double ms = t.ms();
   if (pv.relabel) gather_f32(w->buf[res.iters & 1], g->rl_pi, g->n, out);
   else OLV_HIP(hipMemcpyAsync(out, w->buf[res.iters & 1], (size_t)g->n * 4, hipMemcpyDeviceToDevice, stream()));
@@ -770,9 +893,12 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   const bool relabel = (flags & OLV_GRAPH_ORDER) == 0;
   PrWork *w = nullptr;
   OLV_TRY(get_pr_work(g, &w));
-  // the out-CSR the adjoint runs over: the renumbered copy or the graph's own
+  // the out-CSR the adjoint runs over: the renumbered copy (by the segmented
+  // pull unless OLV_PR_ROW_PULL) or the graph's own
+  const bool blocked = relabel && !(flags & OLV_PR_ROW_PULL);
   if (relabel) OLV_TRY(ensure_side(g, w, false, true));
   else OLV_TRY(ensure_plan(g, false));
+  if (blocked) OLV_TRY(ensure_wblocked(g, w, 1));
   const SegDev d = make_seg_dev(relabel ? w->rl.out.plan : g->plan_out, g->tune);
   const uint32_t *rp = relabel ? w->rl.out.rp : g->row_ptr;
   const uint32_t *ci = relabel ? w->rl.out.col : g->col;
@@ -798,9 +924,10 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   }
   Timer t;
   t.start();
-  const float *wv = dw, *rv = dr, *gv = dg;
+  const float *wv = dw, *rv = dr, *gv = dg, *wp = nullptr;
   if (relabel) {
-    OLV_TRY(slot_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wv));
+    if (blocked) OLV_TRY(piece_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+    else OLV_TRY(slot_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wv));
     gather_f32(dr, g->rl_order, g->n, rb.as<float>());
     gather_f32(dg, g->rl_order, g->n, gb.as<float>());
     rv = rb.as<float>();
@@ -822,8 +949,14 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   for (; it < max_iters;) {
     const uint32_t stop = std::min(max_iters, it + group);
     for (; it < stop; ++it) {
-      AdjOp op{ci, wv, lam[it & 1], lam[(it + 1) & 1], gv, damping};
-      OLV_TRY(enqueue_iteration(d, rp, op, w, it));
+      if (blocked) {
+        const BlockedPlan &bp = w->rl.blk_w[1];
+        const AdjFin fin{lam[it & 1], lam[(it + 1) & 1], gv, damping};
+        OLV_TRY((enqueue_blocked<WSliceVal, true>(bp, make_blk_dev(bp, wp), lam[it & 1], fin, w, it)));
+      } else {
+        AdjOp op{ci, wv, lam[it & 1], lam[(it + 1) & 1], gv, damping};
+        OLV_TRY(enqueue_iteration(d, rp, op, w, it));
+      }
     }
     OLV_TRY(read_ctl(w));
     res = iter_result(*w->h_ctl, it);
