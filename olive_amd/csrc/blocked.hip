@@ -3,6 +3,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <thread>
 #include <vector>
 
 #include "blocked.hpp"
@@ -10,6 +11,30 @@
 namespace olv {
 
 namespace {
+
+// Host threads for a pass over `work` items: one per 2^20 items, at most 16
+// and at most the machine's.
+unsigned host_threads(size_t work) {
+  const unsigned hc = std::max(1u, std::thread::hardware_concurrency());
+  return (unsigned)std::min<size_t>({(size_t)16, (size_t)hc, work / (1u << 20) + 1});
+}
+
+// f(t) for t < T, on T - 1 new host threads and the caller (t = 0); a thread
+// that cannot be started runs its share on the caller.
+template <class F>
+void host_parallel(unsigned T, const F &f) {
+  std::vector<std::thread> th;
+  th.reserve(T);
+  for (unsigned t = 1; t < T; ++t) {
+    try {
+      th.emplace_back(f, t);
+    } catch (...) {
+      f(t);
+    }
+  }
+  f(0);
+  for (auto &x : th) x.join();
+}
 
 template <class T>
 int upload_vec(T **dst, const std::vector<T> &src) {
@@ -49,22 +74,51 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   h.nseg = nseg;
 
   // 1. entries stably sorted by the segment of their source: within a
-  //    segment, rows ascending and each row's entries in slot order
+  //    segment, rows ascending and each row's entries in slot order (a
+  //    counting sort over row ranges on host threads; each entry's slot,
+  //    row and 16-bit offset carried along)
   std::vector<uint32_t> soff((size_t)nseg + 1, 0);
-  for (uint32_t k = 0; k < m; ++k) {
-    if (col[k] >= n) return fail(OLV_EINTERNAL, "blocked plan: entry %u out of range", col[k]);
-    soff[(col[k] >> kSegBits) + 1]++;
-  }
-  for (uint32_t s = 0; s < nseg; ++s) soff[s + 1] += soff[s];
   std::vector<uint32_t> perm(m), prow(m);
+  std::vector<uint16_t> loff(m);
   {
-    std::vector<uint32_t> cur(soff.begin(), soff.end() - (nseg ? 1 : 0));
-    for (uint32_t v = 0; v < n; ++v)
-      for (uint32_t k = rp[v]; k < rp[v + 1]; ++k) {
-        const uint32_t pos = cur[col[k] >> kSegBits]++;
-        perm[pos] = k;
-        prow[pos] = v;
+    const unsigned T = host_threads(m);
+    std::vector<uint32_t> rb(T + 1, n);  // row ranges of about m / T entries
+    for (unsigned t = 0; t < T; ++t)
+      rb[t] = (uint32_t)(std::lower_bound(rp.begin(), rp.end() - 1, (uint32_t)((uint64_t)m * t / T)) - rp.begin());
+    std::vector<uint32_t> cur((size_t)T * nseg, 0);  // per range: counts, then cursors
+    std::vector<uint8_t> bad(T, 0);
+    host_parallel(T, [&](unsigned t) {
+      uint32_t *c = cur.data() + (size_t)t * nseg;
+      for (uint32_t k = rp[rb[t]]; k < rp[rb[t + 1]]; ++k) {
+        if (col[k] >= n) {
+          bad[t] = 1;
+          return;
+        }
+        c[col[k] >> kSegBits]++;
       }
+    });
+    for (unsigned t = 0; t < T; ++t)
+      if (bad[t]) return fail(OLV_EINTERNAL, "blocked plan: an entry is out of range (n = %u)", n);
+    uint32_t run = 0;
+    for (uint32_t s = 0; s < nseg; ++s) {
+      soff[s] = run;
+      for (unsigned t = 0; t < T; ++t) {
+        const uint32_t x = cur[(size_t)t * nseg + s];
+        cur[(size_t)t * nseg + s] = run;
+        run += x;
+      }
+    }
+    soff[nseg] = run;
+    host_parallel(T, [&](unsigned t) {
+      uint32_t *c = cur.data() + (size_t)t * nseg;
+      for (uint32_t v = rb[t]; v < rb[t + 1]; ++v)
+        for (uint32_t k = rp[v]; k < rp[v + 1]; ++k) {
+          const uint32_t pos = c[col[k] >> kSegBits]++;
+          perm[pos] = k;
+          prow[pos] = v;
+          loff[pos] = (uint16_t)(col[k] & (kSegSize - 1));
+        }
+    });
   }
 
   // 2. pieces: runs of one row within a segment, cut at kPieceMax entries;
@@ -153,25 +207,33 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   pbeg.assign(npieces + 1, 0);
   lidx.assign((size_t)o + 8, (uint16_t)kSegSize);
   lslot.assign(with_slots ? (size_t)o + 8 : 0, kNoSlot);
-  for (size_t un = 0; un < units.size(); ++un) {
-    uint32_t e = units[un].w;
-    const uint32_t qe = un + 1 < units.size() ? units[un + 1].z : (uint32_t)npieces;
-    for (uint32_t q = units[un].z; q < qe; ++q) {
-      const Piece &pc = order[q];
-      pbeg[q] = e;
-      for (uint32_t t = 0; t < pc.len; ++t) {
-        const uint32_t k = perm[pc.start + t];
-        lidx[e + t] = (uint16_t)(col[k] & (kSegSize - 1));
-        if (with_slots) lslot[e + t] = k;
+  {
+    const unsigned T = host_threads(o);
+    const size_t nu = units.size();
+    host_parallel(T, [&](unsigned t) {  // units' entries are disjoint
+      for (size_t un = nu * t / T; un < nu * (t + 1) / T; ++un) {
+        uint32_t e = units[un].w;
+        const uint32_t qe = un + 1 < nu ? units[un + 1].z : (uint32_t)npieces;
+        for (uint32_t q = units[un].z; q < qe; ++q) {
+          const Piece &pc = order[q];
+          pbeg[q] = e;
+          std::memcpy(&lidx[e], &loff[pc.start], pc.len * sizeof(uint16_t));
+          if (with_slots) std::memcpy(&lslot[e], &perm[pc.start], pc.len * sizeof(uint32_t));
+          e += pc.len;
+        }
       }
-      e += pc.len;
-    }
+    });
   }
   pbeg[npieces] = o;
   // each row's partials in segment order: cmb_pos[cmb_rp[v] + j] is the
   // piece (and partial) index of the row's j-th piece
   h.cmb_pos.assign(npieces, 0);
-  for (size_t q = 0; q < npieces; ++q) h.cmb_pos[order[q].slot] = (uint32_t)q;
+  {
+    const unsigned T = host_threads(npieces);
+    host_parallel(T, [&](unsigned t) {  // slots are a permutation of the pieces
+      for (size_t q = npieces * t / T; q < npieces * (t + 1) / T; ++q) h.cmb_pos[order[q].slot] = (uint32_t)q;
+    });
+  }
 
   // 4. units to workgroups: longest first, each to the workgroup where it
   //    ends earliest, counting a slice load (~0.6 us) when the workgroup

@@ -65,7 +65,7 @@ struct BlockedPlan {
                                  // on an 8-entry boundary; the gaps and 8 spare entries at the end
                                  // hold kSegSize (the slice's zero slot)
   uint32_t *lslot = nullptr;     // (weighted solves only) CSR slot each entry came from, aligned as lidx
-                                 // (kNoSlot in the gaps and spares)
+                                 // (kNoSlot in the gaps and spares); its owner may map it in place
   uint32_t *cmb_rp = nullptr;    // n + 1: each row's range in cmb_pos
   uint32_t *cmb_pos = nullptr;   // npieces: partial index of each row's pieces, in segment order
   float *partial = nullptr;      // npieces

@@ -54,8 +54,9 @@ struct RelabelSide {
 struct PrRelabel {
   RelabelSide in, out;
   BlockedPlan blk_in;    // source-segmented pull over the renumbered in-CSR
-  // the same for the weighted solves, with each entry's slot: [0] over the
-  // renumbered in-CSR (forward), [1] over the out-CSR (adjoint)
+  // the same for the weighted solves: [0] over the renumbered in-CSR
+  // (forward), [1] over the out-CSR (adjoint); their lslot holds each
+  // entry's edge id (the slot mapped through the side's eid when built)
   BlockedPlan blk_w[2];
   float *inv = nullptr;  // n: 1/outdeg by new number
   void release() {
@@ -241,8 +242,16 @@ static int ensure_blocked(olv_graph *g, PrWork *w) {
   return build_blocked_plan(rp, col.data(), false, (uint32_t)cu_count(), kUnitEdges, bp);
 }
 
+// lslot[e] <- eid[lslot[e]] where lslot[e] is a slot
+__global__ void slot_to_edge_kernel(uint32_t *__restrict__ lslot, const uint32_t *__restrict__ eid, uint32_t nent) {
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += gridDim.x * blockDim.x) {
+    const uint32_t s = lslot[e];
+    if (s != kNoSlot) lslot[e] = eid[s];
+  }
+}
+
 // The segmented pull plan of a weighted solve over the renumbered in-CSR
-// (side 0) or out-CSR (side 1), with every entry's slot and units small
+// (side 0) or out-CSR (side 1), with every entry's edge id and units small
 // enough to stage their weights (built once per side).
 static int ensure_wblocked(olv_graph *g, PrWork *w, int side) {
   BlockedPlan &bp = w->rl.blk_w[side];
@@ -253,7 +262,10 @@ static int ensure_wblocked(olv_graph *g, PrWork *w, int side) {
   std::vector<uint32_t> rp((size_t)n + 1), col(std::max<uint32_t>(m, 1));
   OLV_HIP(hipMemcpy(rp.data(), sd.rp, ((size_t)n + 1) * 4, hipMemcpyDeviceToHost));
   if (m) OLV_HIP(hipMemcpy(col.data(), sd.col, (size_t)m * 4, hipMemcpyDeviceToHost));
-  return build_blocked_plan(rp, col.data(), true, (uint32_t)cu_count(), kUnitEdgesW, bp);
+  OLV_TRY(build_blocked_plan(rp, col.data(), true, (uint32_t)cu_count(), kUnitEdgesW, bp));
+  hipLaunchKernelGGL(slot_to_edge_kernel, dim3(ew_grid(bp.nent)), dim3(256), 0, stream(), bp.lslot, sd.eid, bp.nent);
+  OLV_HIP(hipGetLastError());
+  return OLV_OK;
 }
 
 // ----------------------------------------------------------------- ops
@@ -469,12 +481,12 @@ static int slot_weights(const olv_graph *g, PrWork *pw, int side, const float *w
   return OLV_OK;
 }
 
-// out[e] = w[eid[lslot[e]]], 0 where lslot[e] is kNoSlot
-__global__ void piece_weight_kernel(const float *__restrict__ w, const uint32_t *__restrict__ eid,
-                                    const uint32_t *__restrict__ lslot, uint32_t nent, float *__restrict__ out) {
+// out[e] = w[edge[e]], 0 where edge[e] is kNoSlot
+__global__ void piece_weight_kernel(const float *__restrict__ w, const uint32_t *__restrict__ edge, uint32_t nent,
+                                    float *__restrict__ out) {
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < nent; e += gridDim.x * blockDim.x) {
-    const uint32_t s = lslot[e];
-    out[e] = s != kNoSlot ? w[eid[s]] : 0.f;
+    const uint32_t x = edge[e];
+    out[e] = x != kNoSlot ? w[x] : 0.f;
   }
 }
 
@@ -485,7 +497,7 @@ static int piece_weights(const olv_graph *g, PrWork *pw, int side, const float *
   const BlockedPlan &bp = pw->rl.blk_w[side];
   if (!pw->wpiece[side]) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&pw->wpiece[side]), (size_t)bp.nent * 4));
   if (!(reuse && !staged && pw->wpiece_src[side] == w && pw->wpiece_eid[side] == eid)) {
-    hipLaunchKernelGGL(piece_weight_kernel, dim3(ew_grid(bp.nent)), dim3(256), 0, stream(), w, eid, bp.lslot, bp.nent,
+    hipLaunchKernelGGL(piece_weight_kernel, dim3(ew_grid(bp.nent)), dim3(256), 0, stream(), w, bp.lslot, bp.nent,
                        pw->wpiece[side]);
     OLV_HIP(hipGetLastError());
     pw->wpiece_src[side] = staged ? nullptr : w;
