@@ -57,9 +57,9 @@ enum {
                                         JVP): run in the graph's own vertex
                                         numbering instead of the in-degree
                                         renumbering (same results, bit for bit,
-                                        except that olv_pagerank_f32's segmented
-                                        pull adds its terms in another fixed
-                                        order)                                   */
+                                        except that the PageRank solves'
+                                        default segmented pulls add their terms
+                                        in another fixed order)                  */
 #define OLV_BUILD_TRANSPOSE  0x0100u /* graph_create: also build the in-edge CSR    */
 #define OLV_PR_ROW_PULL      0x0400u /* olv_pagerank_f32 and the weighted
                                         forward, JVP and VJP: one pull per row
