@@ -58,7 +58,7 @@ def _weights_flag(graph, w):
     """OLV_W_UNCHANGED when `w` is the tensor object the previous weighted
     call on this graph received, at the same address, and torch's version
     counter shows no in-place change since: the library may then reuse the
-    copy of the weights it already put in its solvers' slot order.  Writes
+    copy of the weights it already put in its solvers' order.  Writes
     that bypass torch (raw pointers) are not seen, as with any autograd
     version check."""
     last = getattr(graph, "_w_seen", None)
