@@ -201,10 +201,10 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   //    entry.  The last piece of a unit runs up to the next unit's first
   //    entry: the gap holds kSegSize, the staged slice's zero slot, as do 8
   //    spare entries at the end for the staging's 16-byte reads.
-  std::vector<uint32_t> &pbeg = h.pbeg, &lslot = h.lslot;
-  std::vector<uint16_t> &lidx = h.lidx;
+  std::vector<uint32_t> &lslot = h.lslot;
+  std::vector<uint16_t> &pbeg = h.pbeg, &lidx = h.lidx;
   const uint32_t o = edge_cursor;
-  pbeg.assign(npieces + 1, 0);
+  pbeg.assign(npieces, 0);
   lidx.assign((size_t)o + 8, (uint16_t)kSegSize);
   lslot.assign(with_slots ? (size_t)o + 8 : 0, kNoSlot);
   {
@@ -212,11 +212,12 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     const size_t nu = units.size();
     host_parallel(T, [&](unsigned t) {  // units' entries are disjoint
       for (size_t un = nu * t / T; un < nu * (t + 1) / T; ++un) {
-        uint32_t e = units[un].w;
+        const uint32_t e0 = units[un].w;
+        uint32_t e = e0;
         const uint32_t qe = un + 1 < nu ? units[un + 1].z : (uint32_t)npieces;
         for (uint32_t q = units[un].z; q < qe; ++q) {
           const Piece &pc = order[q];
-          pbeg[q] = e;
+          pbeg[q] = (uint16_t)(e - e0);
           std::memcpy(&lidx[e], &loff[pc.start], pc.len * sizeof(uint16_t));
           if (with_slots) std::memcpy(&lslot[e], &perm[pc.start], pc.len * sizeof(uint32_t));
           e += pc.len;
@@ -224,7 +225,6 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
       }
     });
   }
-  pbeg[npieces] = o;
   // each row's partials in segment order: cmb_pos[cmb_rp[v] + j] is the
   // piece (and partial) index of the row's j-th piece
   h.cmb_pos.assign(npieces, 0);

@@ -44,7 +44,7 @@ constexpr uint32_t kSegSize = 1u << kSegBits;  // sources per LDS slice (64 KB o
 static_assert(kSegSize <= 0xffff, "entries hold a slice offset or the zero slot in 16 bits");
 constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 145 KB of LDS)
 constexpr uint32_t kPieceMax = 512;            // longest piece (one wave step); longer ones are cut into chunks
-constexpr uint32_t kUnitEdges = 24576;         // entries per work unit, at most (48 KB staged)
+constexpr uint32_t kUnitEdges = 32768;         // entries per work unit, at most (64 KB staged)
 constexpr uint32_t kUnitEdgesW = 8192;         // with per-entry weights (16 + 32 KB staged)
 constexpr uint32_t kNoSlot = 0xffffffffu;      // lslot of the alignment gaps
 constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
@@ -60,7 +60,8 @@ struct BlockedPlan {
   uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
                                  // (segment << 3 | bucket, first piece, first entry,
                                  // pieces | entries << 16); bucket 0..6 = 1/2/4/.../64 lanes per piece
-  uint32_t *pbeg = nullptr;      // npieces + 1: entry offsets of the pieces
+  uint16_t *pbeg = nullptr;      // npieces: each piece's first entry, counted from its unit's first
+                                 // entry (a unit's last piece ends at the unit's aligned entry count)
   uint16_t *lidx = nullptr;      // source offset within the segment, in piece order; each unit starts
                                  // on an 8-entry boundary; the gaps and 8 spare entries at the end
                                  // hold kSegSize (the slice's zero slot)
@@ -75,8 +76,8 @@ struct BlockedPlan {
     for (void *p : ps)
       if (p) (void)hipFree(p);
     wg_units = nullptr;
-    wg_ptr = pbeg = lslot = cmb_rp = cmb_pos = nullptr;
-    lidx = nullptr;
+    wg_ptr = lslot = cmb_rp = cmb_pos = nullptr;
+    pbeg = lidx = nullptr;
     partial = nullptr;
     ready = false;
   }
@@ -88,8 +89,8 @@ struct BlockedPlan {
 struct BlockedHost {
   uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0, nent = 0;
   std::vector<uint4> units, wg_units;
-  std::vector<uint32_t> wg_ptr, wg_list, pbeg, lslot, cmb_rp, cmb_pos;
-  std::vector<uint16_t> lidx;
+  std::vector<uint32_t> wg_ptr, wg_list, lslot, cmb_rp, cmb_pos;
+  std::vector<uint16_t> pbeg, lidx;
 };
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
                        uint32_t unit_edges, BlockedHost &h);
@@ -107,7 +108,7 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
 struct BlkDev {
   const uint32_t *wg_ptr;
   const uint4 *wg_units;
-  const uint32_t *pbeg;
+  const uint16_t *pbeg;
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
   const uint32_t *cmb_pos;
@@ -182,7 +183,7 @@ __device__ __forceinline__ float blk_group_sum(float v) {
 // partial.
 template <int G, class SV>
 __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const uint16_t *s_lidx,
-                                           const uint32_t *s_pbeg, uint32_t np, uint32_t q0) {
+                                           const uint16_t *s_pbeg, uint32_t np, uint32_t q0) {
   const uint32_t j = threadIdx.x % G;
   for (uint32_t q = threadIdx.x / G; q < np; q += kBlkThreads / G) {
     const float s = blk_group_sum<G>(piece_step<G>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j));
@@ -211,7 +212,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   __shared__ float slice[kSegSize + 4];  // slice[kSegSize] is the zero slot of the padding entries
   __shared__ __align__(16) uint16_t s_lidx[kUE];
   __shared__ __align__(16) float s_w[kWS];
-  __shared__ uint32_t s_pbeg[kUnitPieces + 1];
+  __shared__ uint16_t s_pbeg[kUnitPieces + 2];  // piece starts and the unit's end
   const uint32_t tid = threadIdx.x;
   const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
   const uint4 first = u0 < u1 ? d.wg_units[u0] : make_uint4(0, 0, 0, 0);  // in flight during the checks
@@ -246,7 +247,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   // another.  Entries move 8 at a time (units start on 8-entry boundaries).
   uint4 li[kLi];
   float4 wv[kWv];
-  uint32_t pb[kPi + 1];
+  uint32_t pb[kPi];
   float4 sl[kSl];
   auto load_unit = [&](const uint4 c, uint32_t have) {
     const uint32_t np = c.w & 0xffffu, ne = c.w >> 16, seg = c.x >> 3;
@@ -265,9 +266,9 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       }
     }
 #pragma unroll
-    for (uint32_t j = 0; j <= kPi; ++j) {
+    for (uint32_t j = 0; j < kPi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      pb[j] = i <= np ? d.pbeg[c.y + i] : 0u;
+      pb[j] = i < np ? d.pbeg[c.y + i] : 0u;
     }
     if (seg != have) {
       const uint32_t base = seg << kSegBits, cnt = d.n - base;
@@ -297,7 +298,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   uint4 nd = u0 + 1 < u1 ? d.wg_units[u0 + 1] : cd;
   uint32_t cur = 0xffffffffu;
   for (uint32_t u = u0; u < u1; ++u) {
-    const uint32_t seg = cd.x >> 3, bucket = cd.x & 7u, q0 = cd.y, e0 = cd.z;
+    const uint32_t seg = cd.x >> 3, bucket = cd.x & 7u, q0 = cd.y;
     const uint32_t np = cd.w & 0xffffu, ne = cd.w >> 16;
     __syncthreads();  // the previous unit is done with the staged arrays
     if (seg != cur) {
@@ -318,10 +319,11 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
       }
     }
 #pragma unroll
-    for (uint32_t j = 0; j <= kPi; ++j) {
+    for (uint32_t j = 0; j < kPi; ++j) {
       const uint32_t i = tid + j * kBlkThreads;
-      if (i <= np) s_pbeg[i] = pb[j] - e0;
+      if (i < np) s_pbeg[i] = (uint16_t)pb[j];
     }
+    if (tid == 0) s_pbeg[np] = (uint16_t)ne;
     if (u + 1 < u1) {
       cd = scalar(nd);
       load_unit(cd, seg);
