@@ -186,13 +186,13 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
         if ((uint64_t)edge_cursor + 7 >= 0xffffffffull) return fail(OLV_ERANGE, "blocked plan: entries overflow");
         edge_cursor = (edge_cursor + 7) & ~7u;
         // cost in us, fitted to per-unit timestamps of RMAT-20 iterations
-        // on MI355X: staging and the barriers around it ~1.2, then
-        // kStepUs[b] per step (one piece per group of G lanes) and 0.05 per
+        // on MI355X: staging and the barriers around it ~1.3, then
+        // kStepUs[b] per step (one piece per group of G lanes) and 0.04 per
         // thousand entries
-        constexpr double kStepUs[kBuckets] = {0.38, 0.21, 0.25, 0.28, 0.32, 0.38, 0.49};
+        constexpr double kStepUs[kBuckets] = {0.41, 0.20, 0.29, 0.37, 0.38, 0.41, 0.37};
         const uint32_t per_step = kBlkThreads / kBucketLanes[b];
         const uint32_t steps = (pieces + per_step - 1) / per_step;
-        unit_cost.push_back(1.2 + kStepUs[b] * steps + 5e-5 * edges);
+        unit_cost.push_back(1.3 + kStepUs[b] * steps + 4e-5 * edges);
       }
     }
   }
@@ -236,7 +236,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   }
 
   // 4. units to workgroups: longest first, each to the workgroup where it
-  //    ends earliest, counting a slice load (~0.6 us) when the workgroup
+  //    ends earliest, counting a slice load (~1 us) when the workgroup
   //    has no unit of that segment yet; each workgroup then runs its units
   //    in segment order.  A sentinel unit closes the list (its first piece
   //    and entry are the totals).
@@ -251,7 +251,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     load[0] = 1.5;  // workgroup 0 also applies the stop rule (a residual sum and two barriers)
     std::vector<std::vector<uint32_t>> mine(ng);
     std::vector<uint8_t> has((size_t)ng * nseg, 0);  // workgroup w already loads segment s
-    constexpr double kSliceUs = 0.6;
+    constexpr double kSliceUs = 1.0;
     for (uint32_t u : byc) {
       const uint32_t sg = units[u].x;
       uint32_t best = 0;
