@@ -141,11 +141,13 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   std::vector<uint32_t> &cmb = h.cmb_rp;
   cmb.assign((size_t)n + 1, 0);
   for (uint32_t v = 0; v < n; ++v) cmb[v + 1] = cmb[v] + npc[v];
-  // rows with many partials are combined by a wave each: the prefix of rows
-  // up to the last one with more than kCombineLane partials
-  h.n_heavy = 0;
-  for (uint32_t v = 0; v < n; ++v)
-    if (npc[v] > kCombineLane) h.n_heavy = v + 1;
+  // rows with more partials than one lane combines, listed by tier
+  h.cmb_mid.clear();
+  h.cmb_big.clear();
+  for (uint32_t v = 0; v < n; ++v) {
+    if (npc[v] > kCmbMid) h.cmb_big.push_back(v);
+    else if (npc[v] > kCmbLight) h.cmb_mid.push_back(v);
+  }
   std::vector<uint32_t> slot_cur(cmb.begin(), cmb.end() - 1);
   std::vector<Piece> order;  // final piece order: segment, then bucket (longest first), then row
   order.reserve(npieces);
@@ -301,7 +303,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   bp.n = h.n;
   bp.m = h.m;
   bp.n_live = h.n_live;
-  bp.n_heavy = h.n_heavy;
+  bp.n_mid = (uint32_t)h.cmb_mid.size();
+  bp.n_big = (uint32_t)h.cmb_big.size();
   bp.nseg = h.nseg;
   bp.npieces = h.npieces;
   bp.nunits = h.nunits;
@@ -313,6 +316,8 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));
   OLV_TRY(upload_vec(&bp.cmb_rp, h.cmb_rp));
   OLV_TRY(upload_vec(&bp.cmb_pos, h.cmb_pos));
+  OLV_TRY(upload_vec(&bp.cmb_mid, h.cmb_mid));
+  OLV_TRY(upload_vec(&bp.cmb_big, h.cmb_big));
   OLV_HIP(hipMalloc(reinterpret_cast<void **>(&bp.partial), std::max<size_t>(h.npieces, 1) * sizeof(float)));
   bp.ready = true;
   return OLV_OK;

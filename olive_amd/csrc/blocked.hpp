@@ -16,12 +16,12 @@
 //                 coalesced loads, the unit's entries and piece offsets in
 //                 LDS (and the segment of the source vector, when the
 //                 segment changes), then sums every piece from LDS alone --
-//                 1, 8 or 64 lanes per piece by length -- and stores one
+//                 1 to 64 lanes per piece by length -- and stores one
 //                 partial per piece, in piece order (coalesced stores);
 //   combine       per destination, its partials (listed by cmb_pos in
-//                 segment order) are added in a fixed order -- by one lane,
-//                 or by a wave for the rows with more than kCombineLane
-//                 partials -- and the vertex update runs.
+//                 segment order) are added in a fixed order -- by one lane
+//                 for up to kCmbLight partials, 16 lanes up to kCmbMid, a
+//                 wave beyond -- and the vertex update runs.
 //
 // A piece's terms are summed in a fixed order and the partials in segment
 // order, so an iteration is bit-reproducible from run to run.  With the
@@ -48,12 +48,13 @@ constexpr uint32_t kUnitEdges = 32768;         // entries per work unit, at most
 constexpr uint32_t kUnitEdgesW = 8192;         // with per-entry weights (16 + 32 KB staged)
 constexpr uint32_t kNoSlot = 0xffffffffu;      // lslot of the alignment gaps
 constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
-constexpr uint32_t kCombineLane = 32;          // rows with more partials are combined by a wave
+constexpr uint32_t kCmbLight = 16;             // partials of a row combined by one lane, at most
+constexpr uint32_t kCmbMid = 128;              // by 16 lanes (8 each), at most; a wave beyond
 
 // Device arrays of a segmented pull over one CSR.
 struct BlockedPlan {
   uint32_t n = 0, m = 0, n_live = 0;  // vertices, entries, rows with entries (a prefix of the rows)
-  uint32_t n_heavy = 0;               // rows [0, n_heavy) hold every row with > kCombineLane partials
+  uint32_t n_mid = 0, n_big = 0;      // rows with kCmbLight < partials <= kCmbMid, and more
   uint32_t nseg = 0, npieces = 0, nunits = 0, ngroups = 0;
   uint32_t unit_edges = 0, nent = 0;  // entries per unit at most; entries with gaps and spares
   uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_units
@@ -69,14 +70,16 @@ struct BlockedPlan {
                                  // (kNoSlot in the gaps and spares); its owner may map it in place
   uint32_t *cmb_rp = nullptr;    // n + 1: each row's range in cmb_pos
   uint32_t *cmb_pos = nullptr;   // npieces: partial index of each row's pieces, in segment order
+  uint32_t *cmb_mid = nullptr;   // n_mid: the rows with kCmbLight < partials <= kCmbMid, ascending
+  uint32_t *cmb_big = nullptr;   // n_big: the rows with more than kCmbMid partials, ascending
   float *partial = nullptr;      // npieces
   bool ready = false;
   void release() {
-    void *ps[] = {wg_ptr, wg_units, pbeg, lidx, lslot, cmb_rp, cmb_pos, partial};
+    void *ps[] = {wg_ptr, wg_units, pbeg, lidx, lslot, cmb_rp, cmb_pos, cmb_mid, cmb_big, partial};
     for (void *p : ps)
       if (p) (void)hipFree(p);
     wg_units = nullptr;
-    wg_ptr = lslot = cmb_rp = cmb_pos = nullptr;
+    wg_ptr = lslot = cmb_rp = cmb_pos = cmb_mid = cmb_big = nullptr;
     pbeg = lidx = nullptr;
     partial = nullptr;
     ready = false;
@@ -87,9 +90,9 @@ struct BlockedPlan {
 // (segment, bucket, first piece, first entry), the last a sentinel (-, -,
 // npieces, entries); wg_list: the units of each workgroup.
 struct BlockedHost {
-  uint32_t n = 0, m = 0, n_live = 0, n_heavy = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0, nent = 0;
+  uint32_t n = 0, m = 0, n_live = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0, nent = 0;
   std::vector<uint4> units, wg_units;
-  std::vector<uint32_t> wg_ptr, wg_list, lslot, cmb_rp, cmb_pos;
+  std::vector<uint32_t> wg_ptr, wg_list, lslot, cmb_rp, cmb_pos, cmb_mid, cmb_big;
   std::vector<uint16_t> pbeg, lidx;
 };
 int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, bool with_slots, uint32_t groups,
@@ -112,13 +115,15 @@ struct BlkDev {
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
   const uint32_t *cmb_pos;
+  const uint32_t *cmb_mid, *cmb_big;
   float *partial;
   const float *wpiece;  // per-entry weights in the plan's entry order (weighted passes)
-  uint32_t n, n_live, n_heavy;
+  uint32_t n, n_live, n_mid, n_big;
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p, const float *wpiece = nullptr) {
-  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.partial, wpiece, p.n, p.n_live, p.n_heavy};
+  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg,    p.lidx, p.cmb_rp, p.cmb_pos, p.cmb_mid,
+                p.cmb_big, p.partial,  wpiece, p.n,    p.n_live, p.n_mid,  p.n_big};
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the
@@ -361,46 +366,77 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
 // Combine of iteration k: row v's partials added in a fixed order, then
 // fin.finish(v, sum, err, fin.pre(v)) (pre: the row's other inputs, loaded
 // ahead); the residual goes to errs[k % 3] (ctl != nullptr).
-// Rows [0, n_heavy) go a wave each (lane j adds partials j, j + 64, ...,
-// then a shuffle tree), in the first `heavy_blocks` workgroups; the others a
-// lane each, in segment order, in the remaining workgroups.  Rows without
-// entries (v >= n_live) get a constant update: iterations 0 and 1 write it
-// into both ping-pong buffers, later iterations skip them.
+// Rows by their number of partials: more than kCmbMid (the list cmb_big)
+// a wave each -- lane j adds partials j, j + 64, ... (four loads in flight),
+// then a DPP sum -- in the first big_blocks workgroups; kCmbLight + 1 to
+// kCmbMid (cmb_mid) 16 lanes each -- lane j adds partials j, j + 16, ...,
+// j + 112, all in flight, then a DPP sum -- in the next mid_blocks
+// workgroups; every other row a lane each, up to kCmbLight loads in flight
+// added in segment order, 256 consecutive rows per workgroup in the rest.
+// Rows without entries (v >= n_live) get a constant update: iterations 0
+// and 1 write it into both ping-pong buffers, later iterations skip them.
 template <class Fin>
 __global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, IterCtl *ctl, uint32_t k,
-                                                           uint32_t heavy_blocks) {
+                                                           uint32_t big_blocks, uint32_t mid_blocks) {
   if (ctl && __hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
   __shared__ float s_red[256 / kWave];
   const uint32_t nv = (!ctl || k >= 2) ? d.n_live : d.n;
   float err = 0.f;
   const uint32_t lane = threadIdx.x % kWave;
-  if (blockIdx.x < heavy_blocks) {
-    for (uint32_t v = (blockIdx.x * 256 + threadIdx.x) / kWave; v < d.n_heavy; v += heavy_blocks * (256 / kWave)) {
+  if (blockIdx.x < big_blocks) {
+    for (uint32_t i = (blockIdx.x * 256 + threadIdx.x) / kWave; i < d.n_big; i += big_blocks * (256 / kWave)) {
+      const uint32_t v = d.cmb_big[i];  // the same for the whole wave
       const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
       float s = 0.f;
-      for (uint32_t q = a + lane; q < b; q += kWave) s += d.partial[d.cmb_pos[q]];
-      s = wave_sum(s);
+      for (uint32_t q0 = a + lane; q0 < b; q0 += 4 * kWave) {
+        uint32_t ix[4];
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) ix[t] = q0 + t * kWave < b ? d.cmb_pos[q0 + t * kWave] : 0u;
+        float p[4];
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) p[t] = q0 + t * kWave < b ? d.partial[ix[t]] : 0.f;
+#pragma unroll
+        for (uint32_t t = 0; t < 4; ++t) s += p[t];
+      }
+      s = blk_group_sum<kWave>(s);
       if (lane == 0) fin.finish(v, s, err, fin.pre(v));
     }
-  }
-  const uint32_t lb = blockIdx.x - heavy_blocks, nlb = gridDim.x - heavy_blocks;
-  for (uint32_t v = d.n_heavy + lb * 256 + threadIdx.x; blockIdx.x >= heavy_blocks && v < nv; v += nlb * 256) {
-    const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
-    const auto pre = fin.pre(v);
-    // sixteen loads in flight, added in segment order
-    float s = 0.f;
-    for (uint32_t q = a; q < b; q += 16) {
-      uint32_t ix[16];
+  } else if (blockIdx.x < big_blocks + mid_blocks) {
+    constexpr uint32_t G = 16, kPer = kCmbMid / G;
+    const uint32_t j = threadIdx.x % G;
+    for (uint32_t i = ((blockIdx.x - big_blocks) * 256 + threadIdx.x) / G; i < d.n_mid; i += mid_blocks * (256 / G)) {
+      const uint32_t v = d.cmb_mid[i];  // the same for the group's 16 lanes (one DPP row)
+      const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+      uint32_t ix[kPer];
 #pragma unroll
-      for (uint32_t i = 0; i < 16; ++i) ix[i] = q + i < b ? d.cmb_pos[q + i] : 0u;
-      float p[16];
+      for (uint32_t t = 0; t < kPer; ++t) ix[t] = a + j + t * G < b ? d.cmb_pos[a + j + t * G] : 0u;
+      float p[kPer];
 #pragma unroll
-      for (uint32_t i = 0; i < 16; ++i) p[i] = q + i < b ? d.partial[ix[i]] : 0.f;
+      for (uint32_t t = 0; t < kPer; ++t) p[t] = a + j + t * G < b ? d.partial[ix[t]] : 0.f;
+      float s = 0.f;
 #pragma unroll
-      for (uint32_t i = 0; i < 16; ++i)
-        if (q + i < b) s += p[i];
+      for (uint32_t t = 0; t < kPer; ++t) s += p[t];
+      s = blk_group_sum<G>(s);
+      if (j == 0) fin.finish(v, s, err, fin.pre(v));
     }
-    fin.finish(v, s, err, pre);
+  } else {
+    const uint32_t lb = blockIdx.x - big_blocks - mid_blocks, nlb = gridDim.x - big_blocks - mid_blocks;
+    for (uint32_t v = lb * 256 + threadIdx.x; v < nv; v += nlb * 256) {
+      const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
+      if (b - a > kCmbLight) continue;  // a listed row
+      const auto pre = fin.pre(v);
+      uint32_t ix[kCmbLight];
+#pragma unroll
+      for (uint32_t t = 0; t < kCmbLight; ++t) ix[t] = a + t < b ? d.cmb_pos[a + t] : 0u;
+      float p[kCmbLight];
+#pragma unroll
+      for (uint32_t t = 0; t < kCmbLight; ++t) p[t] = a + t < b ? d.partial[ix[t]] : 0.f;
+      float s = 0.f;
+#pragma unroll
+      for (uint32_t t = 0; t < kCmbLight; ++t)
+        if (a + t < b) s += p[t];
+      fin.finish(v, s, err, pre);
+    }
   }
   if (!ctl) return;
   err = block_sum<256, float>(err, s_red);

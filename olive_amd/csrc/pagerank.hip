@@ -569,14 +569,17 @@ static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, 
 template <class SV, bool kW, class Fin>
 static int enqueue_blocked(const BlockedPlan &bp, const BlkDev &bd, const float *src, const Fin &fin, PrWork *w,
                            uint32_t k, KernelProfile *prof = nullptr) {
-  // combine: a wave per heavy row in the first workgroups, then a lane per row
-  const unsigned gh = std::min<unsigned>((bp.n_heavy + 3) / 4, (unsigned)cu_count() * 4);
-  const unsigned gc = gh + ew_grid(bp.n);
+  // combine: a wave per listed big row, 16 lanes per listed mid row, then a
+  // lane per row
+  const unsigned cu = (unsigned)cu_count();
+  const unsigned gb = std::min<unsigned>((bp.n_big + 3) / 4, cu * 4);
+  const unsigned gm = std::min<unsigned>((bp.n_mid + 15) / 16, cu * 16);
+  const unsigned gc = gb + gm + ew_grid(bp.n);
   hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
   if (a) OLV_HIP(hipEventRecord(a, stream()));
   hipLaunchKernelGGL((blk_segment_kernel<SV, kW>), dim3(bp.ngroups), dim3(kBlkThreads), 0, stream(), bd, SV{}, src,
                      w->ctl, k);
-  hipLaunchKernelGGL(blk_combine_kernel<Fin>, dim3(gc), dim3(256), 0, stream(), bd, fin, w->ctl, k, gh);
+  hipLaunchKernelGGL(blk_combine_kernel<Fin>, dim3(gc), dim3(256), 0, stream(), bd, fin, w->ctl, k, gb, gm);
   if (b) OLV_HIP(hipEventRecord(b, stream()));
   OLV_HIP(hipGetLastError());
   return OLV_OK;
