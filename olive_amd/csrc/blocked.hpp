@@ -20,7 +20,7 @@
 //                 partial per piece, in piece order (coalesced stores);
 //   combine       per destination, its partials (listed by cmb_pos in
 //                 segment order) are added in a fixed order -- by one lane
-//                 for up to kCmbLight partials, 16 lanes up to kCmbMid, a
+//                 for up to kCmbLight partials, 8 lanes up to kCmbMid, a
 //                 wave beyond -- and the vertex update runs.
 //
 // A piece's terms are summed in a fixed order and the partials in segment
@@ -49,7 +49,8 @@ constexpr uint32_t kUnitEdgesW = 8192;         // with per-entry weights (16 + 3
 constexpr uint32_t kNoSlot = 0xffffffffu;      // lslot of the alignment gaps
 constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
 constexpr uint32_t kCmbLight = 16;             // partials of a row combined by one lane, at most
-constexpr uint32_t kCmbMid = 128;              // by 16 lanes (8 each), at most; a wave beyond
+constexpr uint32_t kCmbMid = 128;              // by kCmbMidLanes lanes, at most; a wave beyond
+constexpr uint32_t kCmbMidLanes = 8;           // (16 partials each, all in flight)
 
 // Device arrays of a segmented pull over one CSR.
 struct BlockedPlan {
@@ -369,8 +370,8 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
 // Rows by their number of partials: more than kCmbMid (the list cmb_big)
 // a wave each -- lane j adds partials j, j + 64, ... (four loads in flight),
 // then a DPP sum -- in the first big_blocks workgroups; kCmbLight + 1 to
-// kCmbMid (cmb_mid) 16 lanes each -- lane j adds partials j, j + 16, ...,
-// j + 112, all in flight, then a DPP sum -- in the next mid_blocks
+// kCmbMid (cmb_mid) kCmbMidLanes = 8 lanes each -- lane j adds partials j,
+// j + 8, ..., j + 120, all in flight, then a DPP sum -- in the next mid_blocks
 // workgroups; every other row a lane each, up to kCmbLight loads in flight
 // added in segment order, 256 consecutive rows per workgroup in the rest.
 // Rows without entries (v >= n_live) get a constant update: iterations 0
@@ -402,10 +403,10 @@ __global__ void __launch_bounds__(256) blk_combine_kernel(BlkDev d, Fin fin, Ite
       if (lane == 0) fin.finish(v, s, err, fin.pre(v));
     }
   } else if (blockIdx.x < big_blocks + mid_blocks) {
-    constexpr uint32_t G = 16, kPer = kCmbMid / G;
+    constexpr uint32_t G = kCmbMidLanes, kPer = kCmbMid / G;
     const uint32_t j = threadIdx.x % G;
     for (uint32_t i = ((blockIdx.x - big_blocks) * 256 + threadIdx.x) / G; i < d.n_mid; i += mid_blocks * (256 / G)) {
-      const uint32_t v = d.cmb_mid[i];  // the same for the group's 16 lanes (one DPP row)
+      const uint32_t v = d.cmb_mid[i];  // the same for the group's lanes (inside one DPP row)
       const uint32_t a = d.cmb_rp[v], b = d.cmb_rp[v + 1];
       uint32_t ix[kPer];
 #pragma unroll

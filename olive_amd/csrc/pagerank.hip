@@ -569,11 +569,11 @@ static int enqueue_iteration(const SegDev &d, const uint32_t *rp, const Op &op, 
 template <class SV, bool kW, class Fin>
 static int enqueue_blocked(const BlockedPlan &bp, const BlkDev &bd, const float *src, const Fin &fin, PrWork *w,
                            uint32_t k, KernelProfile *prof = nullptr) {
-  // combine: a wave per listed big row, 16 lanes per listed mid row, then a
-  // lane per row
+  // combine: a wave per listed big row, kCmbMidLanes lanes per listed mid
+  // row, then a lane per row
   const unsigned cu = (unsigned)cu_count();
   const unsigned gb = std::min<unsigned>((bp.n_big + 3) / 4, cu * 4);
-  const unsigned gm = std::min<unsigned>((bp.n_mid + 15) / 16, cu * 16);
+  const unsigned gm = std::min<unsigned>((bp.n_mid + 256 / kCmbMidLanes - 1) / (256 / kCmbMidLanes), cu * 16);
   const unsigned gc = gb + gm + ew_grid(bp.n);
   hipEvent_t a = prof ? prof->next() : nullptr, b = prof ? prof->next() : nullptr;
   if (a) OLV_HIP(hipEventRecord(a, stream()));
