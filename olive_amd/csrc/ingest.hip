@@ -606,9 +606,66 @@ struct ByteReader {
   }
 };
 
+// Bytes of one line read front to back: the 8 words (32 bytes) from the
+// line's first word are requested together when the reader is made; each
+// time the position passes a word, the window shifts by one and the word 8
+// further on is requested, so no byte waits on the load of the one before.
+// Positions must not decrease between reads.  Reads stay within the text
+// buffer's zero padding of at least 64 bytes past its end.
+struct LineReader {
+  static constexpr int kW = 8;
+  const uint32_t *w;
+  size_t cw;  // word index of wd[0]
+  uint32_t wd[kW];
+  __device__ LineReader(const uint8_t *t, size_t b) : w(reinterpret_cast<const uint32_t *>(t)), cw(b >> 2) {
+#pragma unroll
+    for (int i = 0; i < kW; ++i) wd[i] = w[cw + i];
+  }
+  __device__ __forceinline__ uint8_t operator[](size_t p) {
+    while ((p >> 2) > cw) {
+#pragma unroll
+      for (int i = 0; i < kW - 1; ++i) wd[i] = wd[i + 1];
+      wd[kW - 1] = w[cw + kW];
+      ++cw;
+    }
+    return (uint8_t)(wd[0] >> (8 * (p & 3)));
+  }
+};
+
+// is_space for a byte inside a line (never '\n'): ' ' or '\t' '\v' '\f' '\r'
+__device__ __forceinline__ bool is_space_in_line(uint8_t c) { return c == ' ' || (uint8_t)(c - '\t') <= 4u; }
+
+// parse_int for a line read by a LineReader: the same tokens and values,
+// with the first 9 digits accumulated in 32 bits.
+__device__ __forceinline__ bool parse_int_line(LineReader &t, size_t &p, size_t e, long long &val) {
+  bool neg = false;
+  uint8_t ch = t[p];
+  if (ch == '+' || ch == '-') {
+    neg = ch == '-';
+    ++p;
+  }
+  uint32_t lo = 0;
+  long long v = 0;
+  int nd = 0;
+  bool ok = true;
+  for (; p < e; ++p) {
+    ch = t[p];
+    if (is_space_in_line(ch)) break;
+    const uint32_t dgt = (uint32_t)ch - '0';
+    if (dgt > 9u || nd >= 18) ok = false;
+    else if (nd < 9) lo = lo * 10u + dgt;
+    else v = (nd == 9 ? (long long)lo : v) * 10 + dgt;
+    ++nd;
+  }
+  const long long x = nd <= 9 ? (long long)lo : v;
+  val = neg ? -x : x;
+  return ok && nd > 0;
+}
+
 // Parse one token starting at t[p] (not whitespace, p < e) as an integer:
 // [+-]digits, at most 18 digits.  Advances p past the token.
-__device__ __forceinline__ bool parse_int(ByteReader &t, size_t &p, size_t e, long long &val) {
+template <class Reader>
+__device__ __forceinline__ bool parse_int(Reader &t, size_t &p, size_t e, long long &val) {
   bool neg = false;
   if (t[p] == '+' || t[p] == '-') {
     neg = t[p] == '-';
@@ -696,15 +753,15 @@ __global__ void __launch_bounds__(kIB) edge_parse_kernel(EdgeParse a) {
     const uint32_t di = a.didx[k];
     if (a.didx[k + 1] == di || di == 0 || di > a.m) continue;  // not content, header, or beyond m
     size_t b = a.starts[k], e = a.starts[k + 1] - 1;
-    ByteReader t(a.t);
+    LineReader t(a.t, b);
     long long v[3] = {0, 0, 0};
     uint32_t ntok = 0;
     bool ok = true;
     for (;;) {
-      while (b < e && is_space(t[b])) ++b;
+      while (b < e && is_space_in_line(t[b])) ++b;
       if (b >= e) break;
       long long x;
-      const bool okt = parse_int(t, b, e, x);
+      const bool okt = parse_int_line(t, b, e, x);
       if (ntok < 3) {  // columns after the third are not read
         ok = ok && okt;
         v[ntok] = x;
