@@ -28,7 +28,7 @@
 // vertices numbered by descending in-degree (the renumbered CSRs of the
 // PageRank solver) a skewed graph's in-edges fall into few pieces: RMAT-20
 // has 16.8 M in-edges in 2.3 M pieces, most of them in segment 0.  Per
-// iteration the pass reads 2 B per edge and 4 B per piece and writes 4 B per
+// iteration the pass reads 2 B per edge and 2 B per piece and writes 4 B per
 // piece, and every gather is an LDS read.
 #pragma once
 #include <vector>
@@ -42,12 +42,12 @@ namespace olv {
 constexpr uint32_t kSegBits = 14;
 constexpr uint32_t kSegSize = 1u << kSegBits;  // sources per LDS slice (64 KB of fp32), plus a zero slot
 static_assert(kSegSize <= 0xffff, "entries hold a slice offset or the zero slot in 16 bits");
-constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 145 KB of LDS)
+constexpr int kBlkThreads = 1024;              // segment-pass workgroup (one per CU: 130-144 KB of LDS)
 constexpr uint32_t kPieceMax = 512;            // longest piece (one wave step); longer ones are cut into chunks
 constexpr uint32_t kUnitEdges = 32768;         // entries per work unit, at most (64 KB staged)
 constexpr uint32_t kUnitEdgesW = 8192;         // with per-entry weights (16 + 32 KB staged)
 constexpr uint32_t kNoSlot = 0xffffffffu;      // lslot of the alignment gaps
-constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (32 KB staged)
+constexpr uint32_t kUnitPieces = 8192;         // pieces per work unit, at most (16 KB staged)
 constexpr uint32_t kCmbLight = 16;             // partials of a row combined by one lane, at most
 constexpr uint32_t kCmbMid = 128;              // by kCmbMidLanes lanes, at most; a wave beyond
 constexpr uint32_t kCmbMidLanes = 8;           // (16 partials each, all in flight)
