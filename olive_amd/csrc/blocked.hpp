@@ -123,8 +123,8 @@ struct BlkDev {
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p, const float *wpiece = nullptr) {
-  return BlkDev{p.wg_ptr, p.wg_units, p.pbeg,    p.lidx, p.cmb_rp, p.cmb_pos, p.cmb_mid,
-                p.cmb_big, p.partial,  wpiece, p.n,    p.n_live, p.n_mid,  p.n_big};
+  return BlkDev{p.wg_ptr,  p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.cmb_mid,
+                p.cmb_big, p.partial,  wpiece, p.n,    p.n_live, p.n_mid,   p.n_big};
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the

@@ -492,8 +492,8 @@ __global__ void piece_weight_kernel(const float *__restrict__ w, const uint32_t 
 
 // The weights w in the entry order of the weighted segmented pull of side 0
 // (in-CSR) or 1 (out-CSR), cached like slot_weights' copies.
-static int piece_weights(const olv_graph *g, PrWork *pw, int side, const float *w, const uint32_t *eid, bool reuse,
-                         bool staged, const float **out) {
+static int piece_weights(PrWork *pw, int side, const float *w, const uint32_t *eid, bool reuse, bool staged,
+                         const float **out) {
   const BlockedPlan &bp = pw->rl.blk_w[side];
   if (!pw->wpiece[side]) OLV_HIP(hipMalloc(reinterpret_cast<void **>(&pw->wpiece[side]), (size_t)bp.nent * 4));
   if (!(reuse && !staged && pw->wpiece_src[side] == w && pw->wpiece_eid[side] == eid)) {
@@ -505,7 +505,6 @@ static int piece_weights(const olv_graph *g, PrWork *pw, int side, const float *
     if (!reuse && pw->wpiece_src[side ^ 1] == w) pw->wpiece_src[side ^ 1] = nullptr;
   }
   *out = pw->wpiece[side];
-  (void)g;
   return OLV_OK;
 }
 
@@ -797,7 +796,7 @@ int olv_pagerank_weighted_f32(const olv_graph *cg, const float *wts, const float
     tt = tb.as<float>();
   }
   const float *ws = nullptr, *wp = nullptr;
-  if (blocked) OLV_TRY(piece_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+  if (blocked) OLV_TRY(piece_weights(w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
   else OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
   OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, tt, damping, tol, max_iters, t, &res,
@@ -874,7 +873,7 @@ int olv_pagerank_jvp_f32(const olv_graph *cg, const float *wts, const float *ran
   hipLaunchKernelGGL(scale_kernel, dim3(ew_grid(g->n)), dim3(256), 0, stream(), w->buf[2], 1.0f / (1.0f - damping),
                      g->n, w->buf[3]);
   const float *ws = nullptr, *wp = nullptr;
-  if (blocked) OLV_TRY(piece_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+  if (blocked) OLV_TRY(piece_weights(w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
   else OLV_TRY(slot_weights(g, w, 0, dw, pv.eid, (flags & OLV_W_UNCHANGED) != 0, host, &ws));
   IterResult res;
   OLV_TRY(weighted_solve(g, pv.d, pv.rp, pv.ci, w, ws, w->buf[3], damping, tol, max_iters, t, &res,
@@ -941,7 +940,7 @@ int olv_pagerank_vjp_f32(const olv_graph *cg, const float *wts, const float *ran
   t.start();
   const float *wv = dw, *rv = dr, *gv = dg, *wp = nullptr;
   if (relabel) {
-    if (blocked) OLV_TRY(piece_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
+    if (blocked) OLV_TRY(piece_weights(w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wp));
     else OLV_TRY(slot_weights(g, w, 1, dw, w->rl.out.eid, (flags & OLV_W_UNCHANGED) != 0, host, &wv));
     gather_f32(dr, g->rl_order, g->n, rb.as<float>());
     gather_f32(dg, g->rl_order, g->n, gb.as<float>());
