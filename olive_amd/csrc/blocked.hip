@@ -285,6 +285,9 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
     const uint4 a = units[h.wg_list[i]], b = units[h.wg_list[i] + 1];
     h.wg_units[i] = make_uint4(a.x << 3 | a.y, a.z, a.w, (b.z - a.z) | (b.w - a.w) << 16);
   }
+  h.wg_first.assign(ng, make_uint4(0, 0, 0, 0));
+  for (uint32_t w = 0; w < ng; ++w)
+    if (h.wg_ptr[w] < h.wg_ptr[w + 1]) h.wg_first[w] = h.wg_units[h.wg_ptr[w]];
 
   h.npieces = (uint32_t)npieces;
   h.nent = o + 8;
@@ -311,6 +314,7 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   bp.ngroups = h.ngroups;
   OLV_TRY(upload_vec(&bp.wg_ptr, h.wg_ptr));
   OLV_TRY(upload_vec(&bp.wg_units, h.wg_units));
+  OLV_TRY(upload_vec(&bp.wg_first, h.wg_first));
   OLV_TRY(upload_vec(&bp.pbeg, h.pbeg));
   OLV_TRY(upload_vec(&bp.lidx, h.lidx));
   if (with_slots) OLV_TRY(upload_vec(&bp.lslot, h.lslot));

@@ -62,6 +62,7 @@ struct BlockedPlan {
   uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
                                  // (segment << 3 | bucket, first piece, first entry,
                                  // pieces | entries << 16); bucket 0..6 = 1/2/4/.../64 lanes per piece
+  uint4 *wg_first = nullptr;     // ngroups: each workgroup's first unit (zeros if it has none)
   uint16_t *pbeg = nullptr;      // npieces: each piece's first entry, counted from its unit's first
                                  // entry (a unit's last piece ends at the unit's aligned entry count)
   uint16_t *lidx = nullptr;      // source offset within the segment, in piece order; each unit starts
@@ -76,10 +77,10 @@ struct BlockedPlan {
   float *partial = nullptr;      // npieces
   bool ready = false;
   void release() {
-    void *ps[] = {wg_ptr, wg_units, pbeg, lidx, lslot, cmb_rp, cmb_pos, cmb_mid, cmb_big, partial};
+    void *ps[] = {wg_ptr, wg_units, wg_first, pbeg, lidx, lslot, cmb_rp, cmb_pos, cmb_mid, cmb_big, partial};
     for (void *p : ps)
       if (p) (void)hipFree(p);
-    wg_units = nullptr;
+    wg_units = wg_first = nullptr;
     wg_ptr = lslot = cmb_rp = cmb_pos = cmb_mid = cmb_big = nullptr;
     pbeg = lidx = nullptr;
     partial = nullptr;
@@ -92,7 +93,7 @@ struct BlockedPlan {
 // npieces, entries); wg_list: the units of each workgroup.
 struct BlockedHost {
   uint32_t n = 0, m = 0, n_live = 0, nseg = 0, npieces = 0, nunits = 0, ngroups = 0, nent = 0;
-  std::vector<uint4> units, wg_units;
+  std::vector<uint4> units, wg_units, wg_first;
   std::vector<uint32_t> wg_ptr, wg_list, lslot, cmb_rp, cmb_pos, cmb_mid, cmb_big;
   std::vector<uint16_t> pbeg, lidx;
 };
@@ -112,6 +113,7 @@ int build_blocked_plan(const std::vector<uint32_t> &rp, const uint32_t *col, boo
 struct BlkDev {
   const uint32_t *wg_ptr;
   const uint4 *wg_units;
+  const uint4 *wg_first;
   const uint16_t *pbeg;
   const uint16_t *lidx;
   const uint32_t *cmb_rp;
@@ -123,8 +125,8 @@ struct BlkDev {
 };
 
 inline BlkDev make_blk_dev(const BlockedPlan &p, const float *wpiece = nullptr) {
-  return BlkDev{p.wg_ptr,  p.wg_units, p.pbeg, p.lidx, p.cmb_rp, p.cmb_pos, p.cmb_mid,
-                p.cmb_big, p.partial,  wpiece, p.n,    p.n_live, p.n_mid,   p.n_big};
+  return BlkDev{p.wg_ptr,  p.wg_units, p.wg_first, p.pbeg,  p.lidx, p.cmb_rp, p.cmb_pos, p.cmb_mid,
+                p.cmb_big, p.partial,  wpiece,     p.n,     p.n_live, p.n_mid, p.n_big};
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the
@@ -221,7 +223,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   __shared__ uint16_t s_pbeg[kUnitPieces + 2];  // piece starts and the unit's end
   const uint32_t tid = threadIdx.x;
   const uint32_t u0 = d.wg_ptr[blockIdx.x], u1 = d.wg_ptr[blockIdx.x + 1];
-  const uint4 first = u0 < u1 ? d.wg_units[u0] : make_uint4(0, 0, 0, 0);  // in flight during the checks
+  const uint4 first = d.wg_first[blockIdx.x];  // in flight with wg_ptr and during the checks
   if (ctl) {
     if (__hip_atomic_load(&ctl->done, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) return;
     if (blockIdx.x == 0) {
