@@ -44,16 +44,20 @@ int upload_vec(T **dst, const std::vector<T> &src) {
   return OLV_OK;
 }
 
-// Lanes per piece by length: at most eight entries per lane (1, 2, 4, ...,
-// 32 lanes up to 8, 16, 32, ..., 256 entries; a wave up to kPieceMax).
-constexpr uint32_t kBucketLanes[7] = {1, 2, 4, 8, 16, 32, 64};
-constexpr int kBuckets = 7;
-static_assert(kPieceMax <= kStepReads * kBucketLanes[kBuckets - 1], "the widest bucket covers a piece");
-// the fewest lanes that cover the piece with kStepReads reads each (the
-// segment pass sums every piece in one step)
+// Lanes and reads per piece by length: one lane with 2, 4 or 8 reads up to
+// 2, 4, 8 entries, then 8 reads on 2, 4, ..., 32 lanes up to 16, 32, ...,
+// 256 entries, and a wave up to kPieceMax.
+constexpr uint32_t kBucketLanes[9] = {1, 1, 1, 2, 4, 8, 16, 32, 64};
+constexpr uint32_t kBucketReads[9] = {2, 4, 8, 8, 8, 8, 8, 8, 8};
+constexpr int kBuckets = 9;
+static_assert(kBuckets <= (1 << kBucketBits), "a unit's bucket fits its descriptor");
+static_assert(kPieceMax <= kBucketReads[kBuckets - 1] * kBucketLanes[kBuckets - 1], "the widest bucket covers a piece");
+// the first bucket whose lanes and reads cover the piece (the segment pass
+// sums every piece in one step): the fewest reads for short pieces, then the
+// fewest lanes
 inline uint32_t piece_bucket(uint32_t len) {
   uint32_t b = 0;
-  while (b + 1 < kBuckets && len > kStepReads * kBucketLanes[b]) ++b;
+  while (b + 1 < kBuckets && len > kBucketReads[b] * kBucketLanes[b]) ++b;
   return b;
 }
 
@@ -191,7 +195,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
         // on MI355X: staging and the barriers around it ~1.3, then
         // kStepUs[b] per step (one piece per group of G lanes) and 0.04 per
         // thousand entries
-        constexpr double kStepUs[kBuckets] = {0.41, 0.20, 0.29, 0.37, 0.38, 0.41, 0.37};
+        constexpr double kStepUs[kBuckets] = {0.15, 0.25, 0.41, 0.20, 0.29, 0.37, 0.38, 0.41, 0.37};
         const uint32_t per_step = kBlkThreads / kBucketLanes[b];
         const uint32_t steps = (pieces + per_step - 1) / per_step;
         unit_cost.push_back(1.3 + kStepUs[b] * steps + 4e-5 * edges);
@@ -283,7 +287,7 @@ int build_blocked_host(const std::vector<uint32_t> &rp, const uint32_t *col, boo
   h.wg_units.resize(h.wg_list.size());
   for (size_t i = 0; i < h.wg_list.size(); ++i) {
     const uint4 a = units[h.wg_list[i]], b = units[h.wg_list[i] + 1];
-    h.wg_units[i] = make_uint4(a.x << 3 | a.y, a.z, a.w, (b.z - a.z) | (b.w - a.w) << 16);
+    h.wg_units[i] = make_uint4(a.x << kBucketBits | a.y, a.z, a.w, (b.z - a.z) | (b.w - a.w) << 16);
   }
   h.wg_first.assign(ng, make_uint4(0, 0, 0, 0));
   for (uint32_t w = 0; w < ng; ++w)

@@ -60,8 +60,8 @@ struct BlockedPlan {
   uint32_t unit_edges = 0, nent = 0;  // entries per unit at most; entries with gaps and spares
   uint32_t *wg_ptr = nullptr;    // ngroups + 1: each segment-pass workgroup's range in wg_units
   uint4 *wg_units = nullptr;     // nunits: the units of each workgroup in segment order, packed as
-                                 // (segment << 3 | bucket, first piece, first entry,
-                                 // pieces | entries << 16); bucket 0..6 = 1/2/4/.../64 lanes per piece
+                                 // (segment << 4 | bucket, first piece, first entry,
+                                 // pieces | entries << 16); bucket 0..8: lanes and reads per piece
   uint4 *wg_first = nullptr;     // ngroups: each workgroup's first unit (zeros if it has none)
   uint16_t *pbeg = nullptr;      // npieces: each piece's first entry, counted from its unit's first
                                  // entry (a unit's last piece ends at the unit's aligned entry count)
@@ -130,28 +130,28 @@ inline BlkDev make_blk_dev(const BlockedPlan &p, const float *wpiece = nullptr) 
 }
 
 // Lane j's share of a piece of a unit whose pieces take G lanes each: the
-// sum of sv(lidx[k]) over k = beg + j + i G, i < kStepReads (lidx and the
-// slice in LDS), all reads in flight at once and the terms added pairwise.
-// Such a piece has at most kStepReads G entries (the length buckets), so
-// this covers it; the k past its end (up to `end`, the unit's alignment
-// padding, or beyond) read the slice's zero slot (kSegSize) instead of
-// branching.
-constexpr int kStepReads = 8;
+// sum of sv(lidx[k]) over k = beg + j + i G, i < U (lidx and the slice in
+// LDS), all reads in flight at once and the terms added pairwise.  Such a
+// piece has at most U G entries (the length buckets), so this covers it;
+// the k past its end (up to `end`, the unit's alignment padding, or beyond)
+// read the slice's zero slot (kSegSize) instead of branching.
+constexpr int kStepReads = 8;  // reads per lane, at most
+constexpr uint32_t kBucketBits = 4;
 static_assert(kPieceMax <= kStepReads * kWave, "a piece fits one step of a wave");
-template <int G, class SV>
+template <int G, int U, class SV>
 __device__ __forceinline__ float piece_step(const SV &sv, const uint16_t *lidx, uint32_t beg, uint32_t end,
                                             uint32_t j) {
-  uint32_t ix[kStepReads];
+  uint32_t ix[U];
 #pragma unroll
-  for (int i = 0; i < kStepReads; ++i) {
+  for (int i = 0; i < U; ++i) {
     const uint32_t k = beg + j + (uint32_t)i * G;
     ix[i] = k < end ? (uint32_t)lidx[k] : kSegSize;
   }
-  float acc[kStepReads];
+  float acc[U];
 #pragma unroll
-  for (int i = 0; i < kStepReads; ++i) acc[i] = sv(beg + j + (uint32_t)i * G, ix[i]);
+  for (int i = 0; i < U; ++i) acc[i] = sv(beg + j + (uint32_t)i * G, ix[i]);
 #pragma unroll
-  for (int w = kStepReads / 2; w > 0; w >>= 1)
+  for (int w = U / 2; w > 0; w >>= 1)
 #pragma unroll
     for (int i = 0; i < w; ++i) acc[i] += acc[i + w];
   return acc[0];
@@ -187,14 +187,14 @@ __device__ __forceinline__ float blk_group_sum(float v) {
   return v;
 }
 
-// Pieces of a unit summed G lanes each; lane G - 1 of the group stores the
-// partial.
-template <int G, class SV>
+// Pieces of a unit summed G lanes each, U reads a lane; lane G - 1 of the
+// group stores the partial.
+template <int G, int U, class SV>
 __device__ __forceinline__ void blk_pieces(const BlkDev &d, const SV &sv, const uint16_t *s_lidx,
                                            const uint16_t *s_pbeg, uint32_t np, uint32_t q0) {
   const uint32_t j = threadIdx.x % G;
   for (uint32_t q = threadIdx.x / G; q < np; q += kBlkThreads / G) {
-    const float s = blk_group_sum<G>(piece_step<G>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j));
+    const float s = blk_group_sum<G>(piece_step<G, U>(sv, s_lidx, s_pbeg[q], s_pbeg[q + 1], j));
     if (j == G - 1) d.partial[q0 + q] = s;
   }
 }
@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   uint32_t pb[kPi];
   float4 sl[kSl];
   auto load_unit = [&](const uint4 c, uint32_t have) {
-    const uint32_t np = c.w & 0xffffu, ne = c.w >> 16, seg = c.x >> 3;
+    const uint32_t np = c.w & 0xffffu, ne = c.w >> 16, seg = c.x >> kBucketBits;
     const uint4 *lsrc = reinterpret_cast<const uint4 *>(d.lidx + c.z);
 #pragma unroll
     for (uint32_t j = 0; j < kLi; ++j) {
@@ -306,7 +306,7 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
   uint4 nd = u0 + 1 < u1 ? d.wg_units[u0 + 1] : cd;
   uint32_t cur = 0xffffffffu;
   for (uint32_t u = u0; u < u1; ++u) {
-    const uint32_t seg = cd.x >> 3, bucket = cd.x & 7u, q0 = cd.y;
+    const uint32_t seg = cd.x >> kBucketBits, bucket = cd.x & ((1u << kBucketBits) - 1), q0 = cd.y;
     const uint32_t np = cd.w & 0xffffu, ne = cd.w >> 16;
     __syncthreads();  // the previous unit is done with the staged arrays
     if (seg != cur) {
@@ -339,28 +339,35 @@ __global__ void __launch_bounds__(kBlkThreads) blk_segment_kernel(BlkDev d, SV s
     }
     __syncthreads();
     sv.bind(slice, s_w);
-    // G lanes per piece, at most kStepReads entries per lane
+    // G lanes per piece and U reads per lane by bucket (pieces of up to U G
+    // entries)
     switch (bucket) {
-      case 0:  // up to 8 entries
-        blk_pieces<1>(d, sv, s_lidx, s_pbeg, np, q0);
+      case 0:  // up to 2 entries
+        blk_pieces<1, 2>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
-      case 1:
-        blk_pieces<2>(d, sv, s_lidx, s_pbeg, np, q0);
+      case 1:  // up to 4
+        blk_pieces<1, 4>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
-      case 2:
-        blk_pieces<4>(d, sv, s_lidx, s_pbeg, np, q0);
+      case 2:  // up to 8
+        blk_pieces<1, 8>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       case 3:
-        blk_pieces<8>(d, sv, s_lidx, s_pbeg, np, q0);
+        blk_pieces<2, 8>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       case 4:
-        blk_pieces<16>(d, sv, s_lidx, s_pbeg, np, q0);
+        blk_pieces<4, 8>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       case 5:
-        blk_pieces<32>(d, sv, s_lidx, s_pbeg, np, q0);
+        blk_pieces<8, 8>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 6:
+        blk_pieces<16, 8>(d, sv, s_lidx, s_pbeg, np, q0);
+        break;
+      case 7:
+        blk_pieces<32, 8>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
       default:  // up to kPieceMax
-        blk_pieces<kWave>(d, sv, s_lidx, s_pbeg, np, q0);
+        blk_pieces<kWave, 8>(d, sv, s_lidx, s_pbeg, np, q0);
         break;
     }
   }
